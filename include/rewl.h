@@ -1,0 +1,224 @@
+/*
+ * rewl.h — C ABI of the MI355X replica-exchange Wang-Landau engine.
+ *
+ * The reference (Faikert/replica_exchange_wang_landau) has no FFI layer; its seams are C++ APIs.
+ * Each entry point below replaces one of them and cites it (file:line in the reference tree).
+ * A host C++20 shim (replica_exchange_wang_landau_amd/host) keeps the reference's `run_rewl`
+ * signature and `wl_run` command line on top of this ABI.
+ *
+ * Conventions
+ *   - Every call returns an int status (WLX_OK = 0). Codes map 1:1 to the exception classes the
+ *     reference throws; the message of the last failure on the calling thread is returned by
+ *     wlx_last_error() and reproduces the reference texts where its tests assert them.
+ *   - The caller owns every host array; the library owns device buffers and one HIP stream per
+ *     engine. Calls on one handle are not re-entrant: use one host thread per handle.
+ *   - All structs hold only 8-byte members (no padding), so they bind directly from ctypes/cgo.
+ *   - rng_state[4]: in WLX_RNG_XOSHIRO mode, the Xoshiro256** state words (rng.hpp:10-56), as in
+ *     the WLCHKP3 checkpoint (io.cpp:539-556). In WLX_RNG_PHILOX mode production attempts draw
+ *     Philox4x64-10 blocks keyed (seed, walker_id) at counter (attempt/2, 'WALK'); the stream
+ *     position is the attempted count, and rng_state still carries the Xoshiro words used by
+ *     the target-Metropolis initialisation.
+ */
+#ifndef REWL_H
+#define REWL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum wlx_status {
+    WLX_OK = 0,
+    WLX_INVALID_ARGUMENT = 1, /* std::invalid_argument */
+    WLX_RUNTIME_ERROR = 2,    /* std::runtime_error */
+    WLX_LOGIC_ERROR = 3,      /* std::logic_error */
+    WLX_OUT_OF_RANGE = 4,     /* std::out_of_range */
+    WLX_DEVICE_ERROR = 5      /* HIP runtime failure; no reference counterpart */
+};
+
+/* RefinementStage, wang_landau.hpp:58 */
+enum wlx_stage { WLX_STAGE_WANG_LANDAU = 0, WLX_STAGE_INVERSE_TIME = 1, WLX_STAGE_FROZEN = 2 };
+/* proposal/uniform stream: reference-compatible Xoshiro256** or counter-based Philox4x64-10 */
+enum wlx_rng { WLX_RNG_XOSHIRO = 0, WLX_RNG_PHILOX = 1 };
+/* local-field precision: FP64 (reference arithmetic) */
+enum wlx_fields { WLX_FIELDS_FP64 = 0 };
+
+/* trace flags, one byte per recorded attempt */
+enum wlx_trace_flag {
+    WLX_TRACE_ACCEPTED = 1,
+    WLX_TRACE_FORCED = 2,
+    WLX_TRACE_UNIFORM = 4,   /* a uniform was drawn for the decision */
+    WLX_TRACE_IN_WINDOW = 8  /* the proposed bin lies inside the walker's window */
+};
+
+typedef struct wlx_system wlx_system;
+typedef struct wlx_engine wlx_engine;
+
+/* WlParameters, wang_landau.hpp:40-56 (booleans widened to uint64). The pilot-statistics
+ * members (collect_window_statistics, round_trip_margin_fraction) belong to adaptive windows,
+ * which this engine does not implement. */
+typedef struct wlx_params {
+    double flatness;
+    uint64_t minimum_visits;
+    double final_factor;
+    uint64_t check_interval_attempts;
+    uint64_t force_accept_after_attempts;
+    uint64_t inverse_time_enabled;
+    uint64_t initialization_max_attempts;
+    double initialization_target_fraction;
+    double initialization_temperature_fraction;
+    uint64_t initialization_stall_attempts_per_spin;
+    double initialization_temperature_multiplier;
+    double initialization_max_temperature_fraction;
+    uint64_t nalivaiko_mod;
+} wlx_params;
+
+/* The part of RunConfig (io.hpp:12-58) that run_rewl (parallel.cpp:276-581) reads, with every
+ * interval already resolved to attempts (RunConfig::resolve_mcs, io.cpp:210-236). */
+typedef struct wlx_config {
+    double emin;                         /* EnergyGrid::minimum */
+    double emax;                         /* EnergyGrid::maximum */
+    double bin_width;                    /* EnergyGrid::width */
+    uint64_t windows;                    /* RunConfig::windows */
+    uint64_t walkers_per_window;         /* RunConfig::walkers_per_rank */
+    double overlap;                      /* RunConfig::overlap */
+    uint64_t seed;                       /* RunConfig::seed */
+    uint64_t exchange_interval_attempts; /* RunConfig::exchange_interval_attempts */
+    uint64_t max_attempts;               /* RunConfig::max_attempts; 0 = until all frozen */
+    uint64_t rng;                        /* enum wlx_rng */
+    uint64_t fields;                     /* enum wlx_fields */
+    const uint64_t *explicit_windows;    /* NULL or 2*windows words (begin, end) */
+    wlx_params wl;
+} wlx_config;
+
+/* Per-walker scalars (WalkerSnapshot, wang_landau.hpp:74-89, plus driver bookkeeping). */
+typedef struct wlx_walker_info {
+    uint64_t walker_id;
+    uint64_t window;
+    uint64_t window_begin;
+    uint64_t window_end;
+    double energy;
+    double factor;
+    uint64_t attempted;
+    uint64_t accepted;
+    uint64_t forced_accepted;
+    uint64_t last_accepted_attempt;
+    uint64_t stage; /* enum wlx_stage */
+    uint64_t active_bins;
+    uint64_t refinement_active_bins;
+    uint64_t rng_state[4];
+    uint64_t last_flatness_check; /* parallel.cpp:364 */
+    double last_flatness;         /* parallel.cpp:365 */
+} wlx_walker_info;
+
+/* HistogramStatistics, wang_landau.hpp:60-67 */
+typedef struct wlx_histogram_stats {
+    uint64_t active_bins;
+    uint64_t covered_bins;
+    uint64_t minimum;
+    double mean;
+    double min_over_mean;
+    double coverage;
+} wlx_histogram_stats;
+
+/* Run counters (RewlResult, parallel.hpp:23-31) plus device diagnostics. */
+typedef struct wlx_run_status {
+    uint64_t epoch;               /* exchange epochs completed */
+    uint64_t done;                /* all frozen or attempt limit reached (parallel.cpp:527-533) */
+    uint64_t converged;           /* every walker frozen */
+    uint64_t attempted;
+    uint64_t accepted;
+    uint64_t forced_accepted;
+    uint64_t exchange_attempted;
+    uint64_t exchange_accepted;
+    uint64_t ambiguous_decisions; /* log(u) within 4 ulp of its threshold (libm-sensitive) */
+} wlx_run_status;
+
+/* ---- status ---------------------------------------------------------------------------- */
+
+/* Copies the message of the last failed call on this thread (NUL-terminated, truncated to
+ * capacity) and returns its full length. Replaces the what() texts of the reference. */
+size_t wlx_last_error(char *buffer, size_t capacity);
+
+/* ---- coupling system: kernel (a) ------------------------------------------------------- */
+
+/* Builds J_ij on the device. cutoff <= 0 selects the dense N x N backend (DenseCouplings,
+ * physics.cpp:172-184), cutoff > 0 the CSR backend with norm(d) <= cutoff (CsrCouplings,
+ * physics.cpp:197-223). positions/axes are N x 3 row-major; box holds the three box lengths
+ * (0 = open axis), periodic as in Box (physics.hpp:17-20). Validates like Geometry::validate
+ * (physics.cpp:34-56). */
+int wlx_system_create(const double *positions, const double *axes, uint64_t n,
+                      const double box[3], int periodic, double coupling_scale, double cutoff,
+                      wlx_system **out);
+int wlx_system_destroy(wlx_system *system);
+/* Couplings::size (physics.hpp:44) and backend_name: 0 = dense, 1 = csr */
+int wlx_system_info(const wlx_system *system, uint64_t *n, uint64_t *backend, uint64_t *nnz);
+/* Copies J to the host: dense N*N row-major (DenseCouplings::at, physics.cpp:186-188). */
+int wlx_system_read_dense(const wlx_system *system, double *out);
+/* Copies the CSR arrays: offsets[N+1], columns[nnz], values[nnz] (physics.cpp:197-223). */
+int wlx_system_read_csr(const wlx_system *system, uint64_t *offsets, uint64_t *columns,
+                        double *values);
+/* local_fields + total_energy (physics.cpp:238-259) for `count` configurations of N spins,
+ * in the reference's summation order. fields may be NULL. */
+int wlx_system_fields(const wlx_system *system, const int8_t *spins, uint64_t count,
+                      double *fields, double *energies);
+
+/* ---- engine: kernels (b) and the device-side replica exchange --------------------------- */
+
+/* Allocates device-resident walkers for run_rewl (parallel.cpp:276-360): windows from
+ * partition_windows (wang_landau.cpp:40-61) or explicit, walker id = window*walkers + local. */
+int wlx_engine_create(wlx_system *system, const wlx_config *config, wlx_engine **out);
+int wlx_engine_destroy(wlx_engine *engine);
+/* Walker construction + target-Metropolis initialisation (wang_landau.cpp:63-242) of every
+ * walker on the device. initial_spins: NULL, or walkers*N spins used where supplied[w] != 0
+ * (RunConfig::initial_spins_by_walker, parallel.cpp:315-319). On failure returns
+ * WLX_RUNTIME_ERROR with "Walker initialization failed: window W, local walker L: ..."
+ * (parallel.cpp:323-352). */
+int wlx_engine_initialize(wlx_engine *engine, const int8_t *initial_spins,
+                          const uint8_t *supplied);
+/* WangLandauWalker::run_attempts (wang_landau.cpp:286-288) on every walker. */
+int wlx_engine_sample(wlx_engine *engine, uint64_t attempts);
+/* The flatness/coverage check cadence of one epoch (parallel.cpp:395-407). */
+int wlx_engine_check(wlx_engine *engine);
+/* Even/odd replica exchange of one epoch (parallel.cpp:410-421, 157-166). */
+int wlx_engine_exchange(wlx_engine *engine, uint64_t epoch, uint64_t *attempted,
+                        uint64_t *accepted);
+/* The run_rewl epoch loop (parallel.cpp:376-534) for at most max_epochs epochs; stops early
+ * when every walker is frozen or the attempt limit is reached. Resumable. */
+int wlx_engine_run(wlx_engine *engine, uint64_t max_epochs, wlx_run_status *status);
+int wlx_engine_status(wlx_engine *engine, wlx_run_status *status);
+int wlx_engine_walkers(const wlx_engine *engine, uint64_t *walkers, uint64_t *spins,
+                       uint64_t *bins);
+/* WangLandauWalker::snapshot (wang_landau.cpp:415-418). Arrays are full-grid (bins); any may
+ * be NULL. */
+int wlx_engine_read_walker(wlx_engine *engine, uint64_t walker, wlx_walker_info *info,
+                           int8_t *spins, double *fields, double *log_g, uint64_t *histogram,
+                           uint8_t *active, uint8_t *refinement_active);
+/* WangLandauWalker::restore (wang_landau.cpp:420-466): same validation and messages. */
+int wlx_engine_write_walker(wlx_engine *engine, uint64_t walker, const wlx_walker_info *info,
+                            const int8_t *spins, const double *fields, const double *log_g,
+                            const uint64_t *histogram, const uint8_t *active);
+/* WangLandauWalker::histogram_statistics (wang_landau.cpp:344-367), computed on the device. */
+int wlx_engine_histogram_statistics(wlx_engine *engine, uint64_t walker,
+                                    wlx_histogram_stats *out);
+/* ready_for_iteration (wang_landau.cpp:384-387); *ready = 0/1. */
+int wlx_engine_ready(wlx_engine *engine, uint64_t walker, uint64_t *ready);
+/* begin_next_iteration (wang_landau.cpp:389-409); WLX_LOGIC_ERROR if not ready. */
+int wlx_engine_next_iteration(wlx_engine *engine, uint64_t walker);
+/* Records the next `capacity` attempts of every walker (site, bin after the attempt, flags). */
+int wlx_engine_trace(wlx_engine *engine, uint64_t capacity);
+int wlx_engine_read_trace(wlx_engine *engine, uint64_t walker, uint64_t *count,
+                          uint32_t *sites, uint32_t *bins, uint8_t *flags);
+/* Times every sampling-kernel launch of wlx_engine_run with HIP events on the engine stream
+ * (enable != 0) and resets the accumulators; wlx_engine_kernel_time returns the mean launch
+ * duration in ms and the number of timed launches. No reference counterpart. */
+int wlx_engine_set_timing(wlx_engine *engine, int enable);
+int wlx_engine_kernel_time(wlx_engine *engine, double *sample_ms, uint64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* REWL_H */

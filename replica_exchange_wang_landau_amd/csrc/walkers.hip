@@ -1,0 +1,1567 @@
+// Kernel (b): device-resident Wang-Landau walkers with on-device replica exchange.
+//
+// Replaces WangLandauWalker (wang_landau.cpp:63-508), the OpenMP walker loop of run_rewl
+// (parallel.cpp:376-421) and its exchanges (parallel.cpp:157-166). One wavefront owns one
+// walker for a whole batch of attempts:
+//   - the N local fields live in registers, site = k*64 + lane (k < npl), and a flip streams
+//     row J[i, :] from HBM/L2 with coalesced 512-byte accesses (J is symmetric, so the row is
+//     the reference's column walk of physics.cpp:190-195, element for element);
+//   - spins are one bit per (lane, k) in a 64-bit register;
+//   - log_g / H / active masks of the walker's window live in LDS during the batch;
+//   - the RNG (Xoshiro256** or Philox4x64-10) and all decisions are wave-uniform.
+// The arithmetic is the reference's, operation for operation (compiled with
+// -ffp-contract=off), so in Xoshiro mode a walker's trajectory, spins, fields, energy and
+// estimator are bit-identical to the reference walker's.
+//
+// Configurations (spins, fields, energy) sit in "slots"; a walker points at its slot, so an
+// accepted replica exchange (swap_configuration, wang_landau.cpp:483-495) is an O(1) swap of
+// two slot indices on the device.
+#include "engine.h"
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+namespace rewl {
+
+struct WalkerState {
+    uint64_t rng[4];
+    double factor;
+    double last_flatness;
+    uint64_t attempted, accepted, forced, last_accepted;
+    uint64_t last_check;
+    uint64_t ambiguous;
+    uint32_t stage, slot;
+    uint32_t active_count, refine_count;
+    uint32_t begin, end;
+    uint32_t trace_count, escaped;
+};
+static_assert(sizeof(WalkerState) == 128, "walker state layout");
+
+struct Globals {
+    uint64_t epoch;
+    uint64_t done;
+    uint64_t exchange_attempted;
+    uint64_t exchange_accepted;
+    uint64_t exchange_ambiguous;
+    uint64_t escaped;
+    uint64_t pad[2];
+};
+
+struct InitReport {
+    uint64_t status; // 0 ok, 1 target band not reached, 2 spins not +-1
+    double min_reached, max_reached, closest, closest_distance, highest;
+    uint64_t increases, restarts;
+};
+
+struct InitParams {
+    uint64_t max_attempts;
+    double target_fraction;
+    double temperature_fraction;
+    uint64_t stall_per_spin;
+    double temperature_multiplier;
+    double max_temperature_fraction;
+};
+
+struct Args {
+    const double *J;
+    uint32_t n, np;
+    Bounded bounded;
+    Grid grid;
+    WalkerState *state;
+    int8_t *spins;
+    double *fields;
+    double *energy;
+    double *logg;
+    uint64_t *hist;
+    uint8_t *active;
+    uint8_t *refine;
+    uint32_t ws_stride;
+    uint32_t walkers, windows, per_window;
+    uint32_t *trace_site;
+    uint32_t *trace_bin;
+    uint8_t *trace_flag;
+    uint64_t trace_cap;
+    Globals *glob;
+    uint64_t seed;
+    uint64_t exchange_interval, max_attempts, check_interval, force_after, min_visits;
+    double final_factor, flatness;
+    uint32_t inverse_time, nalivaiko, philox, unlimited;
+};
+
+enum : uint32_t { kGate = 1, kCheck = 2, kTrace = 4 };
+
+// ---- wave helpers -------------------------------------------------------------------------
+__device__ inline uint32_t lane_id() { return threadIdx.x; }
+
+__device__ inline double readlane_f64(double v, uint32_t lane)
+{
+    const uint64_t bits = __double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(bits), lane);
+    const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(bits >> 32), lane);
+    return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
+}
+
+__device__ inline uint64_t readlane_u64(uint64_t v, uint32_t lane)
+{
+    const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(v), lane);
+    const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(v >> 32), lane);
+    return (uint64_t(hi) << 32) | lo;
+}
+
+__device__ inline uint64_t shfl_xor_u64(uint64_t v, int mask)
+{
+    const uint32_t lo = __shfl_xor(uint32_t(v), mask);
+    const uint32_t hi = __shfl_xor(uint32_t(v >> 32), mask);
+    return (uint64_t(hi) << 32) | lo;
+}
+
+// h[k] for a wave-uniform k, as a binary tree of selects on the bits of k (an equality
+// chain is turned into a stack lookup table by the compiler, which costs scratch traffic).
+template <int NPL> __device__ inline double select_slot(const double (&h)[NPL], uint32_t k)
+{
+    double level[NPL];
+#pragma unroll
+    for (int j = 0; j < NPL; ++j) level[j] = h[j];
+#pragma unroll
+    for (int width = NPL, bit = 0; width > 1; width >>= 1, ++bit) {
+        const bool upper = (k >> bit) & 1;
+#pragma unroll
+        for (int j = 0; j < width / 2; ++j) level[j] = upper ? level[2 * j + 1] : level[2 * j];
+    }
+    return level[0];
+}
+
+// ---- window bins in LDS -------------------------------------------------------------------
+struct Bins {
+    double *lg;
+    uint64_t *H;
+    uint8_t *act;
+    uint8_t *ref;
+};
+
+__host__ __device__ inline size_t bins_lds_bytes(uint32_t stride) { return size_t(stride) * 18; }
+
+__device__ inline Bins bins_view(unsigned char *smem, uint32_t stride)
+{
+    return {reinterpret_cast<double *>(smem), reinterpret_cast<uint64_t *>(smem + 8 * size_t(stride)),
+            smem + 16 * size_t(stride), smem + 17 * size_t(stride)};
+}
+
+__device__ inline void load_bins(const Args &a, uint32_t w, uint32_t ws, Bins v)
+{
+    const size_t base = size_t(w) * a.ws_stride;
+    for (uint32_t b = lane_id(); b < ws; b += kWave) {
+        v.lg[b] = a.logg[base + b];
+        v.H[b] = a.hist[base + b];
+        v.act[b] = a.active[base + b];
+        v.ref[b] = a.refine[base + b];
+    }
+}
+
+__device__ inline void store_bins(const Args &a, uint32_t w, uint32_t ws, Bins v)
+{
+    const size_t base = size_t(w) * a.ws_stride;
+    for (uint32_t b = lane_id(); b < ws; b += kWave) {
+        a.logg[base + b] = v.lg[b];
+        a.hist[base + b] = v.H[b];
+        a.active[base + b] = v.act[b];
+        a.refine[base + b] = v.ref[b];
+    }
+}
+
+// ---- walker primitives (uniform across the wave) --------------------------------------------
+
+// update_current_bin (wang_landau.cpp:290-303); `b` is window-relative
+__device__ inline void update_current_bin(const Args &a, WalkerState &st, Bins v, uint32_t b)
+{
+    const uint64_t count = v.H[b];
+    const uint8_t seen = v.act[b];
+    if (seen == 0) ++st.active_count;
+    uint8_t refined = 1;
+    if (a.nalivaiko) {
+        refined = v.ref[b];
+        if (refined == 0) ++st.refine_count;
+    }
+    double lg = v.lg[b];
+    if (st.stage != WLX_STAGE_FROZEN) lg += st.factor;
+    if (lane_id() == 0) {
+        v.H[b] = count + 1;
+        v.act[b] = 1;
+        if (a.nalivaiko) v.ref[b] = 1;
+        v.lg[b] = lg;
+    }
+}
+
+struct Stats {
+    uint64_t active, covered, minimum;
+    double mean, min_over_mean, coverage;
+};
+
+// histogram_statistics (wang_landau.cpp:344-367): lanes stride the window, then reduce
+__device__ inline Stats statistics(const Args &a, const WalkerState &st, Bins v, uint32_t ws)
+{
+    const uint8_t *mask = a.nalivaiko ? v.ref : v.act;
+    uint64_t covered = 0, total = 0, minimum = ~uint64_t(0);
+    for (uint32_t b = lane_id(); b < ws; b += kWave) {
+        if (!mask[b]) continue;
+        const uint64_t h = v.H[b];
+        covered += h != 0;
+        minimum = h < minimum ? h : minimum;
+        total += h;
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        covered += shfl_xor_u64(covered, off);
+        total += shfl_xor_u64(total, off);
+        const uint64_t other = shfl_xor_u64(minimum, off);
+        minimum = other < minimum ? other : minimum;
+    }
+    Stats s{};
+    s.active = a.nalivaiko ? st.refine_count : st.active_count;
+    s.covered = covered;
+    if (s.active == 0) return s;
+    s.minimum = minimum;
+    s.mean = extended_quotient(total, s.active);
+    s.min_over_mean = s.mean > 0.0 ? double(minimum) / s.mean : 0.0;
+    s.coverage = double(covered) / double(s.active);
+    return s;
+}
+
+// flat / covered / ready_for_iteration (wang_landau.cpp:369-387)
+__device__ inline bool ready_for_iteration(const Args &a, const WalkerState &st, const Stats &s)
+{
+    if (st.stage != WLX_STAGE_WANG_LANDAU) return false;
+    if (a.inverse_time) return s.active != 0 && s.covered == s.active;
+    return s.active != 0 && s.minimum >= a.min_visits && s.min_over_mean >= a.flatness;
+}
+
+// begin_next_iteration + freeze_if_finished (wang_landau.cpp:389-413), readiness checked
+__device__ inline void next_iteration(const Args &a, WalkerState &st, Bins v, uint32_t ws)
+{
+    st.factor *= 0.5;
+    for (uint32_t b = lane_id(); b < ws; b += kWave) v.H[b] = 0;
+    if (a.nalivaiko) {
+        for (uint32_t b = lane_id(); b < ws; b += kWave) v.ref[b] = 0;
+        st.refine_count = 0;
+    }
+    if (a.inverse_time) {
+        const double inverse_time =
+            st.attempted == 0 ? 1.0 : double(st.active_count) / double(st.attempted);
+        if (st.factor <= inverse_time) {
+            st.stage = WLX_STAGE_INVERSE_TIME;
+            st.factor = inverse_time;
+        }
+    }
+    if (st.factor <= a.final_factor) st.stage = WLX_STAGE_FROZEN;
+}
+
+__device__ inline bool near_threshold(double x, double threshold)
+{
+    return fabs(x - threshold) <= 8.9e-16 * fmax(fabs(x), fabs(threshold));
+}
+
+// local_fields (physics.cpp:238-251) for site k*64+lane: sum over j ascending of J[j][i]*s_j
+// (J is bitwise symmetric); the products are exact, so the sum is the reference's.
+template <int NPL>
+__device__ inline void compute_fields(const Args &a, uint64_t neg, double (&h)[NPL])
+{
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) h[k] = 0.0;
+    for (uint32_t j = 0; j < a.n; ++j) {
+        const uint64_t bits = readlane_u64(neg, j & (kWave - 1));
+        const double s = ((bits >> (j >> 6)) & 1) ? -1.0 : 1.0;
+        const double *row = a.J + size_t(j) * a.np;
+#pragma unroll
+        for (int k = 0; k < NPL; ++k) h[k] += row[k * kWave + lane] * s;
+    }
+}
+
+// total_energy (physics.cpp:253-259): sequential sum over i ascending, via LDS scratch
+template <int NPL>
+__device__ inline double total_energy(const Args &a, uint64_t neg, const double (&h)[NPL],
+                                      double *scratch)
+{
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) {
+        const double s = ((neg >> k) & 1) ? -1.0 : 1.0;
+        scratch[k * kWave + lane] = 0.5 * s * h[k];
+    }
+    __syncthreads();
+    double e = 0.0;
+    if (lane == 0)
+        for (uint32_t i = 0; i < a.n; ++i) e += scratch[i];
+    e = readlane_f64(e, 0);
+    __syncthreads();
+    return e;
+}
+
+template <int NPL> __device__ inline void load_config(const Args &a, uint32_t slot, uint64_t &neg,
+                                                      double (&h)[NPL])
+{
+    const uint32_t lane = lane_id();
+    const size_t base = size_t(slot) * a.np;
+    neg = 0;
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) {
+        h[k] = a.fields[base + k * kWave + lane];
+        if (a.spins[base + k * kWave + lane] < 0) neg |= uint64_t(1) << k;
+    }
+}
+
+template <int NPL> __device__ inline void store_config(const Args &a, uint32_t slot, uint64_t neg,
+                                                       const double (&h)[NPL])
+{
+    const uint32_t lane = lane_id();
+    const size_t base = size_t(slot) * a.np;
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) {
+        a.fields[base + k * kWave + lane] = h[k];
+        a.spins[base + k * kWave + lane] = ((neg >> k) & 1) ? int8_t(-1) : int8_t(1);
+    }
+}
+
+// the flip of an accepted proposal: h += (-2 s_old) J[site, :] (physics.cpp:190-195), spin bit
+template <int NPL>
+__device__ inline void apply_flip(const Args &a, uint32_t site, double sigma, uint64_t &neg,
+                                  double (&h)[NPL])
+{
+    const uint32_t lane = lane_id();
+    const double m = -2.0 * sigma;
+    const double *row = a.J + size_t(site) * a.np;
+    constexpr int kChunk = NPL < 16 ? NPL : 16; // loads in flight per lane
+#pragma unroll
+    for (int c = 0; c < NPL; c += kChunk) {
+        double r[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) r[k] = row[(c + k) * kWave + lane];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) h[c + k] += m * r[k];
+    }
+    if (lane == (site & (kWave - 1))) neg ^= uint64_t(1) << (site >> 6);
+}
+
+// ---------------------------------------------------------------------------------------------
+// sample: `count` attempt_flip calls (wang_landau.cpp:244-284) per walker; with kGate the
+// batch is run_rewl's min(exchange_interval, max - attempted) and a finished run is skipped;
+// with kCheck the flatness cadence of parallel.cpp:395-407 follows the batch.
+// ---------------------------------------------------------------------------------------------
+template <int NPL, bool PHILOX>
+__global__ __launch_bounds__(kWave) void sample_kernel(Args a, uint64_t request, uint32_t flags)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t w = blockIdx.x, lane = lane_id();
+    if ((flags & kGate) && a.glob->done) return;
+    WalkerState st = a.state[w];
+    if (st.escaped) return;
+    const uint32_t ws = st.end - st.begin;
+    Bins v = bins_view(smem, a.ws_stride);
+    load_bins(a, w, ws, v);
+    uint64_t neg;
+    double h[NPL];
+    load_config<NPL>(a, st.slot, neg, h);
+    double energy = a.energy[st.slot];
+    __syncthreads();
+
+    uint64_t count = request;
+    if (flags & kGate) {
+        count = a.exchange_interval;
+        if (!a.unlimited) {
+            const uint64_t remaining =
+                st.attempted < a.max_attempts ? a.max_attempts - st.attempted : 0;
+            count = remaining < count ? remaining : count;
+        }
+    }
+    Xoshiro rng{st.rng[0], st.rng[1], st.rng[2], st.rng[3]};
+    const uint64_t trace_base = uint64_t(w) * a.trace_cap;
+    int64_t cur = grid_index(a.grid, energy);
+    Philox4 block{};
+    for (uint64_t t = 0; t < count; ++t) {
+        uint64_t site;
+        if (PHILOX) {
+            if (t == 0 || (st.attempted & 1) == 0)
+                block = philox4x64(st.attempted >> 1, kPhiloxWalkTag, 0, 0, a.seed,
+                                   uint64_t(w));
+            site = __umul64hi((st.attempted & 1) ? block.v[2] : block.v[0], uint64_t(a.n));
+        } else {
+            site = draw_bounded(rng, a.bounded);
+        }
+        if (cur < int64_t(st.begin) || cur >= int64_t(st.end)) { // wang_landau.cpp:247-248
+            st.escaped = 1;
+            break;
+        }
+        const uint32_t s = uint32_t(site);
+        const double hs = readlane_f64(select_slot<NPL>(h, s >> 6), s & (kWave - 1));
+        const uint64_t bits = readlane_u64(neg, s & (kWave - 1));
+        const double sigma = ((bits >> (s >> 6)) & 1) ? -1.0 : 1.0;
+        const double delta = -2.0 * sigma * hs; // flip_delta, physics.cpp:261-266
+        const double proposed = energy + delta;
+        const int64_t next = grid_index(a.grid, proposed);
+        bool accepted = false, forced = false, drew = false, inside = false;
+        if (next >= int64_t(st.begin) && next < int64_t(st.end)) {
+            inside = true;
+            const double log_ratio = v.lg[cur - st.begin] - v.lg[next - st.begin];
+            const bool force_due =
+                a.force_after != 0 && st.attempted - st.last_accepted >= a.force_after;
+            if (log_ratio >= 0.0)
+                accepted = true;
+            else if (force_due)
+                accepted = forced = true;
+            else {
+                const double u =
+                    PHILOX ? double(((st.attempted & 1) ? block.v[3] : block.v[1]) >> 11) *
+                                 0x1.0p-53
+                           : rng.uniform();
+                drew = true;
+                const double lu = log(fmax(u, 0x1.0p-53));
+                accepted = lu < log_ratio;
+                if (near_threshold(lu, log_ratio)) ++st.ambiguous;
+            }
+        }
+        ++st.attempted;
+        if (accepted) {
+            apply_flip<NPL>(a, s, sigma, neg, h);
+            energy = proposed;
+            ++st.accepted;
+            if (forced) ++st.forced;
+            st.last_accepted = st.attempted;
+            cur = next;
+        }
+        if (st.stage == WLX_STAGE_INVERSE_TIME) { // update_inverse_time_factor, 309-315
+            if (st.active_count != 0 && st.attempted != 0)
+                st.factor = double(st.active_count) / double(st.attempted);
+            if (st.factor <= a.final_factor) st.stage = WLX_STAGE_FROZEN;
+        }
+        update_current_bin(a, st, v, uint32_t(cur - st.begin));
+        if ((flags & kTrace) && st.trace_count < a.trace_cap) {
+            if (lane == 0) {
+                const uint64_t k = trace_base + st.trace_count;
+                a.trace_site[k] = s;
+                a.trace_bin[k] = uint32_t(cur);
+                a.trace_flag[k] = uint8_t((accepted ? WLX_TRACE_ACCEPTED : 0) |
+                                          (forced ? WLX_TRACE_FORCED : 0) |
+                                          (drew ? WLX_TRACE_UNIFORM : 0) |
+                                          (inside ? WLX_TRACE_IN_WINDOW : 0));
+            }
+            ++st.trace_count;
+        }
+    }
+    st.rng[0] = rng.s0;
+    st.rng[1] = rng.s1;
+    st.rng[2] = rng.s2;
+    st.rng[3] = rng.s3;
+
+    if ((flags & kCheck) && !st.escaped && st.stage == WLX_STAGE_WANG_LANDAU) {
+        __syncthreads();
+        const uint64_t interval = a.check_interval > 1 ? a.check_interval : 1;
+        const bool at_limit = !a.unlimited && st.attempted >= a.max_attempts;
+        if (st.attempted - st.last_check >= interval || at_limit) {
+            st.last_check = st.attempted;
+            const Stats s = statistics(a, st, v, ws);
+            st.last_flatness = a.inverse_time ? s.coverage : s.min_over_mean;
+            if (ready_for_iteration(a, st, s)) next_iteration(a, st, v, ws);
+        }
+    }
+    __syncthreads();
+    store_bins(a, w, ws, v);
+    store_config<NPL>(a, st.slot, neg, h);
+    if (lane == 0) {
+        a.energy[st.slot] = energy;
+        a.state[w] = st;
+        if (st.escaped) atomicOr(reinterpret_cast<unsigned long long *>(&a.glob->escaped), 1ull);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// init: walker construction + target-Metropolis initialisation (wang_landau.cpp:91-242)
+// ---------------------------------------------------------------------------------------------
+template <int NPL>
+__global__ __launch_bounds__(kWave) void init_kernel(Args a, const int8_t *initial,
+                                                     const uint8_t *supplied_mask, InitParams ip,
+                                                     InitReport *report)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t w = blockIdx.x, lane = lane_id();
+    WalkerState st = a.state[w];
+    const uint32_t ws = st.end - st.begin;
+    Bins v = bins_view(smem, a.ws_stride);
+    double *scratch = reinterpret_cast<double *>(smem + bins_lds_bytes(a.ws_stride));
+    for (uint32_t b = lane; b < ws; b += kWave) {
+        v.lg[b] = 0.0;
+        v.H[b] = 0;
+        v.act[b] = 0;
+        v.ref[b] = 0;
+    }
+    InitReport rep{};
+    const bool supplied = supplied_mask && supplied_mask[w];
+    uint64_t neg = 0;
+    bool bad_spin = false;
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) {
+        const uint32_t i = k * kWave + lane;
+        const int8_t value = (supplied && i < a.n) ? initial[size_t(w) * a.n + i] : int8_t(1);
+        if (value != 1 && value != -1) bad_spin = true;
+        if (value < 0) neg |= uint64_t(1) << k;
+    }
+    if (__any(bad_spin)) {
+        if (lane == 0) {
+            rep.status = 2;
+            report[w] = rep;
+        }
+        return;
+    }
+    Xoshiro rng = Xoshiro::seeded(a.seed, w); // rng_(master_seed, walker_id)
+    st.factor = 1.0;
+    st.last_flatness = 0.0;
+    st.attempted = st.accepted = st.forced = st.last_accepted = 0;
+    st.last_check = 0;
+    st.ambiguous = 0;
+    st.stage = WLX_STAGE_WANG_LANDAU;
+    st.active_count = st.refine_count = 0;
+    st.trace_count = 0;
+    st.escaped = 0;
+    double h[NPL];
+    compute_fields<NPL>(a, neg, h);
+    double energy = total_energy<NPL>(a, neg, h, scratch);
+
+    const int64_t begin = st.begin, end = st.end;
+    auto in_window = [&](int64_t b) { return b >= begin && b < end; };
+    bool placed = false;
+    if (supplied && in_window(grid_index(a.grid, energy))) {
+        placed = true;
+    } else {
+        const double lower = a.grid.minimum + double(st.begin) * a.grid.width;
+        const double upper = a.grid.minimum + double(st.end) * a.grid.width;
+        const double span = upper - lower;
+        const double target = 0.5 * (lower + upper);
+        const double target_fraction = supplied ? 1.0 : ip.target_fraction;
+        const double half = 0.5 * target_fraction * span;
+        const double tf = ip.temperature_fraction * span;
+        const double t0 = a.grid.width < tf ? tf : a.grid.width;
+        const double tm = ip.max_temperature_fraction * span;
+        const double tmax = t0 < tm ? tm : t0;
+        double temperature = t0;
+        const uint64_t stall = ip.stall_per_spin > ~uint64_t(0) / a.n ? ~uint64_t(0)
+                                                                       : ip.stall_per_spin * a.n;
+        uint64_t since = 0;
+        rep.highest = temperature;
+        rep.min_reached = rep.max_reached = rep.closest = energy;
+        rep.closest_distance = fabs(energy - target);
+        auto record = [&](double c) {
+            if (c < rep.min_reached) rep.min_reached = c;
+            if (rep.max_reached < c) rep.max_reached = c;
+            const double d = fabs(c - target);
+            if (d < rep.closest_distance) {
+                rep.closest_distance = d;
+                rep.closest = c;
+                return true;
+            }
+            return false;
+        };
+        auto in_band = [&](double c) {
+            return in_window(grid_index(a.grid, c)) && fabs(c - target) <= half;
+        };
+        const Bounded two = make_bounded(2);
+        for (uint64_t step = 0; step < ip.max_attempts && !placed; ++step) {
+            if (in_band(energy)) {
+                placed = true;
+                break;
+            }
+            if (since >= stall) {
+                if (temperature < tmax) {
+                    const double raised = temperature * ip.temperature_multiplier;
+                    temperature = raised < tmax ? raised : tmax;
+                    if (rep.highest < temperature) rep.highest = temperature;
+                    ++rep.increases;
+                } else { // reproducible random restart, wang_landau.cpp:187-193
+                    neg = 0;
+                    for (uint32_t i = 0; i < a.n; ++i) {
+                        const bool down = draw_bounded(rng, two) == 0;
+                        if (down && lane == (i & (kWave - 1))) neg |= uint64_t(1) << (i >> 6);
+                    }
+                    compute_fields<NPL>(a, neg, h);
+                    energy = total_energy<NPL>(a, neg, h, scratch);
+                    (void)record(energy);
+                    temperature = t0;
+                    ++rep.restarts;
+                }
+                since = 0;
+                if (in_band(energy)) {
+                    placed = true;
+                    break;
+                }
+            }
+            const uint32_t s = uint32_t(draw_bounded(rng, a.bounded));
+            const double hs = readlane_f64(select_slot<NPL>(h, s >> 6), s & (kWave - 1));
+            const uint64_t bits = readlane_u64(neg, s & (kWave - 1));
+            const double sigma = ((bits >> (s >> 6)) & 1) ? -1.0 : 1.0;
+            const double delta = -2.0 * sigma * hs;
+            const double proposed = energy + delta;
+            const double log_acceptance =
+                (fabs(energy - target) - fabs(proposed - target)) / temperature;
+            bool improved = false;
+            if (log_acceptance >= 0.0 ||
+                log(fmax(rng.uniform(), 0x1.0p-53)) < log_acceptance) {
+                apply_flip<NPL>(a, s, sigma, neg, h);
+                energy = proposed;
+                improved = record(energy);
+            }
+            if (improved)
+                since = 0;
+            else if (since < ~uint64_t(0))
+                ++since;
+        }
+        if (!placed && in_band(energy)) placed = true;
+    }
+    if (placed) {
+        __syncthreads();
+        update_current_bin(a, st, v, uint32_t(grid_index(a.grid, energy) - begin));
+    } else {
+        rep.status = 1;
+    }
+    st.rng[0] = rng.s0;
+    st.rng[1] = rng.s1;
+    st.rng[2] = rng.s2;
+    st.rng[3] = rng.s3;
+    __syncthreads();
+    store_bins(a, w, ws, v);
+    store_config<NPL>(a, st.slot, neg, h);
+    if (lane == 0) {
+        a.energy[st.slot] = energy;
+        a.state[w] = st;
+        report[w] = rep;
+    }
+}
+
+// local_fields + total_energy for stored configurations (one wave per configuration)
+template <int NPL>
+__global__ __launch_bounds__(kWave) void fields_kernel(Args a, const int8_t *spins,
+                                                       double *fields, double *energies)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t c = blockIdx.x, lane = lane_id();
+    uint64_t neg = 0;
+#pragma unroll
+    for (int k = 0; k < NPL; ++k)
+        if (spins[size_t(c) * a.np + k * kWave + lane] < 0) neg |= uint64_t(1) << k;
+    double h[NPL];
+    compute_fields<NPL>(a, neg, h);
+    const double e = total_energy<NPL>(a, neg, h, reinterpret_cast<double *>(smem));
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) fields[size_t(c) * a.np + k * kWave + lane] = h[k];
+    if (lane == 0) energies[c] = e;
+}
+
+// ---------------------------------------------------------------------------------------------
+// exchange: the even/odd schedule of parallel.cpp:410-421. Each walker is in at most one pair
+// per epoch, so all pairs are independent; a swap exchanges two slot indices.
+// ---------------------------------------------------------------------------------------------
+constexpr int kExchangeBlock = 256;
+
+__global__ __launch_bounds__(kExchangeBlock) void exchange_kernel(Args a, uint32_t gated,
+                                                                  uint64_t epoch_arg)
+{
+    if (gated && a.glob->done) return;
+    const uint64_t epoch = gated ? a.glob->epoch : epoch_arg;
+    const uint32_t parity = uint32_t(epoch & 1);
+    const uint32_t P = a.per_window;
+    const uint32_t boundaries = a.windows > parity + 1 ? (a.windows - parity) / 2 : 0;
+    const uint64_t total = uint64_t(boundaries) * P;
+    uint64_t taken = 0, ambiguous = 0;
+    for (uint64_t t = threadIdx.x; t < total; t += blockDim.x) {
+        const uint32_t b = parity + 2 * uint32_t(t / P), p = uint32_t(t % P);
+        const uint32_t L = b * P + p, R = (b + 1) * P + uint32_t((p + epoch) % P);
+        WalkerState &l = a.state[L];
+        WalkerState &r = a.state[R];
+        const double el = a.energy[l.slot], er = a.energy[r.slot];
+        const int64_t x = grid_index(a.grid, el), y = grid_index(a.grid, er);
+        auto inside = [](const WalkerState &s, int64_t bin) {
+            return bin >= int64_t(s.begin) && bin < int64_t(s.end);
+        };
+        // exchange_log_probability (wang_landau.cpp:497-508)
+        if (!inside(l, x) || !inside(l, y) || !inside(r, x) || !inside(r, y)) continue;
+        const double *gl = a.logg + size_t(L) * a.ws_stride - l.begin;
+        const double *gr = a.logg + size_t(R) * a.ws_stride - r.begin;
+        const double lnp = gl[x] - gl[y] + gr[y] - gr[x];
+        if (!(lnp - lnp == 0.0)) continue; // not finite
+        bool accept = lnp >= 0.0;
+        if (!accept) { // local_exchange (parallel.cpp:157-166)
+            double u;
+            if (a.philox) {
+                const Philox4 blk = philox4x64(epoch, kPhiloxExchangeTag, 0, 0, a.seed, b);
+                u = double(blk.v[0] >> 11) * 0x1.0p-53;
+            } else {
+                Xoshiro rng = Xoshiro::seeded(a.seed ^ (epoch * kGolden), b);
+                u = rng.uniform();
+            }
+            const double lu = log(fmax(u, 0x1.0p-53));
+            accept = lu < lnp;
+            ambiguous += near_threshold(lu, lnp);
+        }
+        if (accept) {
+            const uint32_t slot = l.slot;
+            l.slot = r.slot;
+            r.slot = slot;
+            ++taken;
+        }
+    }
+    // termination flags (parallel.cpp:391-393, 527-530)
+    uint64_t not_frozen = 0, below_limit = 0;
+    for (uint32_t k = threadIdx.x; k < a.walkers; k += blockDim.x) {
+        not_frozen += a.state[k].stage != WLX_STAGE_FROZEN;
+        below_limit += a.state[k].attempted < a.max_attempts;
+    }
+    uint64_t vals[4] = {taken, ambiguous, not_frozen, below_limit};
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) vals[q] += shfl_xor_u64(vals[q], off);
+    __shared__ uint64_t sums[kExchangeBlock / kWave][4];
+    const uint32_t wave = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0)
+        for (int q = 0; q < 4; ++q) sums[wave][q] = vals[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t[4] = {0, 0, 0, 0};
+        for (int wv = 0; wv < kExchangeBlock / kWave; ++wv)
+            for (int q = 0; q < 4; ++q) t[q] += sums[wv][q];
+        a.glob->exchange_attempted += total;
+        a.glob->exchange_accepted += t[0];
+        a.glob->exchange_ambiguous += t[1];
+        if (gated) {
+            const bool frozen = t[2] == 0;
+            const bool limit = !a.unlimited && t[3] == 0;
+            a.glob->done = (frozen || limit) ? 1 : 0;
+            a.glob->epoch = epoch + 1;
+        }
+    }
+}
+
+// per-walker operations for the harness: 0 statistics, 1 ready, 2 begin_next_iteration
+__global__ __launch_bounds__(kWave) void walker_op_kernel(Args a, uint32_t w, uint32_t op,
+                                                          uint64_t *out)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    WalkerState st = a.state[w];
+    const uint32_t ws = st.end - st.begin;
+    Bins v = bins_view(smem, a.ws_stride);
+    load_bins(a, w, ws, v);
+    __syncthreads();
+    const Stats s = statistics(a, st, v, ws);
+    const bool ready = ready_for_iteration(a, st, s);
+    if (op == 2 && ready) next_iteration(a, st, v, ws);
+    __syncthreads();
+    if (op == 2 && ready) store_bins(a, w, ws, v);
+    if (lane_id() == 0) {
+        if (op == 2 && ready) a.state[w] = st;
+        out[0] = s.active;
+        out[1] = s.covered;
+        out[2] = s.minimum;
+        out[3] = __double_as_longlong(s.mean);
+        out[4] = __double_as_longlong(s.min_over_mean);
+        out[5] = __double_as_longlong(s.coverage);
+        out[6] = ready ? 1 : 0;
+    }
+}
+
+// ---- dispatch ------------------------------------------------------------------------------
+using SampleFn = void (*)(Args, uint64_t, uint32_t);
+using InitFn = void (*)(Args, const int8_t *, const uint8_t *, InitParams, InitReport *);
+using FieldsFn = void (*)(Args, const int8_t *, double *, double *);
+
+struct KernelSet {
+    SampleFn sample_xo, sample_ph;
+    InitFn init;
+    FieldsFn fields;
+};
+
+template <int NPL> KernelSet kernel_set()
+{
+    return {sample_kernel<NPL, false>, sample_kernel<NPL, true>, init_kernel<NPL>,
+            fields_kernel<NPL>};
+}
+
+KernelSet kernels_for(uint32_t npl)
+{
+    switch (npl) {
+    case 1: return kernel_set<1>();
+    case 2: return kernel_set<2>();
+    case 4: return kernel_set<4>();
+    case 8: return kernel_set<8>();
+    case 16: return kernel_set<16>();
+    case 32: return kernel_set<32>();
+    default: return kernel_set<64>();
+    }
+}
+
+int set_lds_limit(const void *fn, size_t bytes)
+{
+    if (bytes > 64 * 1024)
+        REWL_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+    return WLX_OK;
+}
+
+int system_fields_device(wlx_system *s, const int8_t *d_spins, uint64_t count, double *d_fields,
+                         double *d_energy, hipStream_t stream)
+{
+    Args a{};
+    a.J = s->d_J64;
+    a.n = uint32_t(s->n);
+    a.np = uint32_t(s->np);
+    const KernelSet k = kernels_for(s->npl);
+    const size_t lds = s->np * sizeof(double);
+    if (int status = set_lds_limit(reinterpret_cast<const void *>(k.fields), lds)) return status;
+    hipLaunchKernelGGL(k.fields, dim3(unsigned(count)), dim3(kWave), lds, stream, a, d_spins,
+                       d_fields, d_energy);
+    REWL_HIP(hipGetLastError());
+    return WLX_OK;
+}
+
+template <class T> int device_alloc(T **p, size_t count)
+{
+    REWL_HIP(hipMalloc(reinterpret_cast<void **>(p), std::max<size_t>(1, count) * sizeof(T)));
+    return WLX_OK;
+}
+
+} // namespace rewl
+
+using namespace rewl;
+
+struct wlx_engine {
+    wlx_system *sys = nullptr;
+    wlx_config cfg{};
+    std::vector<uint64_t> window_be;
+    Grid grid{};
+    uint64_t bins = 0;
+    uint32_t windows = 0, per_window = 0, walkers = 0, ws_stride = 8;
+    KernelSet kernels{};
+    WalkerState *d_state = nullptr;
+    int8_t *d_spins = nullptr;
+    double *d_fields = nullptr;
+    double *d_energy = nullptr;
+    double *d_logg = nullptr;
+    uint64_t *d_hist = nullptr;
+    uint8_t *d_active = nullptr;
+    uint8_t *d_refine = nullptr;
+    uint32_t *d_tsite = nullptr;
+    uint32_t *d_tbin = nullptr;
+    uint8_t *d_tflag = nullptr;
+    uint64_t trace_cap = 0;
+    Globals *d_glob = nullptr;
+    InitReport *d_report = nullptr;
+    uint64_t *d_op = nullptr;
+    hipStream_t stream = nullptr;
+    bool initialized = false;
+    size_t lds_sample = 0, lds_init = 0;
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    double sample_ms = 0.0;
+    uint64_t launches = 0;
+    uint64_t epochs_run = 0;
+    uint64_t chunk = 1;
+
+    Args args(uint32_t flags_trace = 0) const
+    {
+        Args a{};
+        a.J = sys->d_J64;
+        a.n = uint32_t(sys->n);
+        a.np = uint32_t(sys->np);
+        a.bounded = make_bounded(sys->n);
+        a.grid = grid;
+        a.state = d_state;
+        a.spins = d_spins;
+        a.fields = d_fields;
+        a.energy = d_energy;
+        a.logg = d_logg;
+        a.hist = d_hist;
+        a.active = d_active;
+        a.refine = d_refine;
+        a.ws_stride = ws_stride;
+        a.walkers = walkers;
+        a.windows = windows;
+        a.per_window = per_window;
+        a.trace_site = d_tsite;
+        a.trace_bin = d_tbin;
+        a.trace_flag = d_tflag;
+        a.trace_cap = trace_cap;
+        a.glob = d_glob;
+        a.seed = cfg.seed;
+        a.exchange_interval = cfg.exchange_interval_attempts;
+        a.max_attempts = cfg.max_attempts;
+        a.check_interval = cfg.wl.check_interval_attempts;
+        a.force_after = cfg.wl.force_accept_after_attempts;
+        a.min_visits = cfg.wl.minimum_visits;
+        a.final_factor = cfg.wl.final_factor;
+        a.flatness = cfg.wl.flatness;
+        a.inverse_time = cfg.wl.inverse_time_enabled ? 1 : 0;
+        a.nalivaiko = cfg.wl.nalivaiko_mod ? 1 : 0;
+        a.philox = cfg.rng == WLX_RNG_PHILOX ? 1 : 0;
+        a.unlimited = cfg.max_attempts == 0 ? 1 : 0;
+        (void)flags_trace;
+        return a;
+    }
+
+    SampleFn sample_fn() const
+    {
+        return cfg.rng == WLX_RNG_PHILOX ? kernels.sample_ph : kernels.sample_xo;
+    }
+
+    int launch_sample(uint64_t request, uint32_t flags)
+    {
+        if (trace_cap) flags |= kTrace;
+        hipLaunchKernelGGL(sample_fn(), dim3(walkers), dim3(kWave), lds_sample, stream, args(),
+                           request, flags);
+        REWL_HIP(hipGetLastError());
+        return WLX_OK;
+    }
+
+    int launch_exchange(uint32_t gated, uint64_t epoch)
+    {
+        hipLaunchKernelGGL(exchange_kernel, dim3(1), dim3(kExchangeBlock), 0, stream, args(),
+                           gated, epoch);
+        REWL_HIP(hipGetLastError());
+        return WLX_OK;
+    }
+
+    int check_escape()
+    {
+        Globals g{};
+        REWL_HIP(hipMemcpyAsync(&g, d_glob, sizeof g, hipMemcpyDeviceToHost, stream));
+        REWL_HIP(hipStreamSynchronize(stream));
+        if (g.escaped) return set_error(WLX_RUNTIME_ERROR, "Walker escaped its energy window");
+        return WLX_OK;
+    }
+};
+
+namespace {
+
+int require_engine(const wlx_engine *e, bool initialized = true)
+{
+    if (!e) return set_error(WLX_INVALID_ARGUMENT, "Null engine");
+    if (initialized && !e->initialized)
+        return set_error(WLX_LOGIC_ERROR, "Engine walkers are not initialized");
+    return WLX_OK;
+}
+
+int require_walker(const wlx_engine *e, uint64_t w)
+{
+    if (int status = require_engine(e)) return status;
+    if (w >= e->walkers) return set_error(WLX_OUT_OF_RANGE, "Walker index out of range");
+    return WLX_OK;
+}
+
+// WlParameters validation (wang_landau.cpp:74-90)
+int validate_parameters(const wlx_params &p)
+{
+    if (!(p.flatness > 0.0 && p.flatness <= 1.0) || !(p.final_factor > 0.0) ||
+        p.check_interval_attempts == 0 || p.initialization_max_attempts == 0 ||
+        !(p.initialization_target_fraction > 0.0 && p.initialization_target_fraction <= 1.0) ||
+        !std::isfinite(p.initialization_temperature_fraction) ||
+        !(p.initialization_temperature_fraction > 0.0) ||
+        p.initialization_stall_attempts_per_spin == 0 ||
+        !std::isfinite(p.initialization_temperature_multiplier) ||
+        !(p.initialization_temperature_multiplier > 1.0) ||
+        !std::isfinite(p.initialization_max_temperature_fraction) ||
+        p.initialization_max_temperature_fraction < p.initialization_temperature_fraction)
+        return set_error(WLX_INVALID_ARGUMENT, "Invalid WL parameters");
+    return WLX_OK;
+}
+
+// partition_windows (wang_landau.cpp:40-61)
+int partition_windows(uint64_t bins, uint64_t count, double overlap, std::vector<uint64_t> &be)
+{
+    if (bins == 0 || count == 0 || count > bins || overlap < 0.0 || overlap >= 1.0)
+        return set_error(WLX_INVALID_ARGUMENT, "Invalid energy-window partition");
+    be.assign(2 * count, 0);
+    if (count == 1) {
+        be[1] = bins;
+        return WLX_OK;
+    }
+    const double denominator = 1.0 + double(count - 1) * (1.0 - overlap);
+    uint64_t size = uint64_t(std::ceil(double(bins) / denominator));
+    size = std::max<uint64_t>(2, std::min(size, bins));
+    const uint64_t span = bins - size;
+    for (uint64_t w = 0; w < count; ++w) {
+        const uint64_t begin = uint64_t(std::llround(double(w * span) / double(count - 1)));
+        be[2 * w] = begin;
+        be[2 * w + 1] = begin + size;
+    }
+    be[2 * count - 1] = bins;
+    for (uint64_t w = 1; w < count; ++w)
+        if (be[2 * w] >= be[2 * w - 1])
+            return set_error(WLX_INVALID_ARGUMENT,
+                             "Energy windows do not overlap; increase overlap or bins");
+    return WLX_OK;
+}
+
+std::string format17(double x)
+{
+    char buffer[64];
+    std::snprintf(buffer, sizeof buffer, "%.17g", x);
+    return buffer;
+}
+
+} // namespace
+
+extern "C" {
+
+int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
+{
+    if (!sys || !c || !out) return set_error(WLX_INVALID_ARGUMENT, "Null argument");
+    *out = nullptr;
+    // EnergyGrid::validate / bins (wang_landau.cpp:13-25)
+    if (!std::isfinite(c->emin) || !std::isfinite(c->emax) || !std::isfinite(c->bin_width) ||
+        !(c->emax > c->emin) || !(c->bin_width > 0.0))
+        return set_error(WLX_INVALID_ARGUMENT, "Invalid energy grid");
+    const double count = (c->emax - c->emin) / c->bin_width;
+    if (std::abs(count - std::round(count)) > 1e-9 * std::max(1.0, count))
+        return set_error(WLX_INVALID_ARGUMENT,
+                         "Energy range must be an integer multiple of bin width");
+    const uint64_t bins = uint64_t(std::llround(count));
+    // RunConfig::validate subset (io.cpp:243-268)
+    if (c->windows == 0 || c->walkers_per_window == 0 || c->overlap < 0.0 || c->overlap >= 1.0 ||
+        c->exchange_interval_attempts == 0 || c->wl.check_interval_attempts == 0)
+        return set_error(WLX_INVALID_ARGUMENT, "Invalid run configuration");
+    if (c->windows > bins) return set_error(WLX_INVALID_ARGUMENT, "More windows than energy bins");
+    if (c->rng != WLX_RNG_XOSHIRO && c->rng != WLX_RNG_PHILOX)
+        return set_error(WLX_INVALID_ARGUMENT, "Unknown RNG mode");
+    if (c->fields != WLX_FIELDS_FP64)
+        return set_error(WLX_INVALID_ARGUMENT, "Unknown field precision mode");
+    if (int status = validate_parameters(c->wl)) return status;
+    auto e = std::make_unique<wlx_engine>();
+    e->sys = sys;
+    e->cfg = *c;
+    e->cfg.explicit_windows = nullptr;
+    e->bins = bins;
+    e->grid = {c->emin, c->emax, c->bin_width, int64_t(bins)};
+    if (c->explicit_windows) {
+        // per-walker window check of the WangLandauWalker constructor (wang_landau.cpp:72-73);
+        // RunConfig's range/ordering rules (io.cpp:257-267) are the host shim's
+        e->window_be.assign(c->explicit_windows, c->explicit_windows + 2 * c->windows);
+        for (uint64_t w = 0; w < c->windows; ++w)
+            if (e->window_be[2 * w] >= e->window_be[2 * w + 1] || e->window_be[2 * w + 1] > bins)
+                return set_error(WLX_INVALID_ARGUMENT, "Invalid walker energy window");
+    } else if (int status = partition_windows(bins, c->windows, c->overlap, e->window_be)) {
+        return status;
+    }
+    if (c->windows * c->walkers_per_window > (uint64_t(1) << 24))
+        return set_error(WLX_INVALID_ARGUMENT, "Too many walkers");
+    e->windows = uint32_t(c->windows);
+    e->per_window = uint32_t(c->walkers_per_window);
+    e->walkers = e->windows * e->per_window;
+    uint64_t ws_max = 0;
+    for (uint32_t w = 0; w < e->windows; ++w)
+        ws_max = std::max(ws_max, e->window_be[2 * w + 1] - e->window_be[2 * w]);
+    e->ws_stride = uint32_t((ws_max + 7) / 8 * 8);
+    e->kernels = kernels_for(sys->npl);
+    e->lds_sample = bins_lds_bytes(e->ws_stride);
+    e->lds_init = e->lds_sample + sys->np * sizeof(double);
+    if (e->lds_init > 160 * 1024)
+        return set_error(WLX_INVALID_ARGUMENT, "Energy windows too wide for LDS residency");
+    if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.sample_xo),
+                                   e->lds_sample))
+        return status;
+    if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.sample_ph),
+                                   e->lds_sample))
+        return status;
+    if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.init), e->lds_init))
+        return status;
+    if (int status = set_lds_limit(reinterpret_cast<const void *>(walker_op_kernel),
+                                   e->lds_sample))
+        return status;
+
+    const size_t W = e->walkers, np = sys->np, S = e->ws_stride;
+    REWL_HIP(hipStreamCreate(&e->stream));
+    if (int st = device_alloc(&e->d_state, W)) return st;
+    if (int st = device_alloc(&e->d_spins, W * np)) return st;
+    if (int st = device_alloc(&e->d_fields, W * np)) return st;
+    if (int st = device_alloc(&e->d_energy, W)) return st;
+    if (int st = device_alloc(&e->d_logg, W * S)) return st;
+    if (int st = device_alloc(&e->d_hist, W * S)) return st;
+    if (int st = device_alloc(&e->d_active, W * S)) return st;
+    if (int st = device_alloc(&e->d_refine, W * S)) return st;
+    if (int st = device_alloc(&e->d_glob, 1)) return st;
+    if (int st = device_alloc(&e->d_report, W)) return st;
+    if (int st = device_alloc(&e->d_op, 8)) return st;
+    std::vector<WalkerState> states(W);
+    for (uint32_t k = 0; k < W; ++k) {
+        WalkerState &s = states[k];
+        std::memset(&s, 0, sizeof s);
+        const uint32_t window = k / e->per_window;
+        s.begin = uint32_t(e->window_be[2 * window]);
+        s.end = uint32_t(e->window_be[2 * window + 1]);
+        s.slot = k;
+        s.factor = 1.0;
+    }
+    REWL_HIP(hipMemcpyAsync(e->d_state, states.data(), W * sizeof(WalkerState),
+                            hipMemcpyHostToDevice, e->stream));
+    REWL_HIP(hipMemsetAsync(e->d_spins, 1, W * np, e->stream));
+    REWL_HIP(hipMemsetAsync(e->d_fields, 0, W * np * sizeof(double), e->stream));
+    REWL_HIP(hipMemsetAsync(e->d_glob, 0, sizeof(Globals), e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    *out = e.release();
+    return WLX_OK;
+}
+
+int wlx_engine_destroy(wlx_engine *e)
+{
+    if (!e) return WLX_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    void *buffers[] = {e->d_state, e->d_spins,  e->d_fields, e->d_energy, e->d_logg,
+                       e->d_hist,  e->d_active, e->d_refine, e->d_tsite,  e->d_tbin,
+                       e->d_tflag, e->d_glob,   e->d_report, e->d_op};
+    for (void *p : buffers)
+        if (p) (void)hipFree(p);
+    for (hipEvent_t ev : e->events) (void)hipEventDestroy(ev);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return WLX_OK;
+}
+
+int wlx_engine_walkers(const wlx_engine *e, uint64_t *walkers, uint64_t *spins, uint64_t *bins)
+{
+    if (!e) return set_error(WLX_INVALID_ARGUMENT, "Null engine");
+    if (walkers) *walkers = e->walkers;
+    if (spins) *spins = e->sys->n;
+    if (bins) *bins = e->bins;
+    return WLX_OK;
+}
+
+int wlx_engine_initialize(wlx_engine *e, const int8_t *initial, const uint8_t *supplied)
+{
+    if (int status = require_engine(e, false)) return status;
+    const wlx_system *s = e->sys;
+    const size_t W = e->walkers;
+    int8_t *d_initial = nullptr;
+    uint8_t *d_supplied = nullptr;
+    if (initial && supplied) {
+        if (int st = device_alloc(&d_initial, W * s->n)) return st;
+        if (int st = device_alloc(&d_supplied, W)) return st;
+        REWL_HIP(hipMemcpyAsync(d_initial, initial, W * s->n, hipMemcpyHostToDevice, e->stream));
+        REWL_HIP(hipMemcpyAsync(d_supplied, supplied, W, hipMemcpyHostToDevice, e->stream));
+    }
+    // every walker starts in its own slot
+    std::vector<WalkerState> states(W);
+    REWL_HIP(hipMemcpyAsync(states.data(), e->d_state, W * sizeof(WalkerState),
+                            hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    for (uint32_t k = 0; k < W; ++k) states[k].slot = k;
+    REWL_HIP(hipMemcpyAsync(e->d_state, states.data(), W * sizeof(WalkerState),
+                            hipMemcpyHostToDevice, e->stream));
+    REWL_HIP(hipMemsetAsync(e->d_glob, 0, sizeof(Globals), e->stream));
+    InitParams ip{e->cfg.wl.initialization_max_attempts,
+                  e->cfg.wl.initialization_target_fraction,
+                  e->cfg.wl.initialization_temperature_fraction,
+                  e->cfg.wl.initialization_stall_attempts_per_spin,
+                  e->cfg.wl.initialization_temperature_multiplier,
+                  e->cfg.wl.initialization_max_temperature_fraction};
+    hipLaunchKernelGGL(e->kernels.init, dim3(e->walkers), dim3(kWave), e->lds_init, e->stream,
+                       e->args(), d_initial, d_supplied, ip, e->d_report);
+    REWL_HIP(hipGetLastError());
+    std::vector<InitReport> reports(W);
+    REWL_HIP(hipMemcpyAsync(reports.data(), e->d_report, W * sizeof(InitReport),
+                            hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    if (d_initial) (void)hipFree(d_initial);
+    if (d_supplied) (void)hipFree(d_supplied);
+    e->epochs_run = 0;
+    e->chunk = 1;
+    for (uint32_t k = 0; k < W; ++k) {
+        const InitReport &r = reports[k];
+        if (r.status == 0) continue;
+        std::string what;
+        if (r.status == 2) {
+            what = "Spins must be +/-1";
+        } else {
+            const uint64_t begin = e->window_be[2 * (k / e->per_window)];
+            const uint64_t end = e->window_be[2 * (k / e->per_window) + 1];
+            const wlx_params &p = e->cfg.wl;
+            const double lower = e->cfg.emin + double(begin) * e->cfg.bin_width;
+            const double upper = e->cfg.emin + double(end) * e->cfg.bin_width;
+            const double span = upper - lower, target = 0.5 * (lower + upper);
+            const bool sup = initial && supplied && supplied[k];
+            const double half = 0.5 * (sup ? 1.0 : p.initialization_target_fraction) * span;
+            const uint64_t stall = p.initialization_stall_attempts_per_spin > ~uint64_t(0) / s->n
+                                       ? ~uint64_t(0)
+                                       : p.initialization_stall_attempts_per_spin * s->n;
+            const int64_t cb = grid_index(e->grid, r.closest);
+            what = "Target Metropolis initialization did not reach target band [" +
+                   format17(target - half) + ", " + format17(target + half) +
+                   "] for energy window bins [" + std::to_string(begin) + ", " +
+                   std::to_string(end) + "), energy bounds [" + format17(lower) + ", " +
+                   format17(upper) + (end == e->bins ? "]" : ")") + " after " +
+                   std::to_string(p.initialization_max_attempts) +
+                   " attempts; reached energy range [" + format17(r.min_reached) + ", " +
+                   format17(r.max_reached) + "], closest energy " + format17(r.closest) +
+                   (cb >= 0 ? " (bin " + std::to_string(cb) + ")"
+                            : std::string(" (outside the energy grid)")) +
+                   ", distance to target band " +
+                   format17(std::max(0.0, r.closest_distance - half)) +
+                   "; adaptive search used " + std::to_string(r.increases) +
+                   " temperature increases up to " + format17(r.highest) + " and " +
+                   std::to_string(r.restarts) + " random restarts (stall interval " +
+                   std::to_string(stall) + " attempts)";
+        }
+        return set_error(WLX_RUNTIME_ERROR,
+                         "Walker initialization failed: window " +
+                             std::to_string(k / e->per_window) + ", local walker " +
+                             std::to_string(k % e->per_window) + ": " + what);
+    }
+    e->initialized = true;
+    return WLX_OK;
+}
+
+int wlx_engine_sample(wlx_engine *e, uint64_t attempts)
+{
+    if (int status = require_engine(e)) return status;
+    if (int status = e->launch_sample(attempts, 0)) return status;
+    return e->check_escape();
+}
+
+int wlx_engine_check(wlx_engine *e)
+{
+    if (int status = require_engine(e)) return status;
+    if (int status = e->launch_sample(0, kCheck)) return status;
+    return e->check_escape();
+}
+
+int wlx_engine_exchange(wlx_engine *e, uint64_t epoch, uint64_t *attempted, uint64_t *accepted)
+{
+    if (int status = require_engine(e)) return status;
+    Globals before{}, after{};
+    REWL_HIP(hipMemcpyAsync(&before, e->d_glob, sizeof before, hipMemcpyDeviceToHost, e->stream));
+    if (int status = e->launch_exchange(0, epoch)) return status;
+    REWL_HIP(hipMemcpyAsync(&after, e->d_glob, sizeof after, hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    if (attempted) *attempted = after.exchange_attempted - before.exchange_attempted;
+    if (accepted) *accepted = after.exchange_accepted - before.exchange_accepted;
+    return WLX_OK;
+}
+
+int wlx_engine_status(wlx_engine *e, wlx_run_status *st)
+{
+    if (int status = require_engine(e, false)) return status;
+    if (!st) return set_error(WLX_INVALID_ARGUMENT, "Null argument");
+    Globals g{};
+    std::vector<WalkerState> states(e->walkers);
+    REWL_HIP(hipMemcpyAsync(&g, e->d_glob, sizeof g, hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipMemcpyAsync(states.data(), e->d_state, states.size() * sizeof(WalkerState),
+                            hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    std::memset(st, 0, sizeof *st);
+    st->epoch = g.epoch;
+    st->done = g.done;
+    st->exchange_attempted = g.exchange_attempted;
+    st->exchange_accepted = g.exchange_accepted;
+    st->ambiguous_decisions = g.exchange_ambiguous;
+    bool converged = true;
+    for (const auto &s : states) {
+        st->attempted += s.attempted;
+        st->accepted += s.accepted;
+        st->forced_accepted += s.forced;
+        st->ambiguous_decisions += s.ambiguous;
+        converged &= s.stage == WLX_STAGE_FROZEN;
+    }
+    st->converged = converged ? 1 : 0;
+    return WLX_OK;
+}
+
+int wlx_engine_run(wlx_engine *e, uint64_t max_epochs, wlx_run_status *status)
+{
+    if (int st = require_engine(e)) return st;
+    uint64_t remaining = max_epochs;
+    constexpr uint64_t kMaxChunk = 64;
+    while (remaining > 0) {
+        const uint64_t n = std::min(remaining, e->chunk);
+        if (e->timing && e->events.size() < 2 * n) {
+            while (e->events.size() < 2 * kMaxChunk) {
+                hipEvent_t ev;
+                REWL_HIP(hipEventCreate(&ev));
+                e->events.push_back(ev);
+            }
+        }
+        for (uint64_t k = 0; k < n; ++k) {
+            if (e->timing) REWL_HIP(hipEventRecord(e->events[2 * k], e->stream));
+            if (int st = e->launch_sample(0, kGate | kCheck)) return st;
+            if (e->timing) REWL_HIP(hipEventRecord(e->events[2 * k + 1], e->stream));
+            if (int st = e->launch_exchange(1, 0)) return st;
+        }
+        Globals g{};
+        REWL_HIP(hipMemcpyAsync(&g, e->d_glob, sizeof g, hipMemcpyDeviceToHost, e->stream));
+        REWL_HIP(hipStreamSynchronize(e->stream));
+        if (e->timing)
+            for (uint64_t k = 0; k < n; ++k) {
+                float ms = 0.0f;
+                REWL_HIP(hipEventElapsedTime(&ms, e->events[2 * k], e->events[2 * k + 1]));
+                e->sample_ms += ms;
+                ++e->launches;
+            }
+        remaining -= n;
+        if (g.escaped) return set_error(WLX_RUNTIME_ERROR, "Walker escaped its energy window");
+        if (g.done) break;
+        e->chunk = std::min(kMaxChunk, e->chunk * 2);
+    }
+    if (status) return wlx_engine_status(e, status);
+    return WLX_OK;
+}
+
+int wlx_engine_read_walker(wlx_engine *e, uint64_t w, wlx_walker_info *info, int8_t *spins,
+                           double *fields, double *log_g, uint64_t *histogram, uint8_t *active,
+                           uint8_t *refinement_active)
+{
+    if (int status = require_walker(e, w)) return status;
+    const wlx_system *s = e->sys;
+    WalkerState st{};
+    REWL_HIP(hipMemcpyAsync(&st, e->d_state + w, sizeof st, hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    const uint32_t ws = st.end - st.begin;
+    const size_t wb = size_t(w) * e->ws_stride;
+    double energy = 0.0;
+    std::vector<int8_t> sp(s->np);
+    std::vector<double> fl(s->np), lg(ws);
+    std::vector<uint64_t> hs(ws);
+    std::vector<uint8_t> ac(ws), rf(ws);
+    REWL_HIP(hipMemcpyAsync(&energy, e->d_energy + st.slot, sizeof energy, hipMemcpyDeviceToHost,
+                            e->stream));
+    REWL_HIP(hipMemcpyAsync(sp.data(), e->d_spins + size_t(st.slot) * s->np, s->np,
+                            hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipMemcpyAsync(fl.data(), e->d_fields + size_t(st.slot) * s->np,
+                            s->np * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipMemcpyAsync(lg.data(), e->d_logg + wb, ws * sizeof(double), hipMemcpyDeviceToHost,
+                            e->stream));
+    REWL_HIP(hipMemcpyAsync(hs.data(), e->d_hist + wb, ws * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipMemcpyAsync(ac.data(), e->d_active + wb, ws, hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipMemcpyAsync(rf.data(), e->d_refine + wb, ws, hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    if (info) {
+        std::memset(info, 0, sizeof *info);
+        info->walker_id = w;
+        info->window = w / e->per_window;
+        info->window_begin = st.begin;
+        info->window_end = st.end;
+        info->energy = energy;
+        info->factor = st.factor;
+        info->attempted = st.attempted;
+        info->accepted = st.accepted;
+        info->forced_accepted = st.forced;
+        info->last_accepted_attempt = st.last_accepted;
+        info->stage = st.stage;
+        info->active_bins = st.active_count;
+        info->refinement_active_bins = st.refine_count;
+        for (int k = 0; k < 4; ++k) info->rng_state[k] = st.rng[k];
+        info->last_flatness_check = st.last_check;
+        info->last_flatness = st.last_flatness;
+    }
+    if (spins) std::memcpy(spins, sp.data(), s->n);
+    if (fields) std::memcpy(fields, fl.data(), s->n * sizeof(double));
+    if (log_g) {
+        std::fill(log_g, log_g + e->bins, 0.0);
+        std::copy(lg.begin(), lg.end(), log_g + st.begin);
+    }
+    if (histogram) {
+        std::fill(histogram, histogram + e->bins, uint64_t(0));
+        std::copy(hs.begin(), hs.end(), histogram + st.begin);
+    }
+    if (active) {
+        std::fill(active, active + e->bins, uint8_t(0));
+        std::copy(ac.begin(), ac.end(), active + st.begin);
+    }
+    if (refinement_active) {
+        std::fill(refinement_active, refinement_active + e->bins, uint8_t(0));
+        std::copy(rf.begin(), rf.end(), refinement_active + st.begin);
+    }
+    return WLX_OK;
+}
+
+int wlx_engine_write_walker(wlx_engine *e, uint64_t w, const wlx_walker_info *info,
+                            const int8_t *spins, const double *fields, const double *log_g,
+                            const uint64_t *histogram, const uint8_t *active)
+{
+    if (int status = require_walker(e, w)) return status;
+    const wlx_system *s = e->sys;
+    // restore validation (wang_landau.cpp:421-430)
+    if (!info || info->walker_id != w || !spins || !fields || !log_g || !histogram || !active)
+        return set_error(WLX_INVALID_ARGUMENT, "Checkpoint is incompatible with walker");
+    if (info->last_accepted_attempt > info->attempted)
+        return set_error(WLX_INVALID_ARGUMENT, "Checkpoint has an invalid last accepted attempt");
+    if (info->forced_accepted > info->accepted)
+        return set_error(WLX_INVALID_ARGUMENT,
+                         "Checkpoint has an invalid forced acceptance count");
+    if (!e->cfg.wl.inverse_time_enabled && info->stage == WLX_STAGE_INVERSE_TIME)
+        return set_error(WLX_INVALID_ARGUMENT, "Checkpoint uses disabled inverse-time refinement");
+    for (uint64_t i = 0; i < s->n; ++i)
+        if (spins[i] != 1 && spins[i] != -1)
+            return set_error(WLX_INVALID_ARGUMENT, "Spins must be +/-1");
+    WalkerState st{};
+    REWL_HIP(hipMemcpyAsync(&st, e->d_state + w, sizeof st, hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    const uint32_t ws = st.end - st.begin;
+    std::vector<uint8_t> ac(ws), rf(ws, 0);
+    uint32_t active_count = 0, refine_count = 0;
+    for (uint32_t b = 0; b < ws; ++b) {
+        ac[b] = active[st.begin + b] != 0 ? 1 : 0;
+        active_count += ac[b];
+        if (e->cfg.wl.nalivaiko_mod) {
+            rf[b] = histogram[st.begin + b] != 0 ? 1 : 0;
+            refine_count += rf[b];
+        }
+    }
+    st.factor = info->factor;
+    st.attempted = info->attempted;
+    st.accepted = info->accepted;
+    st.forced = info->forced_accepted;
+    st.last_accepted = info->last_accepted_attempt;
+    st.stage = uint32_t(info->stage);
+    st.active_count = active_count;
+    st.refine_count = refine_count;
+    for (int k = 0; k < 4; ++k) st.rng[k] = info->rng_state[k];
+    st.escaped = 0;
+    std::vector<int8_t> sp(s->np, 1);
+    std::vector<double> fl(s->np, 0.0);
+    std::memcpy(sp.data(), spins, s->n);
+    std::memcpy(fl.data(), fields, s->n * sizeof(double));
+    const size_t wb = size_t(w) * e->ws_stride;
+    const double energy = info->energy;
+    REWL_HIP(hipMemcpyAsync(e->d_state + w, &st, sizeof st, hipMemcpyHostToDevice, e->stream));
+    REWL_HIP(hipMemcpyAsync(e->d_energy + st.slot, &energy, sizeof energy, hipMemcpyHostToDevice,
+                            e->stream));
+    REWL_HIP(hipMemcpyAsync(e->d_spins + size_t(st.slot) * s->np, sp.data(), s->np,
+                            hipMemcpyHostToDevice, e->stream));
+    REWL_HIP(hipMemcpyAsync(e->d_fields + size_t(st.slot) * s->np, fl.data(),
+                            s->np * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    REWL_HIP(hipMemcpyAsync(e->d_logg + wb, log_g + st.begin, ws * sizeof(double),
+                            hipMemcpyHostToDevice, e->stream));
+    REWL_HIP(hipMemcpyAsync(e->d_hist + wb, histogram + st.begin, ws * sizeof(uint64_t),
+                            hipMemcpyHostToDevice, e->stream));
+    REWL_HIP(hipMemcpyAsync(e->d_active + wb, ac.data(), ws, hipMemcpyHostToDevice, e->stream));
+    REWL_HIP(hipMemcpyAsync(e->d_refine + wb, rf.data(), ws, hipMemcpyHostToDevice, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    // re-verify energy and fields (wang_landau.cpp:459-465)
+    std::vector<double> exact(s->n);
+    double exact_energy = 0.0;
+    if (int status = wlx_system_fields(s, spins, 1, exact.data(), &exact_energy)) return status;
+    if (std::abs(exact_energy - energy) > 1e-9 * std::max(1.0, std::abs(exact_energy)))
+        return set_error(WLX_RUNTIME_ERROR, "Checkpoint energy does not match spin configuration");
+    for (uint64_t i = 0; i < s->n; ++i)
+        if (std::abs(exact[i] - fields[i]) > 1e-9 * std::max(1.0, std::abs(exact[i])))
+            return set_error(WLX_RUNTIME_ERROR,
+                             "Checkpoint local fields do not match spin configuration");
+    return WLX_OK;
+}
+
+static int walker_op(wlx_engine *e, uint64_t w, uint32_t op, uint64_t out[8])
+{
+    if (int status = require_walker(e, w)) return status;
+    hipLaunchKernelGGL(walker_op_kernel, dim3(1), dim3(kWave), e->lds_sample, e->stream, e->args(),
+                       uint32_t(w), op, e->d_op);
+    REWL_HIP(hipGetLastError());
+    REWL_HIP(hipMemcpyAsync(out, e->d_op, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    return WLX_OK;
+}
+
+int wlx_engine_histogram_statistics(wlx_engine *e, uint64_t w, wlx_histogram_stats *out)
+{
+    if (!out) return set_error(WLX_INVALID_ARGUMENT, "Null argument");
+    uint64_t r[8];
+    if (int status = walker_op(e, w, 0, r)) return status;
+    out->active_bins = r[0];
+    out->covered_bins = r[1];
+    out->minimum = r[2];
+    std::memcpy(&out->mean, &r[3], 8);
+    std::memcpy(&out->min_over_mean, &r[4], 8);
+    std::memcpy(&out->coverage, &r[5], 8);
+    return WLX_OK;
+}
+
+int wlx_engine_ready(wlx_engine *e, uint64_t w, uint64_t *ready)
+{
+    if (!ready) return set_error(WLX_INVALID_ARGUMENT, "Null argument");
+    uint64_t r[8];
+    if (int status = walker_op(e, w, 1, r)) return status;
+    *ready = r[6];
+    return WLX_OK;
+}
+
+int wlx_engine_next_iteration(wlx_engine *e, uint64_t w)
+{
+    uint64_t r[8];
+    if (int status = walker_op(e, w, 2, r)) return status;
+    if (!r[6]) return set_error(WLX_LOGIC_ERROR, "WL iteration is not complete");
+    return WLX_OK;
+}
+
+int wlx_engine_trace(wlx_engine *e, uint64_t capacity)
+{
+    if (int status = require_engine(e, false)) return status;
+    (void)hipFree(e->d_tsite);
+    (void)hipFree(e->d_tbin);
+    (void)hipFree(e->d_tflag);
+    e->d_tsite = nullptr;
+    e->d_tbin = nullptr;
+    e->d_tflag = nullptr;
+    e->trace_cap = capacity;
+    if (capacity) {
+        const size_t total = size_t(e->walkers) * capacity;
+        if (int st = device_alloc(&e->d_tsite, total)) return st;
+        if (int st = device_alloc(&e->d_tbin, total)) return st;
+        if (int st = device_alloc(&e->d_tflag, total)) return st;
+    }
+    // reset per-walker trace counters
+    std::vector<WalkerState> states(e->walkers);
+    REWL_HIP(hipMemcpyAsync(states.data(), e->d_state, states.size() * sizeof(WalkerState),
+                            hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    for (auto &s : states) s.trace_count = 0;
+    REWL_HIP(hipMemcpyAsync(e->d_state, states.data(), states.size() * sizeof(WalkerState),
+                            hipMemcpyHostToDevice, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    return WLX_OK;
+}
+
+int wlx_engine_read_trace(wlx_engine *e, uint64_t w, uint64_t *count, uint32_t *sites,
+                          uint32_t *bins, uint8_t *flags)
+{
+    if (int status = require_walker(e, w)) return status;
+    if (!count) return set_error(WLX_INVALID_ARGUMENT, "Null argument");
+    WalkerState st{};
+    REWL_HIP(hipMemcpyAsync(&st, e->d_state + w, sizeof st, hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    *count = st.trace_count;
+    const size_t base = size_t(w) * e->trace_cap;
+    if (st.trace_count) {
+        if (sites)
+            REWL_HIP(hipMemcpyAsync(sites, e->d_tsite + base, st.trace_count * sizeof(uint32_t),
+                                    hipMemcpyDeviceToHost, e->stream));
+        if (bins)
+            REWL_HIP(hipMemcpyAsync(bins, e->d_tbin + base, st.trace_count * sizeof(uint32_t),
+                                    hipMemcpyDeviceToHost, e->stream));
+        if (flags)
+            REWL_HIP(hipMemcpyAsync(flags, e->d_tflag + base, st.trace_count,
+                                    hipMemcpyDeviceToHost, e->stream));
+        REWL_HIP(hipStreamSynchronize(e->stream));
+    }
+    return WLX_OK;
+}
+
+int wlx_engine_set_timing(wlx_engine *e, int enable)
+{
+    if (!e) return set_error(WLX_INVALID_ARGUMENT, "Null engine");
+    e->timing = enable != 0;
+    e->sample_ms = 0.0;
+    e->launches = 0;
+    return WLX_OK;
+}
+
+int wlx_engine_kernel_time(wlx_engine *e, double *sample_ms, uint64_t *launches)
+{
+    if (!e) return set_error(WLX_INVALID_ARGUMENT, "Null engine");
+    if (sample_ms) *sample_ms = e->launches ? e->sample_ms / double(e->launches) : 0.0;
+    if (launches) *launches = e->launches;
+    return WLX_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,281 @@
+"""Parity of the HIP engine (kernels a and b) with the C oracle and the golden fixtures.
+
+Bar: bit-exact for couplings, fields, energies, per-attempt traces, walker states, estimators
+and run counters (Xoshiro mode = the reference's own arithmetic and stream). Every call goes
+through the C ABI (librewl.so).
+"""
+from __future__ import annotations
+
+import json
+
+import numpy as np
+import pytest
+
+import test_oracle_pinning as op
+from harness import (GOLDEN, assert_walkers_equal, c1_config, c1_geometry, c2_config)
+from replica_exchange_wang_landau_amd import abi
+from replica_exchange_wang_landau_amd.engine import (Engine, InvalidArgument, LogicError, RunConfig,
+                                                     System, WlParameters, simple_cubic)
+
+pytestmark = pytest.mark.gpu
+
+
+def pair_systems(gpu, orc, *geometry, **kw):
+    return System(*geometry, backend=gpu, **kw), System(*geometry, backend=orc, **kw)
+
+
+def pair_engines(gpu, orc, geometry, cfg, initialize=True, **kw):
+    sg, so = pair_systems(gpu, orc, *geometry, **kw)
+    eg, eo = Engine(sg, cfg), Engine(so, cfg)
+    if initialize:
+        eg.initialize()
+        eo.initialize()
+    return eg, eo
+
+
+# ---- kernel (a) ---------------------------------------------------------------------------
+def test_dense_couplings_bit_exact(gpu, orc):
+    fx = json.loads((GOLDEN / "couplings.json").read_text())
+    g, o = pair_systems(gpu, orc, *c1_geometry())
+    assert g.dense().tobytes() == o.dense().tobytes() == op.fromhex(fx["c1_dense"]).tobytes()
+    g, o = pair_systems(gpu, orc, *simple_cubic(8, 8, 8, 1.0, (0, 0, 1), True))
+    assert op.digest(g.dense()) == fx["c2_dense_sha256"]
+    g, o = pair_systems(gpu, orc, *simple_cubic(16, 16, 16, 1.0, (0, 0, 1), True))
+    assert g.dense().tobytes() == o.dense().tobytes()
+
+
+def test_random_geometry_dense_and_csr_bit_exact(gpu, orc):
+    fx = json.loads((GOLDEN / "couplings.json").read_text())["random_slab"]
+    pos = np.array([[float.fromhex(v) for v in p] for p in fx["positions"]])
+    ax = np.array([[float.fromhex(v) for v in a] for a in fx["axes"]])
+    box = np.array(fx["box"])
+    g = System(pos, ax, box, True, backend=gpu)
+    assert op.digest(g.dense()) == fx["dense_sha256"]
+    gc, oc = pair_systems(gpu, orc, pos, ax, box, True, cutoff=2.0)
+    assert op.digest(gc.dense()) == fx["csr2_dense_view_sha256"]
+    for a, b in zip(gc.csr(), oc.csr()):
+        assert a.tobytes() == b.tobytes()
+    # uneven N, tilted axes, anisotropic box, both backends
+    rng = np.random.default_rng(11)
+    for n, cutoff in [(97, 0.0), (333, 3.0), (1000, 2.5)]:
+        L = (n / 0.3) ** (1 / 3)
+        p = rng.uniform(0, L, size=(n, 3))
+        a = rng.normal(size=(n, 3))
+        a /= np.sqrt((a * a).sum(axis=1))[:, None]
+        g, o = pair_systems(gpu, orc, p, a, np.array([L, L * 1.1, L * 0.9]), True, cutoff=cutoff)
+        assert g.dense().tobytes() == o.dense().tobytes(), (n, cutoff)
+        if cutoff > 0:
+            for x, y in zip(g.csr(), o.csr()):
+                assert x.tobytes() == y.tobytes()
+
+
+def test_geometry_errors(gpu):
+    axes = np.array([[0, 0, 1.0], [0, 0, 1.0]])
+    with pytest.raises(InvalidArgument, match="Coincident dipole sites"):
+        System(np.zeros((2, 3)), axes, [2, 2, 2], True, backend=gpu)
+    with pytest.raises(InvalidArgument, match="Ising axes must be normalized"):
+        System([[0, 0, 0], [1, 0, 0]], [[0, 0, 2.0], [0, 0, 1]], [2, 2, 2], True, backend=gpu)
+    with pytest.raises(InvalidArgument, match="Geometry contains no sites"):
+        System(np.zeros((0, 3)), np.zeros((0, 3)), [2, 2, 2], True, backend=gpu)
+
+
+def test_fields_and_energies_bit_exact(gpu, orc):
+    rng = np.random.default_rng(3)
+    for geometry in [c1_geometry(), simple_cubic(8, 8, 8, 1.0, (0, 0, 1), True),
+                     simple_cubic(5, 7, 3, 1.0, (0.6, 0, 0.8), True)]:
+        g, o = pair_systems(gpu, orc, *geometry)
+        spins = rng.choice(np.array([-1, 1], dtype=np.int8), size=(9, g.n))
+        fg, eg = g.fields(spins)
+        fo, eo = o.fields(spins)
+        assert fg.tobytes() == fo.tobytes() and eg.tobytes() == eo.tobytes()
+        # E = 1/2 sum s_i h_i against an independent FP64 recompute
+        J = g.dense()
+        e64 = 0.5 * np.einsum("ci,ij,cj->c", spins.astype(float), J, spins.astype(float))
+        np.testing.assert_allclose(eg, e64, rtol=1e-12, atol=1e-12)
+
+
+# ---- kernel (b): walkers -------------------------------------------------------------------
+def test_c1_walker_trace_bit_exact(gpu, orc):
+    fx = json.loads((GOLDEN / "walkers.json").read_text())["c1_walker0"]
+    eg, eo = pair_engines(gpu, orc, c1_geometry(), c1_config())
+    for e in (eg, eo):
+        e.trace(fx["attempts"])
+        e.sample(fx["attempts"])
+    for k in range(2):
+        sg, bg, fg = eg.read_trace(k)
+        so, bo, fo = eo.read_trace(k)
+        assert np.array_equal(sg, so) and np.array_equal(bg, bo) and np.array_equal(fg, fo)
+        assert_walkers_equal(eg, eo, k)
+    _, bins, flags = eg.read_trace(0)
+    assert np.packbits((flags & 1).astype(np.uint8)).tobytes().hex() == fx["accepted_bits"]
+    op.check_state(eg.read_walker(0), fx["state"])
+
+
+def test_forced_nalivaiko_traditional_bit_exact(gpu, orc):
+    fx = json.loads((GOLDEN / "walkers.json").read_text())["c2_walker5_forced_nalivaiko"]
+    eg, eo = op.forced_nalivaiko_engine(gpu), op.forced_nalivaiko_engine(orc)
+    for e in (eg, eo):
+        e.trace(fx["attempts"])
+        e.sample(fx["attempts"])
+        e.check()
+    _, bins, _ = eg.read_trace(5)
+    assert op.digest(bins.astype(np.int64)) == fx["bins_sha256"]
+    for k in range(eg.walkers):
+        assert_walkers_equal(eg, eo, k)
+    assert eg.read_walker(5).forced_accepted == fx["state"]["forced_accepted"] > 0
+
+
+def test_initialization_bit_exact_and_failure_text(gpu, orc):
+    pos, axes, box, per = simple_cubic(2, 1, 1, 1.0, (1, 0, 0), False)
+    cfg = RunConfig(emin=-2.5, emax=3.5, bin_width=1.0, windows=1, walkers_per_window=3,
+                    seed=123, wl=WlParameters(flatness=0.8, minimum_visits=1,
+                                              check_interval_attempts=10,
+                                              initialization_max_attempts=100))
+    eg, eo = pair_engines(gpu, orc, (pos, axes, box, per), cfg)
+    for k in range(3):
+        assert_walkers_equal(eg, eo, k)
+    g, o = op.init_failure_engine(gpu), op.init_failure_engine(orc)
+    with pytest.raises(RuntimeError) as gi:
+        g.initialize()
+    with pytest.raises(RuntimeError) as oi:
+        o.initialize()
+    assert str(gi.value) == str(oi.value)
+    for text in op.INIT_FAILURE_FRAGMENTS:
+        assert text in str(gi.value)
+
+
+def test_c2_initialization_and_supplied_starts_bit_exact(gpu, orc):
+    cfg = c2_config()
+    geo = simple_cubic(8, 8, 8, 1.0, (0, 0, 1), True)
+    eg, eo = pair_engines(gpu, orc, geo, cfg)
+    for k in range(eg.walkers):
+        assert_walkers_equal(eg, eo, k)
+    # supplied starts: in-window ones are kept, the rest run the target search
+    rng = np.random.default_rng(5)
+    spins = rng.choice(np.array([-1, 1], dtype=np.int8), size=(eg.walkers, 512))
+    supplied = (np.arange(eg.walkers) % 3 != 0).astype(np.uint8)
+    eg2, eo2 = pair_engines(gpu, orc, geo, cfg, initialize=False)
+    eg2.initialize(spins, supplied)
+    eo2.initialize(spins, supplied)
+    for k in range(eg.walkers):
+        assert_walkers_equal(eg2, eo2, k)
+
+
+@pytest.mark.parametrize("name", ["c1_prefix", "exact_regression_2x2x1",
+                                  "rewl_2x2x2_two_windows", "c2_prefix_20mcs"])
+def test_run_rewl_golden_on_gpu(gpu, name):
+    fx = next(r for r in json.loads((GOLDEN / "runs.json").read_text()) if r["name"] == name)
+    nx, ny, nz, periodic = fx["geometry"]
+    s = System(*simple_cubic(nx, ny, nz, 1.0, (0, 0, 1), periodic), backend=gpu)
+    e = Engine(s, op.config_from_fixture(fx))
+    e.initialize()
+    st = e.run()
+    op.check_run_against_fixture(e, fx, st)
+
+
+@pytest.mark.parametrize("rng_mode", [abi.RNG_XOSHIRO, abi.RNG_PHILOX])
+def test_c2_epochs_bit_exact(gpu, orc, rng_mode):
+    """C2 at full size: 128 walkers, 40 exchange epochs, every walker bit-identical."""
+    cfg = c2_config(max_attempts=512 * 40, rng=rng_mode)
+    eg, eo = pair_engines(gpu, orc, simple_cubic(8, 8, 8, 1.0, (0, 0, 1), True), cfg)
+    sg, so = eg.run(), eo.run()
+    for key in ("epoch", "done", "attempted", "accepted", "exchange_attempted",
+                "exchange_accepted"):
+        assert getattr(sg, key) == getattr(so, key), key
+    assert sg.attempted == 128 * 512 * 40 and sg.exchange_accepted > 0
+    for k in range(eg.walkers):
+        assert_walkers_equal(eg, eo, k)
+    # energies against an FP64 recompute from the spins (1e-10 relative)
+    spins = np.stack([eg.read_walker(k).spins for k in range(eg.walkers)])
+    _, exact = eg.system.fields(spins)
+    energies = np.array([eg.read_walker(k).energy for k in range(eg.walkers)])
+    np.testing.assert_allclose(energies, exact, rtol=1e-10, atol=1e-10)
+
+
+def test_exchange_schedule_bit_exact(gpu, orc):
+    cfg = c2_config(max_attempts=0)
+    eg, eo = pair_engines(gpu, orc, simple_cubic(8, 8, 8, 1.0, (0, 0, 1), True), cfg)
+    for epoch in range(6):
+        for e in (eg, eo):
+            e.sample(512)
+        assert eg.exchange(epoch) == eo.exchange(epoch)
+    for k in range(eg.walkers):
+        assert_walkers_equal(eg, eo, k)
+
+
+def test_statistics_iteration_and_restore_on_gpu(gpu, orc):
+    pos, axes, box, per = simple_cubic(1, 1, 1, 1.0, (0, 0, 1), False)
+    cfg = RunConfig(emin=-1, emax=1, bin_width=0.5, seed=11,
+                    wl=WlParameters(flatness=0.8, minimum_visits=1, final_factor=1e-8,
+                                    check_interval_attempts=1))
+    e = Engine(System(pos, axes, box, per, backend=gpu), cfg)
+    e.initialize()
+    snap = e.read_walker(0)
+    snap.active[:] = 0
+    snap.histogram[:] = 0
+    snap.active[0] = snap.active[1] = 1
+    snap.histogram[0], snap.histogram[1] = 80, 120
+    e.write_walker(0, snap)
+    st = e.statistics(0)
+    assert (st.active_bins, st.covered_bins, st.minimum) == (2, 2, 80)
+    assert st.mean == 100.0 and abs(st.min_over_mean - 0.8) <= 1e-14
+    snap = e.read_walker(0)
+    snap.active[2] = 1
+    e.write_walker(0, snap)
+    assert not e.ready(0)
+    with pytest.raises(LogicError, match="WL iteration is not complete"):
+        e.next_iteration(0)
+    snap = e.read_walker(0)
+    snap.info.attempted = 1000
+    snap.active[2] = 0
+    e.write_walker(0, snap)
+    e.next_iteration(0)
+    assert e.read_walker(0).factor == 0.5
+    snap = e.read_walker(0)
+    snap.info.stage = abi.STAGE_INVERSE_TIME
+    snap.info.factor = 0.002
+    e.write_walker(0, snap)
+    e.sample(1)
+    assert abs(e.read_walker(0).factor - 2.0 / 1001.0) <= 1e-14
+    e.sample(1)
+    assert abs(e.read_walker(0).factor - 3.0 / 1002.0) <= 1e-14
+    bad = e.read_walker(0)
+    bad.info.energy += 1.0
+    with pytest.raises(RuntimeError, match="Checkpoint energy does not match"):
+        e.write_walker(0, bad)
+
+
+def test_extended_quotient_means_bit_exact(gpu, orc):
+    """Histogram means use the reference's long double division; check awkward ratios."""
+    pos, axes, box, per = simple_cubic(1, 1, 1, 1.0, (0, 0, 1), False)
+    cfg = RunConfig(emin=-1, emax=1, bin_width=0.25, seed=3,
+                    wl=WlParameters(check_interval_attempts=1))
+    eg, eo = pair_engines(gpu, orc, (pos, axes, box, per), cfg)
+    rng = np.random.default_rng(1)
+    for _ in range(40):
+        snap = eg.read_walker(0)
+        snap.active[:] = 1
+        snap.histogram[:] = rng.integers(0, 2**40, size=len(snap.histogram), dtype=np.uint64)
+        snap.histogram[rng.integers(0, 8)] = rng.integers(1, 7)
+        eg.write_walker(0, snap)
+        eo.write_walker(0, snap)
+        a, b = eg.statistics(0), eo.statistics(0)
+        assert a.mean.hex() == b.mean.hex() and a.min_over_mean.hex() == b.min_over_mean.hex()
+
+
+def test_c1_converges_to_exact_dos(gpu, orc):
+    """C1 plain WL to ln f = 1e-6: GPU run bit-identical to the oracle's, DOS within 0.02 of
+    exact enumeration after mean-offset alignment over the occupied bins."""
+    cfg = c1_config()
+    eg, eo = pair_engines(gpu, orc, c1_geometry(), cfg)
+    sg = eg.run()
+    so = eo.run()
+    assert sg.converged and so.converged
+    assert (sg.attempted, sg.accepted, sg.epoch) == (so.attempted, so.accepted, so.epoch)
+    for k in range(2):
+        assert_walkers_equal(eg, eo, k)
+    lg, _, _, valid = op.fragment(eg, 0)
+    exact = json.loads((GOLDEN / "exact_c1.json").read_text())["counts"]
+    occupied = sorted(int(b) for b in exact)
+    diff = np.array([lg[b] - np.log(exact[str(b)]) for b in occupied])
+    assert np.max(np.abs(diff - diff.mean())) < 0.02
