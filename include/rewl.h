@@ -164,6 +164,13 @@ int wlx_system_read_csr(const wlx_system *system, uint64_t *offsets, uint64_t *c
  * in the reference's summation order. fields may be NULL. */
 int wlx_system_fields(const wlx_system *system, const int8_t *spins, uint64_t count,
                       double *fields, double *energies);
+/* The wl_run --pilot sweep (wl_run.cpp:84-104): from all spins +1, `attempts` unconditional
+ * single-spin flips drawn from Xoshiro256**(seed, 0), tracking the sampled energy range. */
+int wlx_system_pilot(const wlx_system *system, uint64_t seed, uint64_t attempts,
+                     double *minimum, double *maximum);
+/* Name of the device the library runs on and its compute-unit count. No reference
+ * counterpart. */
+int wlx_device_info(char *name, size_t capacity, uint64_t *compute_units);
 
 /* ---- engine: kernels (b) and the device-side replica exchange --------------------------- */
 
@@ -196,6 +203,8 @@ int wlx_engine_walkers(const wlx_engine *engine, uint64_t *walkers, uint64_t *sp
 int wlx_engine_read_walker(wlx_engine *engine, uint64_t walker, wlx_walker_info *info,
                            int8_t *spins, double *fields, double *log_g, uint64_t *histogram,
                            uint8_t *active, uint8_t *refinement_active);
+/* Scalars of every walker at once (walkers entries), for progress lines and statistics. */
+int wlx_engine_read_infos(wlx_engine *engine, wlx_walker_info *infos);
 /* WangLandauWalker::restore (wang_landau.cpp:420-466): same validation and messages. */
 int wlx_engine_write_walker(wlx_engine *engine, uint64_t walker, const wlx_walker_info *info,
                             const int8_t *spins, const double *fields, const double *log_g,

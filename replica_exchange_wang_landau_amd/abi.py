@@ -130,6 +130,8 @@ EXPORTS = [
     "wlx_system_read_dense",
     "wlx_system_read_csr",
     "wlx_system_fields",
+    "wlx_system_pilot",
+    "wlx_device_info",
     "wlx_engine_create",
     "wlx_engine_destroy",
     "wlx_engine_initialize",
@@ -140,6 +142,7 @@ EXPORTS = [
     "wlx_engine_status",
     "wlx_engine_walkers",
     "wlx_engine_read_walker",
+    "wlx_engine_read_infos",
     "wlx_engine_write_walker",
     "wlx_engine_histogram_statistics",
     "wlx_engine_ready",

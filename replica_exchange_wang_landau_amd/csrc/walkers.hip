@@ -654,6 +654,34 @@ __global__ __launch_bounds__(kWave) void fields_kernel(Args a, const int8_t *spi
     if (lane == 0) energies[c] = e;
 }
 
+// wl_run's --pilot sweep (wl_run.cpp:86-97): unconditional flips from all spins +1
+template <int NPL>
+__global__ __launch_bounds__(kWave) void pilot_kernel(Args a, uint64_t seed, uint64_t attempts,
+                                                      double *range)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint64_t neg = 0;
+    double h[NPL];
+    compute_fields<NPL>(a, neg, h);
+    double energy = total_energy<NPL>(a, neg, h, reinterpret_cast<double *>(smem));
+    double lo = energy, hi = energy;
+    Xoshiro rng = Xoshiro::seeded(seed, 0);
+    for (uint64_t t = 0; t < attempts; ++t) {
+        const uint32_t s = uint32_t(draw_bounded(rng, a.bounded));
+        const double hs = readlane_f64(select_slot<NPL>(h, s >> 6), s & (kWave - 1));
+        const uint64_t bits = readlane_u64(neg, s & (kWave - 1));
+        const double sigma = ((bits >> (s >> 6)) & 1) ? -1.0 : 1.0;
+        energy += -2.0 * sigma * hs;
+        apply_flip<NPL>(a, s, sigma, neg, h);
+        lo = energy < lo ? energy : lo;
+        hi = hi < energy ? energy : hi;
+    }
+    if (lane_id() == 0) {
+        range[0] = lo;
+        range[1] = hi;
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // exchange: the even/odd schedule of parallel.cpp:410-421. Each walker is in at most one pair
 // per epoch, so all pairs are independent; a swap exchanges two slot indices.
@@ -770,17 +798,19 @@ __global__ __launch_bounds__(kWave) void walker_op_kernel(Args a, uint32_t w, ui
 using SampleFn = void (*)(Args, uint64_t, uint32_t);
 using InitFn = void (*)(Args, const int8_t *, const uint8_t *, InitParams, InitReport *);
 using FieldsFn = void (*)(Args, const int8_t *, double *, double *);
+using PilotFn = void (*)(Args, uint64_t, uint64_t, double *);
 
 struct KernelSet {
     SampleFn sample_xo, sample_ph;
     InitFn init;
     FieldsFn fields;
+    PilotFn pilot;
 };
 
 template <int NPL> KernelSet kernel_set()
 {
     return {sample_kernel<NPL, false>, sample_kernel<NPL, true>, init_kernel<NPL>,
-            fields_kernel<NPL>};
+            fields_kernel<NPL>, pilot_kernel<NPL>};
 }
 
 KernelSet kernels_for(uint32_t npl)
@@ -816,6 +846,30 @@ int system_fields_device(wlx_system *s, const int8_t *d_spins, uint64_t count, d
     hipLaunchKernelGGL(k.fields, dim3(unsigned(count)), dim3(kWave), lds, stream, a, d_spins,
                        d_fields, d_energy);
     REWL_HIP(hipGetLastError());
+    return WLX_OK;
+}
+
+int system_pilot_device(wlx_system *s, uint64_t seed, uint64_t attempts, double range[2])
+{
+    Args a{};
+    a.J = s->d_J64;
+    a.n = uint32_t(s->n);
+    a.np = uint32_t(s->np);
+    a.bounded = make_bounded(s->n);
+    const KernelSet k = kernels_for(s->npl);
+    const size_t lds = s->np * sizeof(double);
+    if (int status = set_lds_limit(reinterpret_cast<const void *>(k.pilot), lds)) return status;
+    double *d_range = nullptr;
+    REWL_HIP(hipMalloc(&d_range, 2 * sizeof(double)));
+    hipLaunchKernelGGL(k.pilot, dim3(1), dim3(kWave), lds, s->stream, a, seed, attempts, d_range);
+    hipError_t launch = hipGetLastError();
+    hipError_t copy = hipMemcpyAsync(range, d_range, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                     s->stream);
+    hipError_t sync = hipStreamSynchronize(s->stream);
+    (void)hipFree(d_range);
+    REWL_HIP(launch);
+    REWL_HIP(copy);
+    REWL_HIP(sync);
     return WLX_OK;
 }
 
@@ -1006,6 +1060,31 @@ std::string format17(double x)
 } // namespace
 
 extern "C" {
+
+int wlx_system_pilot(const wlx_system *s, uint64_t seed, uint64_t attempts, double *minimum,
+                     double *maximum)
+{
+    if (!s || !minimum || !maximum) return set_error(WLX_INVALID_ARGUMENT, "Null argument");
+    double range[2];
+    if (int status = system_pilot_device(const_cast<wlx_system *>(s), seed, attempts, range))
+        return status;
+    *minimum = range[0];
+    *maximum = range[1];
+    return WLX_OK;
+}
+
+int wlx_device_info(char *name, size_t capacity, uint64_t *compute_units)
+{
+    int device = 0;
+    REWL_HIP(hipGetDevice(&device));
+    hipDeviceProp_t prop{};
+    REWL_HIP(hipGetDeviceProperties(&prop, device));
+    if (name && capacity) {
+        std::snprintf(name, capacity, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (compute_units) *compute_units = uint64_t(prop.multiProcessorCount);
+    return WLX_OK;
+}
 
 int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
 {
@@ -1372,6 +1451,41 @@ int wlx_engine_read_walker(wlx_engine *e, uint64_t w, wlx_walker_info *info, int
     if (refinement_active) {
         std::fill(refinement_active, refinement_active + e->bins, uint8_t(0));
         std::copy(rf.begin(), rf.end(), refinement_active + st.begin);
+    }
+    return WLX_OK;
+}
+
+int wlx_engine_read_infos(wlx_engine *e, wlx_walker_info *infos)
+{
+    if (int status = require_engine(e)) return status;
+    if (!infos) return set_error(WLX_INVALID_ARGUMENT, "Null argument");
+    std::vector<WalkerState> states(e->walkers);
+    std::vector<double> energies(e->walkers);
+    REWL_HIP(hipMemcpyAsync(states.data(), e->d_state, states.size() * sizeof(WalkerState),
+                            hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipMemcpyAsync(energies.data(), e->d_energy, energies.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    for (uint32_t w = 0; w < e->walkers; ++w) {
+        const WalkerState &st = states[w];
+        wlx_walker_info &info = infos[w];
+        std::memset(&info, 0, sizeof info);
+        info.walker_id = w;
+        info.window = w / e->per_window;
+        info.window_begin = st.begin;
+        info.window_end = st.end;
+        info.energy = energies[st.slot];
+        info.factor = st.factor;
+        info.attempted = st.attempted;
+        info.accepted = st.accepted;
+        info.forced_accepted = st.forced;
+        info.last_accepted_attempt = st.last_accepted;
+        info.stage = st.stage;
+        info.active_bins = st.active_count;
+        info.refinement_active_bins = st.refine_count;
+        for (int k = 0; k < 4; ++k) info.rng_state[k] = st.rng[k];
+        info.last_flatness_check = st.last_check;
+        info.last_flatness = st.last_flatness;
     }
     return WLX_OK;
 }

@@ -1,0 +1,256 @@
+#include "rewl.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+
+namespace rewl::host {
+
+void check(int status)
+{
+    if (status == WLX_OK) return;
+    std::string message(wlx_last_error(nullptr, 0), '\0');
+    wlx_last_error(message.data(), message.size() + 1);
+    switch (status) {
+    case WLX_INVALID_ARGUMENT: throw std::invalid_argument(message);
+    case WLX_LOGIC_ERROR: throw std::logic_error(message);
+    case WLX_OUT_OF_RANGE: throw std::out_of_range(message);
+    default: throw std::runtime_error(message);
+    }
+}
+
+Couplings::Couplings(const Geometry &g, double scale, double cutoff)
+    : n_(g.size()), csr_(cutoff > 0.0)
+{
+    g.validate();
+    const auto p = g.flat_positions(), a = g.flat_axes();
+    const double box[3] = {g.box.x, g.box.y, g.box.z};
+    check(wlx_system_create(p.data(), a.data(), g.size(), box, g.periodic ? 1 : 0, scale, cutoff,
+                            &system_));
+}
+
+Couplings::~Couplings() { wlx_system_destroy(system_); }
+
+PilotRange pilot_energy_range(const Couplings &c, std::uint64_t seed, std::uint64_t attempts)
+{
+    PilotRange r;
+    check(wlx_system_pilot(c.handle(), seed, attempts, &r.minimum, &r.maximum));
+    return r;
+}
+
+namespace {
+
+class Engine {
+  public:
+    Engine(wlx_system *system, const wlx_config &config) { check(wlx_engine_create(system, &config, &e_)); }
+    ~Engine() { wlx_engine_destroy(e_); }
+    wlx_engine *get() const { return e_; }
+
+  private:
+    wlx_engine *e_{};
+};
+
+WalkerState download(wlx_engine *e, std::uint64_t k, std::size_t n, std::size_t bins)
+{
+    WalkerState w;
+    w.spins.resize(n);
+    w.fields.resize(n);
+    w.log_g.resize(bins);
+    w.histogram.resize(bins);
+    w.active.resize(bins);
+    w.refinement.resize(bins);
+    wlx_walker_info info{};
+    check(wlx_engine_read_walker(e, k, &info, w.spins.data(), w.fields.data(), w.log_g.data(),
+                                 w.histogram.data(), w.active.data(), w.refinement.data()));
+    w.id = info.walker_id;
+    w.window = info.window;
+    w.bins_window = {info.window_begin, info.window_end};
+    w.energy = info.energy;
+    w.factor = info.factor;
+    w.attempted = info.attempted;
+    w.accepted = info.accepted;
+    w.forced_accepted = info.forced_accepted;
+    w.last_accepted_attempt = info.last_accepted_attempt;
+    w.stage = int(info.stage);
+    w.active_bins = info.active_bins;
+    w.refinement_active_bins = info.refinement_active_bins;
+    std::memcpy(w.rng_state, info.rng_state, sizeof w.rng_state);
+    w.last_flatness = info.last_flatness;
+    return w;
+}
+
+// WangLandauWalker::restore of a WLCHKP record into walker 0 (wang_landau.cpp:420-466)
+void restore(wlx_engine *e, const WalkerState &s, std::size_t n, std::size_t bins)
+{
+    if (s.id != 0 || s.spins.size() != n || s.fields.size() != n || s.log_g.size() != bins ||
+        s.histogram.size() != bins || s.active.size() != bins)
+        throw std::invalid_argument("Checkpoint is incompatible with walker");
+    wlx_walker_info info{};
+    info.walker_id = s.id;
+    info.energy = s.energy;
+    info.factor = s.factor;
+    info.attempted = s.attempted;
+    info.accepted = s.accepted;
+    info.forced_accepted = s.forced_accepted;
+    info.last_accepted_attempt = s.last_accepted_attempt;
+    info.stage = std::uint64_t(s.stage);
+    std::memcpy(info.rng_state, s.rng_state, sizeof info.rng_state);
+    check(wlx_engine_write_walker(e, 0, &info, s.spins.data(), s.fields.data(), s.log_g.data(),
+                                  s.histogram.data(), s.active.data()));
+}
+
+std::string progress_line(const RunConfig &c, const std::vector<wlx_walker_info> &infos,
+                          std::size_t per_window, std::size_t spins)
+{ // parallel.cpp:468-519
+    std::uint64_t attempted = 0;
+    double factor = 0.0, flatness = 1.0;
+    std::uint64_t stages[3]{};
+    for (std::size_t k = 0; k < infos.size(); ++k) {
+        const auto &i = infos[k];
+        if (k % per_window == 0) attempted = std::max<std::uint64_t>(attempted, i.attempted);
+        factor = std::max(factor, i.factor);
+        ++stages[std::min<std::uint64_t>(i.stage, 2)];
+        if (i.stage == WLX_STAGE_WANG_LANDAU) flatness = std::min(flatness, i.last_flatness);
+    }
+    std::ostringstream line;
+    line << "progress mcs=" << static_cast<double>(attempted) / static_cast<double>(spins)
+         << " attempted_flips_per_walker=" << attempted
+         << (c.wl.inverse_time_enabled ? " coverage=" : " flatness=") << flatness
+         << " factor=" << factor << " stages=" << stages[0] << '/' << stages[1] << '/'
+         << stages[2];
+    return line.str();
+}
+
+} // namespace
+
+RewlResult run_rewl(std::shared_ptr<const Couplings> couplings, const RunConfig &c,
+                    const EngineOptions &options)
+{ // parallel.cpp:276-581, shared-memory path on one GPU
+    if (!couplings) throw std::invalid_argument("Null couplings");
+    c.validate(!c.smoke_test);
+    if (c.adaptive_windows.enabled)
+        throw std::invalid_argument("Adaptive energy windows are not supported by this engine");
+    const std::size_t n = couplings->size(), bins = c.grid.bins();
+    const auto windows =
+        c.explicit_windows.empty() ? partition_windows(bins, c.windows, c.overlap)
+                                   : c.explicit_windows;
+    std::vector<std::uint64_t> be;
+    for (const auto &w : windows) be.insert(be.end(), {w.begin, w.end});
+
+    wlx_config cfg{};
+    cfg.emin = c.grid.minimum;
+    cfg.emax = c.grid.maximum;
+    cfg.bin_width = c.grid.width;
+    cfg.windows = c.windows;
+    cfg.walkers_per_window = c.walkers_per_rank;
+    cfg.overlap = c.overlap;
+    cfg.seed = c.seed;
+    cfg.exchange_interval_attempts = c.exchange.attempts;
+    cfg.max_attempts = c.max_limit.attempts;
+    cfg.rng = std::uint64_t(options.rng);
+    cfg.fields = WLX_FIELDS_FP64;
+    cfg.explicit_windows = be.data();
+    const auto &p = c.wl;
+    cfg.wl = {p.flatness,
+              p.minimum_visits,
+              p.final_factor,
+              p.check_interval_attempts,
+              p.force_accept_after_attempts,
+              p.inverse_time_enabled ? 1u : 0u,
+              p.initialization_max_attempts,
+              p.initialization_target_fraction,
+              p.initialization_temperature_fraction,
+              p.initialization_stall_attempts_per_spin,
+              p.initialization_temperature_multiplier,
+              p.initialization_max_temperature_fraction,
+              p.nalivaiko_mod ? 1u : 0u};
+    Engine engine(couplings->handle(), cfg);
+    wlx_engine *e = engine.get();
+    check(wlx_engine_initialize(e, nullptr, nullptr));
+
+    const bool checkpointing = !c.checkpoint_path.empty() && c.windows == 1 &&
+                               c.walkers_per_rank == 1;
+    if (checkpointing) { // parallel.cpp:354-360
+        try {
+            restore(e, load_checkpoint(c.checkpoint_path), n, bins);
+        } catch (const std::exception &error) {
+            std::cerr << "warning: checkpoint was not restored: " << error.what()
+                      << "; starting a new run\n";
+        }
+    }
+
+    const auto period = std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+        std::chrono::duration<double>(c.progress_interval_seconds));
+    auto next_progress = std::chrono::steady_clock::now() + period;
+    std::size_t progress_width = 0;
+    std::uint64_t last_checkpoint = 0;
+    wlx_run_status st{};
+    std::uint64_t chunk = 1;
+    const std::size_t walkers = c.windows * c.walkers_per_rank;
+    std::vector<wlx_walker_info> infos(walkers);
+    for (;;) {
+        // one epoch at a time while a checkpoint may fall due, otherwise growing chunks
+        check(wlx_engine_run(e, checkpointing ? 1 : chunk, &st));
+        chunk = std::min<std::uint64_t>(chunk * 2, 4096);
+        if (c.progress_interval_seconds > 0.0 && std::chrono::steady_clock::now() >= next_progress) {
+            check(wlx_engine_read_infos(e, infos.data()));
+            const std::string text = progress_line(c, infos, c.walkers_per_rank, n);
+            std::cout << '\r' << text;
+            if (progress_width > text.size())
+                std::cout << std::string(progress_width - text.size(), ' ');
+            std::cout << std::flush;
+            progress_width = std::max(progress_width, text.size());
+            next_progress = std::chrono::steady_clock::now() + period;
+        }
+        if (checkpointing) { // parallel.cpp:522-526
+            check(wlx_engine_read_infos(e, infos.data()));
+            if (infos[0].attempted - last_checkpoint >= c.checkpoint.attempts) {
+                save_checkpoint(c.checkpoint_path, download(e, 0, n, bins));
+                last_checkpoint = infos[0].attempted;
+            }
+        }
+        if (st.done) break;
+    }
+    if (c.progress_interval_seconds > 0.0 && progress_width != 0) std::cout << '\n';
+
+    RewlResult result;
+    std::vector<WalkerState> states;
+    states.reserve(walkers);
+    for (std::size_t k = 0; k < walkers; ++k) states.push_back(download(e, k, n, bins));
+    bool converged = true;
+    for (std::size_t w = 0; w < c.windows; ++w) { // parallel.cpp:541-579
+        std::vector<const WalkerState *> group;
+        for (std::size_t l = 0; l < c.walkers_per_rank; ++l) {
+            const WalkerState &s = states[w * c.walkers_per_rank + l];
+            group.push_back(&s);
+            converged &= s.stage == WLX_STAGE_FROZEN;
+            const auto h = histogram_statistics(s, c.wl.nalivaiko_mod);
+            result.walker_statistics.push_back({s.id, w, 0, s.attempted, s.accepted,
+                                                s.forced_accepted,
+                                                s.attempted - s.last_accepted_attempt, s.energy,
+                                                s.factor, h.active_bins, h.minimum, h.mean,
+                                                h.min_over_mean, 0});
+            result.attempted += s.attempted;
+            result.accepted += s.accepted;
+            result.forced_accepted += s.forced_accepted;
+        }
+        result.fragments.push_back(summarize_walkers(windows[w], group, bins, false));
+    }
+    result.converged = converged;
+    result.exchange_attempted = st.exchange_attempted;
+    result.exchange_accepted = st.exchange_accepted;
+    char device[256] = {0};
+    std::uint64_t cus = 0;
+    if (wlx_device_info(device, sizeof device, &cus) == WLX_OK) result.engine.device = device;
+    result.engine.rng = options.rng == WLX_RNG_PHILOX ? "philox4x64-10" : "xoshiro256**";
+    result.engine.fields = "fp64";
+    result.engine.walkers = walkers;
+    result.engine.epochs = st.epoch;
+    result.engine.ambiguous_decisions = st.ambiguous_decisions;
+    return result;
+}
+
+} // namespace rewl::host

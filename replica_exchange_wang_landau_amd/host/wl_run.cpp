@@ -1,0 +1,136 @@
+// wl_run: the reference's command-line driver (apps/wl_run.cpp) on the MI355X engine.
+// Same options, INI keys, stdout lines, output files and exit codes (0 converged or smoke
+// test, 2 unconverged, 1 on error); the walkers run on the GPU.
+#include "rewl.hpp"
+
+#include <chrono>
+#include <cstdlib>
+#include <exception>
+#include <iostream>
+#include <random>
+#include <string>
+
+using namespace rewl::host;
+
+namespace {
+
+std::uint64_t scramble(std::uint64_t v)
+{
+    v += 0x9e3779b97f4a7c15ULL;
+    v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    v = (v ^ (v >> 27)) * 0x94d049bb133111ebULL;
+    return v ^ (v >> 31);
+}
+
+// a fresh seed from the clock and the OS entropy source
+std::uint64_t random_seed()
+{
+    std::uint64_t seed = static_cast<std::uint64_t>(
+        std::chrono::high_resolution_clock::now().time_since_epoch().count());
+    try {
+        std::random_device device;
+        for (int i = 0; i < 4; ++i) seed = scramble(seed ^ static_cast<std::uint64_t>(device()));
+    } catch (const std::exception &) {
+    }
+    return scramble(seed);
+}
+
+int engine_rng_from_environment()
+{
+    const char *v = std::getenv("REWL_RNG");
+    return (v && std::string(v) == "philox") ? WLX_RNG_PHILOX : WLX_RNG_XOSHIRO;
+}
+
+} // namespace
+
+int main(int argc, char **argv)
+{
+    try {
+        for (int i = 1; i < argc; ++i)
+            if (std::string(argv[i]) == "--help" || std::string(argv[i]) == "-h") {
+                std::cout << usage(argv[0]);
+                return 0;
+            }
+        RunConfig config = parse_arguments(argc, argv);
+        if (!config.seed_explicit) config.seed = random_seed();
+        std::cout << "seed=" << config.seed
+                  << " source=" << (config.seed_explicit ? "configured" : "random") << '\n';
+        const Geometry geometry = geometry_from_config(config);
+        config.resolve_mcs(geometry.size());
+        config.validate(!config.smoke_test && !config.pilot);
+        if (config.pilot && config.max_attempts() == 0)
+            throw std::invalid_argument("Pilot mode requires a positive max-mcs value");
+        const auto &wl = config.wl;
+        std::cout << "spins=" << geometry.size()
+                  << " exchange_interval_mcs=" << config.exchange.mcs
+                  << " check_interval_mcs=" << config.check.mcs
+                  << " max_mcs=" << config.max_limit.mcs
+                  << " checkpoint_interval_mcs=" << config.checkpoint.mcs
+                  << " force_accept_after_mcs=" << config.force_accept.mcs
+                  << " inverse_time=" << (wl.inverse_time_enabled ? "yes" : "no")
+                  << " initialization_max_attempts=" << wl.initialization_max_attempts
+                  << " initialization_target_fraction=" << wl.initialization_target_fraction
+                  << " initialization_temperature_fraction="
+                  << wl.initialization_temperature_fraction
+                  << " initialization_stall_attempts_per_spin="
+                  << wl.initialization_stall_attempts_per_spin
+                  << " initialization_temperature_multiplier="
+                  << wl.initialization_temperature_multiplier
+                  << " initialization_max_temperature_fraction="
+                  << wl.initialization_max_temperature_fraction
+                  << " progress_interval_seconds=" << config.progress_interval_seconds << '\n';
+        if (config.uses_legacy_attempt_units())
+            std::cerr << "warning: legacy attempt-based interval option used; prefer the "
+                         "corresponding *_mcs option\n";
+        if (!config.pilot && wl.force_accept_after_attempts != 0)
+            std::cerr << "warning: force_accept_after_mcs=" << config.force_accept.mcs
+                      << " enables non-standard forced acceptance and may bias the DOS\n";
+        auto couplings =
+            std::make_shared<const Couplings>(geometry, config.coupling_scale, config.cutoff);
+        if (config.pilot) { // wl_run.cpp:84-104
+            const auto r = pilot_energy_range(*couplings, config.seed, config.max_attempts());
+            const double padding = std::max(1e-12, 0.05 * (r.maximum - r.minimum));
+            std::cout << "pilot_only=true attempted_flips=" << config.max_attempts()
+                      << " mcs=" << config.max_limit.mcs << " sampled_min=" << r.minimum
+                      << " sampled_max=" << r.maximum
+                      << " suggested_emin=" << r.minimum - padding
+                      << " suggested_emax=" << r.maximum + padding
+                      << "\nwarning: sampled bounds do not prove the complete energy range\n";
+            return 0;
+        }
+        EngineOptions options;
+        options.rng = engine_rng_from_environment();
+        const auto start = std::chrono::steady_clock::now();
+        const RewlResult result = run_rewl(couplings, config, options);
+        const double elapsed =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+        const auto dos =
+            stitch_dos(config.grid, result.fragments, config.complete_range, geometry.size());
+        write_dos_csv(config.output_prefix + "_dos.csv", dos);
+        write_thermodynamics_csv(config.output_prefix + "_thermo.csv",
+                                 thermodynamics(dos, config.temperatures));
+        for (std::size_t i = 0; i < result.fragments.size(); ++i)
+            write_fragment_csv(config.output_prefix + "_window_" + std::to_string(i) + ".csv",
+                               config.grid, result.fragments[i], i);
+        write_metadata_json(config.output_prefix + "_metadata.json", config, couplings->size(),
+                            couplings->backend_name(), result.attempted, result.accepted,
+                            result.forced_accepted, result.exchange_attempted,
+                            result.exchange_accepted, result.converged, 1, 1, &result.engine);
+        if (!result.converged)
+            write_workers_stat_csv(config.output_prefix + "_workers_stat.csv",
+                                   result.walker_statistics, geometry.size());
+        std::cout << "attempted_flips=" << result.attempted << " aggregate_mcs="
+                  << static_cast<double>(result.attempted) / static_cast<double>(geometry.size())
+                  << " accepted=" << result.accepted << " forced=" << result.forced_accepted
+                  << " exchanges=" << result.exchange_accepted << '/' << result.exchange_attempted
+                  << " converged=" << (result.converged ? "yes" : "no")
+                  << " elapsed_seconds=" << elapsed << '\n';
+        if (!result.converged)
+            std::cerr << "warning: max-mcs reached before final factor; walker statistics written to "
+                      << config.output_prefix << "_workers_stat.csv\n";
+        return result.converged || config.smoke_test ? 0 : 2;
+    } catch (const std::exception &error) {
+        std::cerr << "wl_run: " << error.what() << '\n';
+        return 1;
+    }
+}
