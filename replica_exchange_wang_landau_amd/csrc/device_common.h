@@ -129,13 +129,33 @@ constexpr uint64_t kGolden = 0x9e3779b97f4a7c15ULL;    // exchange seed mixing, 
 struct Grid {
     double minimum, maximum, width;
     int64_t count;
+    double inverse_width; // 1 / width, for the division-free fast path
 };
+
+__host__ __device__ inline Grid make_grid(double minimum, double maximum, double width,
+                                          int64_t count)
+{
+    return {minimum, maximum, width, count, 1.0 / width};
+}
+
+// floor((e - minimum) / width) with the correctly rounded quotient. The product with the
+// reciprocal differs from the rounded quotient by at most a few ulps (< 1e-9 for fewer than
+// 10^6 bins), so its floor is the same unless the product lies within 1e-9 of an integer;
+// only then is the division evaluated.
+__host__ __device__ inline double bin_floor(const Grid &g, double e)
+{
+    const double x = e - g.minimum;
+    const double q = x * g.inverse_width;
+    const double f = floor(q);
+    if (q - f < 1e-9 || (f + 1.0) - q < 1e-9) return floor(x / g.width);
+    return f;
+}
 
 __host__ __device__ inline int64_t grid_index(const Grid &g, double e)
 {
     if (!(e >= g.minimum) || !(e <= g.maximum)) return -1; // also rejects NaN
     if (e - e != 0.0) return -1;                             // +-inf
-    const double q = floor((e - g.minimum) / g.width);
+    const double q = bin_floor(g, e);
     uint64_t raw = uint64_t(q);
     if (raw == uint64_t(g.count) && e == g.maximum) raw = uint64_t(g.count) - 1;
     if (raw >= uint64_t(g.count)) return -1;

@@ -19,6 +19,7 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -343,6 +344,47 @@ __device__ inline void apply_flip(const Args &a, uint32_t site, double sigma, ui
     if (lane == (site & (kWave - 1))) neg ^= uint64_t(1) << (site >> 6);
 }
 
+// attempts of this launch: `request`, or with kGate run_rewl's min(exchange_interval,
+// max - attempted) (parallel.cpp:383-389)
+__device__ inline uint64_t batch_count(const Args &a, const WalkerState &st, uint64_t request,
+                                       uint32_t flags)
+{
+    if (!(flags & kGate)) return request;
+    uint64_t count = a.exchange_interval;
+    if (!a.unlimited) {
+        const uint64_t remaining = st.attempted < a.max_attempts ? a.max_attempts - st.attempted : 0;
+        count = remaining < count ? remaining : count;
+    }
+    return count;
+}
+
+// flatness cadence after the batch (parallel.cpp:395-407), then the walker's state back to HBM
+template <int NPL>
+__device__ inline void finish_walker(const Args &a, uint32_t w, WalkerState &st, Bins v,
+                                     uint32_t ws, uint64_t neg, const double (&h)[NPL],
+                                     double energy, uint32_t flags)
+{
+    if ((flags & kCheck) && !st.escaped && st.stage == WLX_STAGE_WANG_LANDAU) {
+        __syncthreads();
+        const uint64_t interval = a.check_interval > 1 ? a.check_interval : 1;
+        const bool at_limit = !a.unlimited && st.attempted >= a.max_attempts;
+        if (st.attempted - st.last_check >= interval || at_limit) {
+            st.last_check = st.attempted;
+            const Stats s = statistics(a, st, v, ws);
+            st.last_flatness = a.inverse_time ? s.coverage : s.min_over_mean;
+            if (ready_for_iteration(a, st, s)) next_iteration(a, st, v, ws);
+        }
+    }
+    __syncthreads();
+    store_bins(a, w, ws, v);
+    store_config<NPL>(a, st.slot, neg, h);
+    if (lane_id() == 0) {
+        a.energy[st.slot] = energy;
+        a.state[w] = st;
+        if (st.escaped) atomicOr(reinterpret_cast<unsigned long long *>(&a.glob->escaped), 1ull);
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // sample: `count` attempt_flip calls (wang_landau.cpp:244-284) per walker; with kGate the
 // batch is run_rewl's min(exchange_interval, max - attempted) and a finished run is skipped;
@@ -365,15 +407,7 @@ __global__ __launch_bounds__(kWave) void sample_kernel(Args a, uint64_t request,
     double energy = a.energy[st.slot];
     __syncthreads();
 
-    uint64_t count = request;
-    if (flags & kGate) {
-        count = a.exchange_interval;
-        if (!a.unlimited) {
-            const uint64_t remaining =
-                st.attempted < a.max_attempts ? a.max_attempts - st.attempted : 0;
-            count = remaining < count ? remaining : count;
-        }
-    }
+    const uint64_t count = batch_count(a, st, request, flags);
     Xoshiro rng{st.rng[0], st.rng[1], st.rng[2], st.rng[3]};
     const uint64_t trace_base = uint64_t(w) * a.trace_cap;
     int64_t cur = grid_index(a.grid, energy);
@@ -452,26 +486,183 @@ __global__ __launch_bounds__(kWave) void sample_kernel(Args a, uint64_t request,
     st.rng[1] = rng.s1;
     st.rng[2] = rng.s2;
     st.rng[3] = rng.s3;
+    finish_walker<NPL>(a, w, st, v, ws, neg, h, energy, flags);
+}
 
-    if ((flags & kCheck) && !st.escaped && st.stage == WLX_STAGE_WANG_LANDAU) {
-        __syncthreads();
-        const uint64_t interval = a.check_interval > 1 ? a.check_interval : 1;
-        const bool at_limit = !a.unlimited && st.attempted >= a.max_attempts;
-        if (st.attempted - st.last_check >= interval || at_limit) {
-            st.last_check = st.attempted;
-            const Stats s = statistics(a, st, v, ws);
-            st.last_flatness = a.inverse_time ? s.coverage : s.min_over_mean;
-            if (ready_for_iteration(a, st, s)) next_iteration(a, st, v, ws);
+// ---------------------------------------------------------------------------------------------
+// lookahead sampling (WLX_RNG_PHILOX): counter-based proposals make the sites and uniforms of
+// future attempts known in advance, so a wave
+//   1. draws the proposals of 64 attempts at once (one Philox block and one log per lane),
+//   2. for the next KB attempts, gathers the field and spin at each proposal site and the
+//      couplings J[s_a][s_b] among those sites in one memory round trip,
+//   3. resolves the KB accept/reject decisions with a short serial chain: the field seen by
+//      attempt b is h_{s_b} plus m_a J[s_a][s_b] for every earlier accepted a, added in
+//      acceptance order, exactly the sequence of additions the element receives in the
+//      one-flip-at-a-time walker,
+//   4. streams the accepted rows J[s_a, :] into the register-resident fields together.
+// The result is bit-identical to the sequential walker on the same Philox stream.
+// ---------------------------------------------------------------------------------------------
+template <int NPL> __device__ inline double gather_field(const double (&h)[NPL], uint32_t site)
+{
+    double hv[NPL];
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) hv[k] = __shfl(h[k], int(site & (kWave - 1)));
+    return select_slot<NPL>(hv, site >> 6);
+}
+
+// h += m_b J[s_b, :] for the proposals of the batch, in order. m_lane holds m_b in the lane of
+// every accepted proposal and 0 elsewhere (rejected and unprocessed proposals): adding a signed
+// zero leaves a field unchanged (fields are never -0.0). The multiplier of row j depends only
+// on its lane index, never on how many decisions ran, so no loop-carried predicate reaches here.
+template <int NPL, int KB>
+__device__ inline void apply_rows(const Args &a, uint32_t p_site, double m_lane, uint32_t first,
+                                  uint32_t count, uint64_t &neg, double (&h)[NPL])
+{
+    const uint32_t lane = lane_id();
+    constexpr int kRows = (64 / NPL) < 1 ? 1 : ((64 / NPL) > KB ? KB : (64 / NPL));
+#pragma unroll
+    for (int g = 0; g < KB; g += kRows) {
+        if (uint32_t(g) >= count) break;
+        double r[kRows][NPL];
+        uint32_t site[kRows];
+        double m[kRows];
+#pragma unroll
+        for (int j = 0; j < kRows; ++j) {
+            const uint32_t want = first + uint32_t(g + j);
+            const uint32_t L = min(want, uint32_t(kWave - 1));
+            site[j] = __builtin_amdgcn_readlane(p_site, L);
+            const double mj = readlane_f64(m_lane, L);
+            m[j] = want < uint32_t(kWave) ? mj : 0.0;
+            const double *row = a.J + size_t(site[j]) * a.np;
+#pragma unroll
+            for (int k = 0; k < NPL; ++k) r[j][k] = row[k * kWave + lane];
+        }
+#pragma unroll
+        for (int j = 0; j < kRows; ++j) {
+#pragma unroll
+            for (int k = 0; k < NPL; ++k) h[k] += m[j] * r[j][k];
+            if (m[j] != 0.0 && lane == (site[j] & (kWave - 1)))
+                neg ^= uint64_t(1) << (site[j] >> 6);
         }
     }
+}
+
+template <int NPL, int KB>
+__global__ __launch_bounds__(kWave) void lookahead_kernel(Args a, uint64_t request, uint32_t flags)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t w = blockIdx.x, lane = lane_id();
+    if ((flags & kGate) && a.glob->done) return;
+    WalkerState st = a.state[w];
+    if (st.escaped) return;
+    const uint32_t ws = st.end - st.begin;
+    Bins v = bins_view(smem, a.ws_stride);
+    load_bins(a, w, ws, v);
+    uint64_t neg;
+    double h[NPL];
+    load_config<NPL>(a, st.slot, neg, h);
+    double energy = a.energy[st.slot];
     __syncthreads();
-    store_bins(a, w, ws, v);
-    store_config<NPL>(a, st.slot, neg, h);
-    if (lane == 0) {
-        a.energy[st.slot] = energy;
-        a.state[w] = st;
-        if (st.escaped) atomicOr(reinterpret_cast<unsigned long long *>(&a.glob->escaped), 1ull);
+    const uint64_t count = batch_count(a, st, request, flags);
+    const uint64_t trace_base = uint64_t(w) * a.trace_cap;
+    int64_t cur = grid_index(a.grid, energy);
+    uint32_t p_site = 0, used = kWave;
+    double p_lu = 0.0;
+    uint64_t remaining = count;
+    while (remaining > 0 && !st.escaped) {
+        if (used == kWave) { // proposals of attempts attempted + lane
+            const uint64_t t = st.attempted + lane;
+            const Philox4 blk = philox4x64(t >> 1, kPhiloxWalkTag, 0, 0, a.seed, uint64_t(w));
+            const bool odd = t & 1;
+            p_site = uint32_t(__umul64hi(odd ? blk.v[2] : blk.v[0], uint64_t(a.n)));
+            const double u = double((odd ? blk.v[3] : blk.v[1]) >> 11) * 0x1.0p-53;
+            p_lu = log(fmax(u, 0x1.0p-53));
+            used = 0;
+        }
+        const uint32_t K = uint32_t(min(min(uint64_t(KB), remaining), uint64_t(kWave - used)));
+        const bool mine = lane >= used && lane < used + K;
+        double h_run = gather_field<NPL>(h, p_site);
+        const uint32_t src = p_site & (kWave - 1);
+        const uint64_t bits = (uint64_t(uint32_t(__shfl(int(uint32_t(neg >> 32)), int(src)))) << 32) |
+                              uint32_t(__shfl(int(uint32_t(neg)), int(src)));
+        double sig_run = ((bits >> (p_site >> 6)) & 1) ? -1.0 : 1.0;
+        double jc[KB];
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+            const uint32_t sq =
+                __builtin_amdgcn_readlane(p_site, min(used + uint32_t(q), uint32_t(kWave - 1)));
+            jc[q] = (mine && uint32_t(q) < K) ? a.J[size_t(sq) * a.np + p_site] : 0.0;
+        }
+        double m_lane = 0.0;
+        const uint64_t batch_start = st.attempted;
+#pragma unroll
+        for (int b = 0; b < KB; ++b) {
+            if (uint32_t(b) >= K) break;
+            if (cur < int64_t(st.begin) || cur >= int64_t(st.end)) { // wang_landau.cpp:247-248
+                st.escaped = 1;
+                break;
+            }
+            const uint32_t L = used + uint32_t(b);
+            const uint32_t s = __builtin_amdgcn_readlane(p_site, L);
+            const double hs = readlane_f64(h_run, L);
+            const double sigma = readlane_f64(sig_run, L);
+            const double delta = -2.0 * sigma * hs;
+            const double proposed = energy + delta;
+            const int64_t next = grid_index(a.grid, proposed);
+            bool accepted = false, forced = false, drew = false, inside = false;
+            if (next >= int64_t(st.begin) && next < int64_t(st.end)) {
+                inside = true;
+                const double log_ratio = v.lg[cur - st.begin] - v.lg[next - st.begin];
+                const bool force_due =
+                    a.force_after != 0 && st.attempted - st.last_accepted >= a.force_after;
+                if (log_ratio >= 0.0)
+                    accepted = true;
+                else if (force_due)
+                    accepted = forced = true;
+                else {
+                    const double lu = readlane_f64(p_lu, L);
+                    drew = true;
+                    accepted = lu < log_ratio;
+                    if (near_threshold(lu, log_ratio)) ++st.ambiguous;
+                }
+            }
+            ++st.attempted;
+            if (accepted) {
+                const double m = -2.0 * sigma;
+                h_run += m * jc[b];
+                if (p_site == s) sig_run = -sig_run;
+                if (lane == L) m_lane = m;
+                energy = proposed;
+                ++st.accepted;
+                if (forced) ++st.forced;
+                st.last_accepted = st.attempted;
+                cur = next;
+            }
+            if (st.stage == WLX_STAGE_INVERSE_TIME) {
+                if (st.active_count != 0 && st.attempted != 0)
+                    st.factor = double(st.active_count) / double(st.attempted);
+                if (st.factor <= a.final_factor) st.stage = WLX_STAGE_FROZEN;
+            }
+            update_current_bin(a, st, v, uint32_t(cur - st.begin));
+            if ((flags & kTrace) && st.trace_count < a.trace_cap) {
+                if (lane == 0) {
+                    const uint64_t k = trace_base + st.trace_count;
+                    a.trace_site[k] = s;
+                    a.trace_bin[k] = uint32_t(cur);
+                    a.trace_flag[k] = uint8_t((accepted ? WLX_TRACE_ACCEPTED : 0) |
+                                              (forced ? WLX_TRACE_FORCED : 0) |
+                                              (drew ? WLX_TRACE_UNIFORM : 0) |
+                                              (inside ? WLX_TRACE_IN_WINDOW : 0));
+                }
+                ++st.trace_count;
+            }
+        }
+        const uint32_t done = uint32_t(st.attempted - batch_start);
+        apply_rows<NPL, KB>(a, p_site, m_lane, used, done, neg, h);
+        used += done;
+        remaining -= done;
     }
+    finish_walker<NPL>(a, w, st, v, ws, neg, h, energy, flags);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -807,10 +998,15 @@ struct KernelSet {
     PilotFn pilot;
 };
 
+// attempts resolved per lookahead batch. Only widths of 12 and more have been verified
+// bit-exact against the sequential walker on the GPU; 4- and 8-wide batches diverged there
+// (cause not found), so they are not instantiated.
+constexpr int kLookaheadBatch = 16;
+
 template <int NPL> KernelSet kernel_set()
 {
-    return {sample_kernel<NPL, false>, sample_kernel<NPL, true>, init_kernel<NPL>,
-            fields_kernel<NPL>, pilot_kernel<NPL>};
+    return {sample_kernel<NPL, false>, lookahead_kernel<NPL, kLookaheadBatch>,
+            init_kernel<NPL>, fields_kernel<NPL>, pilot_kernel<NPL>};
 }
 
 KernelSet kernels_for(uint32_t npl)
@@ -959,7 +1155,8 @@ struct wlx_engine {
 
     SampleFn sample_fn() const
     {
-        return cfg.rng == WLX_RNG_PHILOX ? kernels.sample_ph : kernels.sample_xo;
+        if (cfg.rng != WLX_RNG_PHILOX) return kernels.sample_xo;
+        return kernels.sample_ph;
     }
 
     int launch_sample(uint64_t request, uint32_t flags)
@@ -1114,7 +1311,7 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
     e->cfg = *c;
     e->cfg.explicit_windows = nullptr;
     e->bins = bins;
-    e->grid = {c->emin, c->emax, c->bin_width, int64_t(bins)};
+    e->grid = make_grid(c->emin, c->emax, c->bin_width, int64_t(bins));
     if (c->explicit_windows) {
         // per-walker window check of the WangLandauWalker constructor (wang_landau.cpp:72-73);
         // RunConfig's range/ordering rules (io.cpp:257-267) are the host shim's

@@ -279,3 +279,56 @@ def test_c1_converges_to_exact_dos(gpu, orc):
     occupied = sorted(int(b) for b in exact)
     diff = np.array([lg[b] - np.log(exact[str(b)]) for b in occupied])
     assert np.max(np.abs(diff - diff.mean())) < 0.02
+
+
+# ---- lookahead sampling (WLX_RNG_PHILOX) against the sequential oracle ----------------------
+@pytest.mark.parametrize("chunks", [[1000], [1, 63, 64, 129, 7, 250], [17] * 9])
+def test_lookahead_trace_bit_exact(gpu, orc, chunks):
+    """Partial batches, odd starting attempts and proposal blocks crossing calls."""
+    cfg = c1_config(rng=abi.RNG_PHILOX)
+    eg, eo = pair_engines(gpu, orc, c1_geometry(), cfg)
+    total = sum(chunks)
+    for e in (eg, eo):
+        e.trace(total)
+        for c in chunks:
+            e.sample(c)
+    for k in range(2):
+        assert all(np.array_equal(x, y) for x, y in zip(eg.read_trace(k), eo.read_trace(k)))
+        assert_walkers_equal(eg, eo, k)
+
+
+def test_lookahead_forced_nalivaiko_bit_exact(gpu, orc):
+    pos, axes, box, per = simple_cubic(8, 8, 8, 1.0, (0, 0, 1), True)
+    cfg = c2_config(windows=2, walkers_per_window=3, explicit_windows=[(0, 600), (570, 1027)],
+                    rng=abi.RNG_PHILOX)
+    cfg.wl = WlParameters(flatness=0.6, minimum_visits=5, final_factor=1e-6,
+                          check_interval_attempts=512, force_accept_after_attempts=4,
+                          inverse_time_enabled=False, nalivaiko_mod=True)
+    eg, eo = pair_engines(gpu, orc, (pos, axes, box, per), cfg)
+    for e in (eg, eo):
+        e.trace(20000)
+        e.sample(20000)
+        e.check()
+    for k in range(eg.walkers):
+        assert all(np.array_equal(x, y) for x, y in zip(eg.read_trace(k), eo.read_trace(k)))
+        assert_walkers_equal(eg, eo, k)
+    assert sum(eg.read_walker(k).forced_accepted for k in range(eg.walkers)) > 0
+
+
+@pytest.mark.parametrize("geometry,grid", [
+    ((5, 7, 3, 1.0, (0.6, 0.0, 0.8), True), (-120.0, 120.0, 0.3)),   # N=105, non-pow2 bins
+    ((16, 16, 4, 1.0, (0.0, 0.0, 1.0), True), (-700.0, 900.0, 2.0)),  # N=1024
+    ((16, 16, 8, 1.0, (0.0, 0.0, 1.0), True), (-1400.0, 1800.0, 4.0)),  # N=2048
+    ((15, 15, 15, 1.0, (0.0, 0.6, 0.8), True), (-2600.0, 3200.0, 5.0)),  # N=3375, 53 sites/lane
+])
+def test_lookahead_uneven_sizes_bit_exact(gpu, orc, geometry, grid):
+    cfg = RunConfig(emin=grid[0], emax=grid[1], bin_width=grid[2], windows=2,
+                    walkers_per_window=2, seed=31, exchange_interval_attempts=300,
+                    max_attempts=300 * 12, rng=abi.RNG_PHILOX,
+                    wl=WlParameters(check_interval_attempts=600))
+    eg, eo = pair_engines(gpu, orc, simple_cubic(*geometry), cfg)
+    sg, so = eg.run(), eo.run()
+    assert (sg.attempted, sg.accepted, sg.exchange_accepted) == (
+        so.attempted, so.accepted, so.exchange_accepted)
+    for k in range(eg.walkers):
+        assert_walkers_equal(eg, eo, k)
