@@ -8,7 +8,10 @@ Workload (the reference's GPU-scale plain REWL case): simple-cubic 8x8x8 periodi
 
 A step is one exchange epoch: every walker runs one 512-attempt batch (sampling kernel),
 then the flatness cadence and the even/odd exchange run on the device. `value` is the
-whole-job attempted-flip rate over K timed epochs, inputs resident in HBM.
+whole-job attempted-flip rate over K timed epochs, inputs resident in HBM, with the
+counter-based (Philox4x64-10) proposal stream and the lookahead sampling kernel. The same
+workload on the Xoshiro stream of the reference (sequential kernel, trajectories identical to
+the reference's) is timed beside it and reported under config.xoshiro.
 
 The roofline object prices the sampling kernel's dense row stream: each accepted flip
 streams one row J[i, :] of N doubles (N * 8 bytes); achieved = accepted flips per launch *
@@ -45,6 +48,7 @@ def c2_config(max_attempts=0, rng=0):
 
 
 def gpu_leg(steps: int, warmup: int, rng: int):
+    """Time `steps` exchange epochs after `warmup` untimed ones."""
     from replica_exchange_wang_landau_amd import Engine, System
 
     system = System.simple_cubic(8, 8, 8, 1.0, (0.0, 0.0, 1.0), True)
@@ -108,21 +112,41 @@ def cpu_leg(mcs_per_walker: int):
                       f"({attempted} attempts incl. walker init) in {seconds:.2f} s"}
 
 
+def calibrate_copy(mib: int):
+    """A plain device copy of known size (read mib MiB, write mib MiB), five times: the
+    reference point that scales FETCH_SIZE / WRITE_SIZE counter readings to bytes."""
+    import torch
+
+    src = torch.ones(mib * 2**20 // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4,
+                      "launches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=300)
-    ap.add_argument("--rng", choices=["xoshiro", "philox"], default="xoshiro")
-    ap.add_argument("--cpu-mcs", type=int, default=800)
+    ap.add_argument("--rng", choices=["xoshiro", "philox"], default="philox")
+    ap.add_argument("--cpu-mcs", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
+                    help="only run a plain copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
-              "(the path would shard by energy window, see DESIGN.md)", file=sys.stderr)
+        print("bench.py: this engine runs on one GPU only; there is no multi-GPU path "
+              "(the walkers would shard by energy window across GPUs)", file=sys.stderr)
         sys.exit(2)
     rng = 0 if args.rng == "xoshiro" else 1
     g = gpu_leg(args.steps, args.warmup, rng)
+    other = gpu_leg(args.steps, args.warmup, 1 - rng)
     cpu = None if args.no_cpu else cpu_leg(args.cpu_mcs)
     line = {
         "metric": "attempted flips/s (all walkers)",
@@ -144,6 +168,10 @@ def main():
             "epochs_timed": g["epochs"], "attempted": g["attempted"],
             "acceptance": g["accepted"] / max(1, g["attempted"]),
             "ambiguous_log_decisions": g["ambiguous"],
+            ("philox" if rng == 0 else "xoshiro"): {
+                "value": other["rate"], "kernel_ms": other["kernel_ms"],
+                "ms_per_step": other["seconds"] * 1e3 / max(1, other["epochs"]),
+                "acceptance": other["accepted"] / max(1, other["attempted"])},
         },
         "roofline": {
             "bound": "hbm",
@@ -152,7 +180,7 @@ def main():
             "unit": "GB/s",
             "frac": g["achieved_gbs"] / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "sample_kernel<8,xoshiro>" if rng == 0 else "sample_kernel<8,philox>",
+            "kernel": "sample_kernel<8,false>" if rng == 0 else "lookahead_kernel<8,16>",
             "kernel_ms": g["kernel_ms"],
             "bytes_per_launch": g["per_launch_bytes"],
         },
