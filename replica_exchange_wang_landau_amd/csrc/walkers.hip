@@ -510,12 +510,35 @@ template <int NPL> __device__ inline double gather_field(const double (&h)[NPL],
     return select_slot<NPL>(hv, site >> 6);
 }
 
-// h += m_b J[s_b, :] for the proposals of the batch, in order. m_lane holds m_b in the lane of
-// every accepted proposal and 0 elsewhere (rejected and unprocessed proposals): adding a signed
-// zero leaves a field unchanged (fields are never -0.0). The multiplier of row j depends only
-// on its lane index, never on how many decisions ran, so no loop-carried predicate reaches here.
+// EnergyGrid::index (wang_landau.cpp:27-34) for a proposed energy: `in` is false outside
+// [minimum, maximum] (and for NaN); otherwise `idx` is the bin, with E == maximum folded into
+// the last bin. Same result as grid_index; the quotient's floor comes from the reciprocal
+// product unless that lies within 1e-9 of an integer (then the IEEE division decides).
+__device__ inline uint32_t grid_bin(const Grid &g, double e, bool &in)
+{
+    in = (e >= g.minimum) && (e <= g.maximum); // min/max are finite: excludes +-inf, NaN
+    const double x = e - g.minimum;
+    const double q = x * g.inverse_width;
+    double f = floor(q);
+    if (q - f < 1e-9 || (f + 1.0) - q < 1e-9) f = floor(x / g.width);
+    uint32_t idx = in ? uint32_t(f) : 0u;
+    if (idx == uint32_t(g.count) && e == g.maximum) idx = uint32_t(g.count) - 1;
+    in = in && idx < uint32_t(g.count);
+    return idx;
+}
+
+__device__ inline double hi_double(uint32_t hi)
+{
+    return __hiloint2double(int(hi), 0);
+}
+
+// h += m_b J[s_b, :] for the proposals of the batch, in order. m_hi holds the high word of
+// m_b = -2 sigma_b (+-2.0; the low word is zero) in the lane of every accepted proposal and 0
+// elsewhere. m_b J is exact (a power-of-two scaling), so fma(m_b, J, h) rounds exactly like
+// the reference's h + (m_b J) (physics.cpp:190-195); m = 0 leaves a field unchanged (fields
+// are never -0.0).
 template <int NPL, int KB>
-__device__ inline void apply_rows(const Args &a, uint32_t p_site, double m_lane, uint32_t first,
+__device__ inline void apply_rows(const Args &a, uint32_t p_site, uint32_t m_hi, uint32_t first,
                                   uint32_t count, uint64_t &neg, double (&h)[NPL])
 {
     const uint32_t lane = lane_id();
@@ -531,8 +554,8 @@ __device__ inline void apply_rows(const Args &a, uint32_t p_site, double m_lane,
             const uint32_t want = first + uint32_t(g + j);
             const uint32_t L = min(want, uint32_t(kWave - 1));
             site[j] = __builtin_amdgcn_readlane(p_site, L);
-            const double mj = readlane_f64(m_lane, L);
-            m[j] = want < uint32_t(kWave) ? mj : 0.0;
+            const uint32_t mh = __builtin_amdgcn_readlane(m_hi, L);
+            m[j] = hi_double(want < uint32_t(kWave) ? mh : 0u);
             const double *row = a.J + size_t(site[j]) * a.np;
 #pragma unroll
             for (int k = 0; k < NPL; ++k) r[j][k] = row[k * kWave + lane];
@@ -540,16 +563,19 @@ __device__ inline void apply_rows(const Args &a, uint32_t p_site, double m_lane,
 #pragma unroll
         for (int j = 0; j < kRows; ++j) {
 #pragma unroll
-            for (int k = 0; k < NPL; ++k) h[k] += m[j] * r[j][k];
+            for (int k = 0; k < NPL; ++k) h[k] = fma(m[j], r[j][k], h[k]);
             if (m[j] != 0.0 && lane == (site[j] & (kWave - 1)))
                 neg ^= uint64_t(1) << (site[j] >> 6);
         }
     }
 }
 
-template <int NPL, int KB>
+constexpr int kLookaheadBatch = 16;
+
+template <int NPL>
 __global__ __launch_bounds__(kWave) void lookahead_kernel(Args a, uint64_t request, uint32_t flags)
 {
+    constexpr int KB = kLookaheadBatch;
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t w = blockIdx.x, lane = lane_id();
     if ((flags & kGate) && a.glob->done) return;
@@ -565,103 +591,145 @@ __global__ __launch_bounds__(kWave) void lookahead_kernel(Args a, uint64_t reque
     __syncthreads();
     const uint64_t count = batch_count(a, st, request, flags);
     const uint64_t trace_base = uint64_t(w) * a.trace_cap;
-    int64_t cur = grid_index(a.grid, energy);
-    uint32_t p_site = 0, used = kWave;
-    double p_lu = 0.0;
-    uint64_t remaining = count;
-    while (remaining > 0 && !st.escaped) {
-        if (used == kWave) { // proposals of attempts attempted + lane
-            const uint64_t t = st.attempted + lane;
-            const Philox4 blk = philox4x64(t >> 1, kPhiloxWalkTag, 0, 0, a.seed, uint64_t(w));
-            const bool odd = t & 1;
-            p_site = uint32_t(__umul64hi(odd ? blk.v[2] : blk.v[0], uint64_t(a.n)));
-            const double u = double((odd ? blk.v[3] : blk.v[1]) >> 11) * 0x1.0p-53;
-            p_lu = log(fmax(u, 0x1.0p-53));
-            used = 0;
-        }
-        const uint32_t K = uint32_t(min(min(uint64_t(KB), remaining), uint64_t(kWave - used)));
-        const bool mine = lane >= used && lane < used + K;
-        double h_run = gather_field<NPL>(h, p_site);
-        const uint32_t src = p_site & (kWave - 1);
-        const uint64_t bits = (uint64_t(uint32_t(__shfl(int(uint32_t(neg >> 32)), int(src)))) << 32) |
-                              uint32_t(__shfl(int(uint32_t(neg)), int(src)));
-        double sig_run = ((bits >> (p_site >> 6)) & 1) ? -1.0 : 1.0;
-        double jc[KB];
-#pragma unroll
-        for (int q = 0; q < KB; ++q) {
-            const uint32_t sq =
-                __builtin_amdgcn_readlane(p_site, min(used + uint32_t(q), uint32_t(kWave - 1)));
-            jc[q] = (mine && uint32_t(q) < K) ? a.J[size_t(sq) * a.np + p_site] : 0.0;
-        }
-        double m_lane = 0.0;
-        const uint64_t batch_start = st.attempted;
-#pragma unroll
-        for (int b = 0; b < KB; ++b) {
-            if (uint32_t(b) >= K) break;
-            if (cur < int64_t(st.begin) || cur >= int64_t(st.end)) { // wang_landau.cpp:247-248
-                st.escaped = 1;
-                break;
+    const int64_t start_bin = grid_index(a.grid, energy);
+    // the current bin can only leave the window through an accepted move, which requires the
+    // proposed bin to lie inside: the check of wang_landau.cpp:247-248 is decided up front
+    if (count > 0 && (start_bin < int64_t(st.begin) || start_bin >= int64_t(st.end))) {
+        st.escaped = 1;
+    } else if (count > 0) {
+        const uint32_t begin = st.begin, end = st.end;
+        uint32_t cur = uint32_t(start_bin);
+        double lg_cur = v.lg[cur - begin];
+        bool cur_seen = v.act[cur - begin] != 0;
+        bool cur_ref = !a.nalivaiko || v.ref[cur - begin] != 0;
+        uint32_t p_site = 0, used = kWave;
+        double p_lu = 0.0;
+        uint64_t remaining = count;
+        while (remaining > 0) {
+            if (used == kWave) { // proposals of attempts attempted + lane
+                const uint64_t t = st.attempted + lane;
+                const Philox4 blk = philox4x64(t >> 1, kPhiloxWalkTag, 0, 0, a.seed, uint64_t(w));
+                const bool odd = t & 1;
+                p_site = uint32_t(__umul64hi(odd ? blk.v[2] : blk.v[0], uint64_t(a.n)));
+                const double u = double((odd ? blk.v[3] : blk.v[1]) >> 11) * 0x1.0p-53;
+                p_lu = log(fmax(u, 0x1.0p-53));
+                used = 0;
             }
-            const uint32_t L = used + uint32_t(b);
-            const uint32_t s = __builtin_amdgcn_readlane(p_site, L);
-            const double hs = readlane_f64(h_run, L);
-            const double sigma = readlane_f64(sig_run, L);
-            const double delta = -2.0 * sigma * hs;
-            const double proposed = energy + delta;
-            const int64_t next = grid_index(a.grid, proposed);
-            bool accepted = false, forced = false, drew = false, inside = false;
-            if (next >= int64_t(st.begin) && next < int64_t(st.end)) {
-                inside = true;
-                const double log_ratio = v.lg[cur - st.begin] - v.lg[next - st.begin];
+            const uint32_t K =
+                uint32_t(min(min(uint64_t(KB), remaining), uint64_t(kWave - used)));
+            // field and -2 sigma at each proposal site; couplings among the batch's sites
+            double h_run = gather_field<NPL>(h, p_site);
+            const uint32_t src = p_site & (kWave - 1);
+            const uint64_t bits =
+                (uint64_t(uint32_t(__shfl(int(uint32_t(neg >> 32)), int(src)))) << 32) |
+                uint32_t(__shfl(int(uint32_t(neg)), int(src)));
+            uint32_t dsig_hi = ((bits >> (p_site >> 6)) & 1) ? 0x40000000u : 0xC0000000u;
+            double jc[KB];
+#pragma unroll
+            for (int q = 0; q < KB; ++q) {
+                const uint32_t sq = __builtin_amdgcn_readlane(
+                    p_site, min(used + uint32_t(q), uint32_t(kWave - 1)));
+                jc[q] = a.J[size_t(sq) * a.np + p_site];
+            }
+            // 1/t factors (update_inverse_time_factor, wang_landau.cpp:309-315): lane x holds
+            // active / attempted for the attempt of lane x, computed for the batch at once
+            double fac_lane = 0.0;
+            if (st.stage == WLX_STAGE_INVERSE_TIME)
+                fac_lane = double(st.active_count) /
+                           double(st.attempted + 1 + uint64_t(lane) - uint64_t(used));
+            uint32_t m_hi = 0;
+            double lr_lane = 0.0;
+            uint64_t drew_mask = 0;
+            const uint64_t batch_start = st.attempted;
+#pragma unroll
+            for (int b = 0; b < KB; ++b) {
+                if (uint32_t(b) >= K) break;
+                const uint32_t L = used + uint32_t(b);
+                const double hs = readlane_f64(h_run, L);
+                const uint32_t ms_hi = __builtin_amdgcn_readlane(dsig_hi, L);
+                const double ms = hi_double(ms_hi); // -2 sigma
+                const double proposed = energy + ms * hs; // flip_delta, physics.cpp:261-266
+                bool in;
+                const uint32_t idx = grid_bin(a.grid, proposed, in);
+                const bool inside = in && idx >= begin && idx < end;
+                const uint32_t rel = inside ? idx - begin : 0u;
+                const double lgn = v.lg[rel];
+                const bool seen_n = v.act[rel] != 0;
+                const bool ref_n = !a.nalivaiko || v.ref[rel] != 0;
+                const double log_ratio = lg_cur - lgn;
                 const bool force_due =
                     a.force_after != 0 && st.attempted - st.last_accepted >= a.force_after;
-                if (log_ratio >= 0.0)
-                    accepted = true;
-                else if (force_due)
-                    accepted = forced = true;
-                else {
-                    const double lu = readlane_f64(p_lu, L);
-                    drew = true;
-                    accepted = lu < log_ratio;
-                    if (near_threshold(lu, log_ratio)) ++st.ambiguous;
+                const double lu = readlane_f64(p_lu, L);
+                const bool up = log_ratio >= 0.0;
+                const bool drew = inside && !up && !force_due;
+                const bool forced = inside && !up && force_due;
+                const bool accepted = inside && (up || force_due || lu < log_ratio);
+                ++st.attempted;
+                drew_mask |= drew ? uint64_t(1) << L : 0;
+                lr_lane = lane == L ? log_ratio : lr_lane;
+                m_hi = lane == L ? (accepted ? ms_hi : 0u) : m_hi;
+                h_run = fma(accepted ? ms : 0.0, jc[b], h_run);
+                const uint32_t s = __builtin_amdgcn_readlane(p_site, L);
+                dsig_hi ^= (accepted && p_site == s) ? 0x80000000u : 0u;
+                energy = accepted ? proposed : energy;
+                cur = accepted ? idx : cur;
+                lg_cur = accepted ? lgn : lg_cur;
+                cur_seen = accepted ? seen_n : cur_seen;
+                cur_ref = accepted ? ref_n : cur_ref;
+                st.accepted += accepted ? 1 : 0;
+                st.forced += forced ? 1 : 0;
+                st.last_accepted = accepted ? st.attempted : st.last_accepted;
+                if (st.stage == WLX_STAGE_INVERSE_TIME) {
+                    if (st.active_count != 0) st.factor = readlane_f64(fac_lane, L);
+                    if (st.factor <= a.final_factor) st.stage = WLX_STAGE_FROZEN;
                 }
-            }
-            ++st.attempted;
-            if (accepted) {
-                const double m = -2.0 * sigma;
-                h_run += m * jc[b];
-                if (p_site == s) sig_run = -sig_run;
-                if (lane == L) m_lane = m;
-                energy = proposed;
-                ++st.accepted;
-                if (forced) ++st.forced;
-                st.last_accepted = st.attempted;
-                cur = next;
-            }
-            if (st.stage == WLX_STAGE_INVERSE_TIME) {
-                if (st.active_count != 0 && st.attempted != 0)
-                    st.factor = double(st.active_count) / double(st.attempted);
-                if (st.factor <= a.final_factor) st.stage = WLX_STAGE_FROZEN;
-            }
-            update_current_bin(a, st, v, uint32_t(cur - st.begin));
-            if ((flags & kTrace) && st.trace_count < a.trace_cap) {
+                // update_current_bin (wang_landau.cpp:290-303)
+                const uint32_t crel = cur - begin;
+                if (!cur_seen || !cur_ref) {
+                    if (!cur_seen) {
+                        ++st.active_count;
+                        cur_seen = true;
+                        if (lane == 0) v.act[crel] = 1;
+                        if (st.stage == WLX_STAGE_INVERSE_TIME)
+                            fac_lane = double(st.active_count) /
+                                       double(st.attempted + uint64_t(lane) - uint64_t(L));
+                    }
+                    if (!cur_ref) {
+                        ++st.refine_count;
+                        cur_ref = true;
+                        if (lane == 0) v.ref[crel] = 1;
+                    }
+                }
+                lg_cur = lg_cur + (st.stage != WLX_STAGE_FROZEN ? st.factor : 0.0);
                 if (lane == 0) {
-                    const uint64_t k = trace_base + st.trace_count;
-                    a.trace_site[k] = s;
-                    a.trace_bin[k] = uint32_t(cur);
-                    a.trace_flag[k] = uint8_t((accepted ? WLX_TRACE_ACCEPTED : 0) |
-                                              (forced ? WLX_TRACE_FORCED : 0) |
-                                              (drew ? WLX_TRACE_UNIFORM : 0) |
-                                              (inside ? WLX_TRACE_IN_WINDOW : 0));
+                    v.lg[crel] = lg_cur;
+                    __hip_atomic_fetch_add(&v.H[crel], uint64_t(1), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
-                ++st.trace_count;
+                if ((flags & kTrace) && st.trace_count < a.trace_cap) {
+                    if (lane == 0) {
+                        const uint64_t k = trace_base + st.trace_count;
+                        a.trace_site[k] = s;
+                        a.trace_bin[k] = cur;
+                        a.trace_flag[k] = uint8_t((accepted ? WLX_TRACE_ACCEPTED : 0) |
+                                                  (forced ? WLX_TRACE_FORCED : 0) |
+                                                  (drew ? WLX_TRACE_UNIFORM : 0) |
+                                                  (inside ? WLX_TRACE_IN_WINDOW : 0));
+                    }
+                    ++st.trace_count;
+                }
             }
+            if (drew_mask) { // decisions whose log(u) lay within 4 ulp of the threshold
+                const bool amb = ((drew_mask >> lane) & 1) && near_threshold(p_lu, lr_lane);
+                st.ambiguous += __popcll(__ballot(amb));
+            }
+            const uint32_t done = uint32_t(st.attempted - batch_start);
+            apply_rows<NPL, KB>(a, p_site, m_hi, used, done, neg, h);
+            used += done;
+            remaining -= done;
         }
-        const uint32_t done = uint32_t(st.attempted - batch_start);
-        apply_rows<NPL, KB>(a, p_site, m_lane, used, done, neg, h);
-        used += done;
-        remaining -= done;
     }
+    __syncthreads();
     finish_walker<NPL>(a, w, st, v, ws, neg, h, energy, flags);
 }
 
@@ -998,14 +1066,9 @@ struct KernelSet {
     PilotFn pilot;
 };
 
-// attempts resolved per lookahead batch. Only widths of 12 and more have been verified
-// bit-exact against the sequential walker on the GPU; 4- and 8-wide batches diverged there
-// (cause not found), so they are not instantiated.
-constexpr int kLookaheadBatch = 16;
-
 template <int NPL> KernelSet kernel_set()
 {
-    return {sample_kernel<NPL, false>, lookahead_kernel<NPL, kLookaheadBatch>,
+    return {sample_kernel<NPL, false>, lookahead_kernel<NPL>,
             init_kernel<NPL>, fields_kernel<NPL>, pilot_kernel<NPL>};
 }
 
