@@ -130,24 +130,32 @@ struct Grid {
     double minimum, maximum, width;
     int64_t count;
     double inverse_width; // 1 / width, for the division-free fast path
+    double margin;        // distance to an integer below which the fast path defers to division
+    double margin_hi;     // 1 - margin
 };
 
+// The reciprocal product q = (e - minimum) * (1 / width) and the correctly rounded quotient
+// Q = (e - minimum) / width both lie within 1.5 and 0.5 ulp(q) of the exact quotient, so their
+// floors can differ only when an integer lies within 2 ulp(q) of q. `margin` is at least
+// 4 ulp of the largest quotient (count), and never below 1e-9.
 __host__ __device__ inline Grid make_grid(double minimum, double maximum, double width,
                                           int64_t count)
 {
-    return {minimum, maximum, width, count, 1.0 / width};
+    double margin = 1e-9;
+    const double ulp_bound = ldexp(4.0, ilogb(double(count) + 1.0) - 52);
+    if (ulp_bound > margin) margin = ulp_bound;
+    return {minimum, maximum, width, count, 1.0 / width, margin, 1.0 - margin};
 }
 
-// floor((e - minimum) / width) with the correctly rounded quotient. The product with the
-// reciprocal differs from the rounded quotient by at most a few ulps (< 1e-9 for fewer than
-// 10^6 bins), so its floor is the same unless the product lies within 1e-9 of an integer;
-// only then is the division evaluated.
+// floor((e - minimum) / width) with the correctly rounded quotient: the reciprocal product's
+// floor unless its fractional part lies within `margin` of 0 or 1 (then the division decides).
 __host__ __device__ inline double bin_floor(const Grid &g, double e)
 {
     const double x = e - g.minimum;
     const double q = x * g.inverse_width;
     const double f = floor(q);
-    if (q - f < 1e-9 || (f + 1.0) - q < 1e-9) return floor(x / g.width);
+    const double fr = q - f; // exact: f <= q < f + 1
+    if (fr < g.margin || fr > g.margin_hi) return floor(x / g.width);
     return f;
 }
 

@@ -512,15 +512,11 @@ template <int NPL> __device__ inline double gather_field(const double (&h)[NPL],
 
 // EnergyGrid::index (wang_landau.cpp:27-34) for a proposed energy: `in` is false outside
 // [minimum, maximum] (and for NaN); otherwise `idx` is the bin, with E == maximum folded into
-// the last bin. Same result as grid_index; the quotient's floor comes from the reciprocal
-// product unless that lies within 1e-9 of an integer (then the IEEE division decides).
+// the last bin. Same result as grid_index.
 __device__ inline uint32_t grid_bin(const Grid &g, double e, bool &in)
 {
     in = (e >= g.minimum) && (e <= g.maximum); // min/max are finite: excludes +-inf, NaN
-    const double x = e - g.minimum;
-    const double q = x * g.inverse_width;
-    double f = floor(q);
-    if (q - f < 1e-9 || (f + 1.0) - q < 1e-9) f = floor(x / g.width);
+    const double f = bin_floor(g, e);
     uint32_t idx = in ? uint32_t(f) : 0u;
     if (idx == uint32_t(g.count) && e == g.maximum) idx = uint32_t(g.count) - 1;
     in = in && idx < uint32_t(g.count);
@@ -728,6 +724,228 @@ __global__ __launch_bounds__(kWave) void lookahead_kernel(Args a, uint64_t reque
             used += done;
             remaining -= done;
         }
+    }
+    __syncthreads();
+    finish_walker<NPL>(a, w, st, v, ws, neg, h, energy, flags);
+}
+
+
+// h += m_b J[s_b, :] for the `count` proposals of a batch (lanes first..), without the spin
+// update (see apply_rows for the exactness argument)
+template <int NPL, int KB>
+__device__ inline void rows_fma(const Args &a, uint32_t p_site, uint32_t m_hi, uint32_t first,
+                                uint32_t count, double (&h)[NPL])
+{
+    const uint32_t lane = lane_id();
+    constexpr int kRows = (64 / NPL) < 1 ? 1 : ((64 / NPL) > KB ? KB : (64 / NPL));
+#pragma unroll
+    for (int g = 0; g < KB; g += kRows) {
+        if (uint32_t(g) >= count) break;
+        double r[kRows][NPL];
+        double m[kRows];
+#pragma unroll
+        for (int j = 0; j < kRows; ++j) {
+            const uint32_t want = first + uint32_t(g + j);
+            const uint32_t L = min(want, uint32_t(kWave - 1));
+            const uint32_t site = __builtin_amdgcn_readlane(p_site, L);
+            const uint32_t mh = __builtin_amdgcn_readlane(m_hi, L);
+            m[j] = hi_double(want < uint32_t(kWave) ? mh : 0u);
+            const double *row = a.J + size_t(site) * a.np;
+#pragma unroll
+            for (int k = 0; k < NPL; ++k) r[j][k] = row[k * kWave + lane];
+        }
+#pragma unroll
+        for (int j = 0; j < kRows; ++j)
+#pragma unroll
+            for (int k = 0; k < NPL; ++k) h[k] = fma(m[j], r[j][k], h[k]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// lookahead sampling, common configuration: no trace, no forced acceptance, no Nalivaiko mask
+// (the general kernel above covers those). Same decisions in the same order, with less work on
+// the serial chain:
+//   - the log_g increment of every attempt of a batch (the WL factor, the 1/t factors
+//     active/attempted with the attempt at which the stage freezes, or 0 when frozen) is
+//     prepared lane-parallel before the chain runs;
+//   - H[cur] += 1 of each attempt is recorded per lane and applied by one LDS atomic per batch
+//     (nothing reads H inside a batch);
+//   - spin flips go through a 64-word LDS box (one XOR per accepted proposal, one read per
+//     lane) instead of a lane test per row;
+//   - counters within a call are 32-bit.
+// ---------------------------------------------------------------------------------------------
+constexpr size_t kFlipBoxBytes = kWave * sizeof(uint64_t);
+
+// `value` in lane `lane` of `old`, the other lanes unchanged
+__device__ inline uint32_t writelane_u32(uint32_t old, uint32_t value, uint32_t lane)
+{
+    return lane_id() == lane ? value : old;
+}
+
+// Per-lane factor for the attempts of lanes >= first, whose attempted count (after its
+// increment) is base + (lane - first) + 1: the WL factor, the 1/t factor active/attempted
+// (update_inverse_time_factor, wang_landau.cpp:309-315; unchanged while no bin is active), or
+// 0 once frozen, so that update_current_bin's log_g increment is this value.
+__device__ inline double batch_factors(uint32_t stage, double factor, uint32_t active,
+                                       uint64_t base, uint32_t first)
+{
+    if (stage == WLX_STAGE_FROZEN) return 0.0;
+    if (stage == WLX_STAGE_WANG_LANDAU || active == 0) return factor;
+    const uint64_t t = base + 1 + uint64_t(lane_id()) - uint64_t(first); // >= 1 from `first`
+    return double(active) / double(t);
+}
+
+template <int NPL>
+__global__ __launch_bounds__(kWave) void lookahead_fast_kernel(Args a, uint64_t request,
+                                                               uint32_t flags)
+{
+    constexpr int KB = kLookaheadBatch;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t w = blockIdx.x, lane = lane_id();
+    if ((flags & kGate) && a.glob->done) return;
+    WalkerState st = a.state[w];
+    if (st.escaped) return;
+    const uint32_t ws = st.end - st.begin;
+    Bins v = bins_view(smem, a.ws_stride);
+    uint64_t *flipbox = reinterpret_cast<uint64_t *>(smem + bins_lds_bytes(a.ws_stride));
+    load_bins(a, w, ws, v);
+    flipbox[lane] = 0;
+    uint64_t neg;
+    double h[NPL];
+    load_config<NPL>(a, st.slot, neg, h);
+    double energy = a.energy[st.slot];
+    __syncthreads();
+    const uint64_t count = batch_count(a, st, request, flags);
+    const int64_t start_bin = grid_index(a.grid, energy);
+    if (count > 0 && (start_bin < int64_t(st.begin) || start_bin >= int64_t(st.end))) {
+        st.escaped = 1; // wang_landau.cpp:247-248, decided up front as in the general kernel
+    } else if (count > 0) {
+        const uint32_t begin = st.begin;
+        const uint64_t att0 = st.attempted;
+        uint32_t cur = uint32_t(start_bin);
+        double lg_cur = v.lg[cur - begin];
+        bool cur_seen = v.act[cur - begin] != 0;
+        uint32_t stage = st.stage, active = st.active_count;
+        double factor = st.factor;
+        uint32_t done_total = 0, n_acc = 0, last_acc = 0; // last_acc: 1 + index, 0 = none
+        uint32_t p_site = 0, used = kWave;
+        double p_lu = 0.0;
+        uint64_t remaining = count;
+        while (remaining > 0) {
+            if (used == kWave) { // proposals of attempts att0 + done_total + lane
+                const uint64_t t = att0 + done_total + lane;
+                const Philox4 blk = philox4x64(t >> 1, kPhiloxWalkTag, 0, 0, a.seed, uint64_t(w));
+                const bool odd = t & 1;
+                p_site = uint32_t(__umul64hi(odd ? blk.v[2] : blk.v[0], uint64_t(a.n)));
+                const double u = double((odd ? blk.v[3] : blk.v[1]) >> 11) * 0x1.0p-53;
+                p_lu = log(fmax(u, 0x1.0p-53));
+                used = 0;
+            }
+            const uint32_t rem32 = remaining < uint64_t(kWave) ? uint32_t(remaining) : kWave;
+            const uint32_t K = min(min(uint32_t(KB), rem32), kWave - used);
+            double h_run = gather_field<NPL>(h, p_site);
+            const uint32_t src = p_site & (kWave - 1);
+            const uint64_t bits =
+                (uint64_t(uint32_t(__shfl(int(uint32_t(neg >> 32)), int(src)))) << 32) |
+                uint32_t(__shfl(int(uint32_t(neg)), int(src)));
+            uint32_t dsig_hi = ((bits >> (p_site >> 6)) & 1) ? 0x40000000u : 0xC0000000u;
+            double jc[KB];
+#pragma unroll
+            for (int q = 0; q < KB; ++q) {
+                const uint32_t sq = __builtin_amdgcn_readlane(
+                    p_site, min(used + uint32_t(q), uint32_t(kWave - 1)));
+                jc[q] = a.J[size_t(sq) * a.np + p_site];
+            }
+            double inc = batch_factors(stage, factor, active, att0 + done_total, used);
+            uint32_t m_hi = 0, cur_lane = 0;
+            double lr_lane = 0.0;
+#pragma unroll
+            for (int b = 0; b < KB; ++b) {
+                if (uint32_t(b) >= K) continue;
+                const uint32_t L = used + uint32_t(b);
+                const double hs = readlane_f64(h_run, L);
+                const uint32_t ms_hi = __builtin_amdgcn_readlane(dsig_hi, L);
+                const double ms = hi_double(ms_hi); // -2 sigma
+                const double proposed = energy + ms * hs; // flip_delta, physics.cpp:261-266
+                // EnergyGrid::index, wang_landau.cpp:27-34, and the window test: in VGPRs
+                const bool in = proposed >= a.grid.minimum && proposed <= a.grid.maximum;
+                const double fb = bin_floor(a.grid, proposed);
+                int32_t idx = int32_t(in ? fb : 0.0); // converted only when in range
+                if (proposed == a.grid.maximum) idx = min(idx, int32_t(a.grid.count) - 1);
+                const uint32_t rel = uint32_t(idx) - begin; // idx < end <= count when rel < ws
+                const bool inside = in && rel < ws;
+                const uint32_t rr = inside ? rel : 0u;
+                const double lgn = v.lg[rr];
+                const bool seen_n = v.act[rr] != 0;
+                const double log_ratio = lg_cur - lgn;
+                const double lu = readlane_f64(p_lu, L);
+                const bool up = log_ratio >= 0.0;
+                const bool accepted = inside && (up || lu < log_ratio);
+                // log ratio of a drawn decision, +inf when no uniform was drawn (outside)
+                lr_lane = lane == L ? (inside ? log_ratio : __builtin_huge_val()) : lr_lane;
+
+                m_hi = writelane_u32(m_hi, accepted ? ms_hi : 0u, L);
+                h_run = fma(accepted ? ms : 0.0, jc[b], h_run);
+                const uint32_t s = __builtin_amdgcn_readlane(p_site, L);
+                dsig_hi ^= (accepted && p_site == s) ? 0x80000000u : 0u;
+                energy = accepted ? proposed : energy;
+                cur = accepted ? uint32_t(idx) : cur;
+                lg_cur = accepted ? lgn : lg_cur;
+                cur_seen = accepted ? seen_n : cur_seen;
+                n_acc += accepted ? 1u : 0u;
+                last_acc = accepted ? done_total + uint32_t(b) + 1 : last_acc;
+                const uint32_t crel = cur - begin;
+                const double f = readlane_f64(inc, L);
+                if (stage == WLX_STAGE_INVERSE_TIME) { // wang_landau.cpp:309-315
+                    factor = f;
+                    if (factor <= a.final_factor) stage = WLX_STAGE_FROZEN;
+                }
+                if (!cur_seen) { // first visit: update_current_bin (wang_landau.cpp:294-297)
+                    ++active;
+                    cur_seen = true;
+                    if (lane == 0) v.act[crel] = 1;
+                    if (stage == WLX_STAGE_INVERSE_TIME) // later attempts see the new count
+                        inc = lane > L ? batch_factors(stage, factor, active,
+                                                       att0 + done_total + b + 1, L + 1)
+                                       : inc;
+                }
+                lg_cur = lg_cur + (stage != WLX_STAGE_FROZEN ? f : 0.0);
+                if (lane == 0) v.lg[crel] = lg_cur;
+                cur_lane = writelane_u32(cur_lane, crel, L);
+            }
+            const uint32_t last = used + K - 1;
+            const bool mine = lane >= used && lane <= last;
+            if (mine) // H[cur] += 1 of every attempt of the batch
+                __hip_atomic_fetch_add(&v.H[cur_lane], uint64_t(1), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            { // drawn decisions (log ratio < 0) whose log(u) lay within 4 ulp of the threshold
+                const bool amb = mine && lr_lane < 0.0 && near_threshold(p_lu, lr_lane);
+                st.ambiguous += __popcll(__ballot(amb));
+            }
+            // spin flips of the accepted proposals, in any order (XOR commutes)
+            if (mine && m_hi != 0)
+                __hip_atomic_fetch_xor(&flipbox[p_site & (kWave - 1)], uint64_t(1) << (p_site >> 6),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __syncthreads();
+            const uint64_t flips = flipbox[lane];
+            neg ^= flips;
+            flipbox[lane] = 0;
+            rows_fma<NPL, KB>(a, p_site, m_hi, used, K, h);
+            used += K;
+            remaining -= K;
+            done_total += K;
+        }
+        // reload the fields the loop did not touch instead of keeping them in registers
+        asm volatile("" ::: "memory");
+        const uint64_t accepted0 = st.accepted, ambiguous = st.ambiguous;
+        st = a.state[w];
+        st.attempted = att0 + done_total;
+        st.accepted = accepted0 + n_acc;
+        if (last_acc) st.last_accepted = att0 + last_acc;
+        st.stage = stage;
+        st.factor = factor;
+        st.active_count = active;
+        st.ambiguous = ambiguous;
     }
     __syncthreads();
     finish_walker<NPL>(a, w, st, v, ws, neg, h, energy, flags);
@@ -1060,7 +1278,7 @@ using FieldsFn = void (*)(Args, const int8_t *, double *, double *);
 using PilotFn = void (*)(Args, uint64_t, uint64_t, double *);
 
 struct KernelSet {
-    SampleFn sample_xo, sample_ph;
+    SampleFn sample_xo, sample_ph, sample_ph_fast;
     InitFn init;
     FieldsFn fields;
     PilotFn pilot;
@@ -1068,7 +1286,7 @@ struct KernelSet {
 
 template <int NPL> KernelSet kernel_set()
 {
-    return {sample_kernel<NPL, false>, lookahead_kernel<NPL>,
+    return {sample_kernel<NPL, false>, lookahead_kernel<NPL>, lookahead_fast_kernel<NPL>,
             init_kernel<NPL>, fields_kernel<NPL>, pilot_kernel<NPL>};
 }
 
@@ -1219,6 +1437,8 @@ struct wlx_engine {
     SampleFn sample_fn() const
     {
         if (cfg.rng != WLX_RNG_PHILOX) return kernels.sample_xo;
+        if (trace_cap == 0 && cfg.wl.force_accept_after_attempts == 0 && !cfg.wl.nalivaiko_mod)
+            return kernels.sample_ph_fast;
         return kernels.sample_ph;
     }
 
@@ -1374,6 +1594,8 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
     e->cfg = *c;
     e->cfg.explicit_windows = nullptr;
     e->bins = bins;
+    if (bins >= (uint64_t(1) << 31))
+        return set_error(WLX_INVALID_ARGUMENT, "Energy grid too large for the device engine");
     e->grid = make_grid(c->emin, c->emax, c->bin_width, int64_t(bins));
     if (c->explicit_windows) {
         // per-walker window check of the WangLandauWalker constructor (wang_landau.cpp:72-73);
@@ -1395,16 +1617,16 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
         ws_max = std::max(ws_max, e->window_be[2 * w + 1] - e->window_be[2 * w]);
     e->ws_stride = uint32_t((ws_max + 7) / 8 * 8);
     e->kernels = kernels_for(sys->npl);
-    e->lds_sample = bins_lds_bytes(e->ws_stride);
+    e->lds_sample = bins_lds_bytes(e->ws_stride) + kFlipBoxBytes;
     e->lds_init = e->lds_sample + sys->np * sizeof(double);
     if (e->lds_init > 160 * 1024)
         return set_error(WLX_INVALID_ARGUMENT, "Energy windows too wide for LDS residency");
     if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.sample_xo),
                                    e->lds_sample))
         return status;
-    if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.sample_ph),
-                                   e->lds_sample))
-        return status;
+    for (SampleFn fn : {e->kernels.sample_ph, e->kernels.sample_ph_fast})
+        if (int status = set_lds_limit(reinterpret_cast<const void *>(fn), e->lds_sample))
+            return status;
     if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.init), e->lds_init))
         return status;
     if (int status = set_lds_limit(reinterpret_cast<const void *>(walker_op_kernel),
