@@ -332,3 +332,55 @@ def test_lookahead_uneven_sizes_bit_exact(gpu, orc, geometry, grid):
         so.attempted, so.accepted, so.exchange_accepted)
     for k in range(eg.walkers):
         assert_walkers_equal(eg, eo, k)
+
+
+def _stages_config(max_epochs):
+    return RunConfig(emin=-60.0, emax=80.0, bin_width=0.5, windows=2, walkers_per_window=2,
+                     seed=5, exchange_interval_attempts=54, max_attempts=54 * max_epochs,
+                     rng=abi.RNG_PHILOX,
+                     wl=WlParameters(final_factor=1e-3, check_interval_attempts=540))
+
+
+def test_lookahead_stages_through_freeze_bit_exact(gpu, orc):
+    """3x3x3 periodic REWL on the Philox stream through WL -> 1/t -> frozen (the common-
+    configuration kernel), compared with the oracle after every 500 epochs."""
+    eg, eo = pair_engines(gpu, orc, simple_cubic(3, 3, 3, 1.0, (0, 0, 1), True),
+                          _stages_config(4000))
+    stages = set()
+    for _ in range(8):
+        sg, so = eg.run(500), eo.run(500)
+        assert (sg.attempted, sg.accepted, sg.exchange_accepted, sg.epoch) == (
+            so.attempted, so.accepted, so.exchange_accepted, so.epoch)
+        for k in range(eg.walkers):
+            assert_walkers_equal(eg, eo, k)
+            stages.add(int(eg.read_walker(k).stage))
+    assert stages == {abi.STAGE_WANG_LANDAU, abi.STAGE_INVERSE_TIME, abi.STAGE_FROZEN}
+
+
+def test_lookahead_first_visits_in_inverse_time_bit_exact(gpu, orc):
+    """Walkers restored into the 1/t stage with some visited bins cleared rediscover them:
+    the active count, and with it every later 1/t factor of the batch, changes mid-batch."""
+    eg, eo = pair_engines(gpu, orc, simple_cubic(3, 3, 3, 1.0, (0, 0, 1), True),
+                          _stages_config(0))
+    eg.run(300)
+    eo.run(300)
+    rng = np.random.default_rng(7)
+    for k in range(eg.walkers):
+        snap = eg.read_walker(k)
+        cur = int(np.floor((snap.energy + 60.0) / 0.5))
+        lo, hi = int(snap.window_begin), int(snap.window_end)
+        visited = [b for b in range(lo, hi) if snap.active[b] and b != cur]
+        for b in rng.choice(visited, size=min(6, len(visited)), replace=False):
+            snap.active[b] = 0
+            snap.histogram[b] = 0
+        snap.info.active_bins = int(snap.active[lo:hi].sum())
+        snap.info.stage = abi.STAGE_INVERSE_TIME
+        snap.info.factor = snap.info.active_bins / snap.info.attempted
+        eg.write_walker(k, snap)
+        eo.write_walker(k, snap)
+    for chunk in (5000, 777, 20000):
+        eg.sample(chunk)
+        eo.sample(chunk)
+        for k in range(eg.walkers):
+            assert_walkers_equal(eg, eo, k)
+    assert any(eg.read_walker(k).active_bins > 0 for k in range(eg.walkers))
