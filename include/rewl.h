@@ -196,6 +196,12 @@ int wlx_engine_exchange(wlx_engine *engine, uint64_t epoch, uint64_t *attempted,
  * when every walker is frozen or the attempt limit is reached. Resumable. */
 int wlx_engine_run(wlx_engine *engine, uint64_t max_epochs, wlx_run_status *status);
 int wlx_engine_status(wlx_engine *engine, wlx_run_status *status);
+/* Restores the run counters of a resumed run: the exchange epoch (which drives the even/odd
+ * parity and the exchange random stream, parallel.cpp:410-421) and the exchange tallies, as
+ * stored by a multi-walker checkpoint next to the per-walker WLCHKP3 records. Clears the
+ * finished flag; the next epoch re-evaluates it (parallel.cpp:527-533). */
+int wlx_engine_set_run_state(wlx_engine *engine, uint64_t epoch, uint64_t exchange_attempted,
+                             uint64_t exchange_accepted);
 int wlx_engine_walkers(const wlx_engine *engine, uint64_t *walkers, uint64_t *spins,
                        uint64_t *bins);
 /* WangLandauWalker::snapshot (wang_landau.cpp:415-418). Arrays are full-grid (bins); any may
@@ -209,6 +215,11 @@ int wlx_engine_read_infos(wlx_engine *engine, wlx_walker_info *infos);
 int wlx_engine_write_walker(wlx_engine *engine, uint64_t walker, const wlx_walker_info *info,
                             const int8_t *spins, const double *fields, const double *log_g,
                             const uint64_t *histogram, const uint8_t *active);
+/* The driver's flatness-check cadence of one walker (the last_flatness_check and
+ * last_flatness entries of run_rewl, parallel.cpp:364-365, which restore leaves at 0 like the
+ * reference's 1x1 resume); a multi-walker checkpoint restores them for an exact resume. */
+int wlx_engine_set_walker_cadence(wlx_engine *engine, uint64_t walker,
+                                  uint64_t last_flatness_check, double last_flatness);
 /* WangLandauWalker::histogram_statistics (wang_landau.cpp:344-367), computed on the device. */
 int wlx_engine_histogram_statistics(wlx_engine *engine, uint64_t walker,
                                     wlx_histogram_stats *out);

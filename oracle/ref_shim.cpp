@@ -5,6 +5,7 @@
 // oracle (wl_oracle.c) and to write the golden fixtures under tests/golden/. It drives the
 // reference's public C++ API: Geometry + DenseCouplings/CsrCouplings (physics.hpp:22-84),
 // WangLandauWalker (wang_landau.hpp:91-138) and run_rewl (parallel.hpp:35-36).
+#include "wl/analysis.hpp"
 #include "wl/parallel.hpp"
 
 #include "../include/rewl.h"
@@ -315,6 +316,34 @@ int ref_run_rewl(void *system, const wlx_config *c, double *frag_log_g, double *
         counters[3] = result.exchange_attempted;
         counters[4] = result.exchange_accepted;
         *converged = result.converged ? 1 : 0;
+        return WLX_OK;
+    } catch (const std::exception &e) {
+        return report(e);
+    }
+}
+
+// stitch_dos (analysis.cpp:45-130) over `count` fragments of `bins` bins each; fragment f
+// covers [begin_end[2f], begin_end[2f+1]). Writes the stitched log_g and valid mask.
+int ref_stitch(double emin, double emax, double width, uint64_t bins, uint64_t count,
+               const uint64_t *begin_end, const double *log_g, const double *error,
+               const uint64_t *histogram, const uint8_t *valid, int complete_range,
+               uint64_t spins, double *out_log_g, uint8_t *out_valid)
+{
+    try {
+        std::vector<wl::DosFragment> fragments;
+        for (uint64_t f = 0; f < count; ++f) {
+            wl::DosFragment d{{begin_end[2 * f], begin_end[2 * f + 1]},
+                              std::vector<double>(log_g + f * bins, log_g + (f + 1) * bins),
+                              std::vector<std::uint64_t>(histogram + f * bins,
+                                                         histogram + (f + 1) * bins),
+                              std::vector<double>(error + f * bins, error + (f + 1) * bins),
+                              std::vector<std::uint8_t>(valid + f * bins, valid + (f + 1) * bins)};
+            fragments.push_back(std::move(d));
+        }
+        const auto dos = wl::stitch_dos({emin, emax, width}, fragments, complete_range != 0,
+                                        spins);
+        std::copy(dos.log_g.begin(), dos.log_g.end(), out_log_g);
+        std::copy(dos.valid.begin(), dos.valid.end(), out_valid);
         return WLX_OK;
     } catch (const std::exception &e) {
         return report(e);

@@ -970,6 +970,25 @@ int orc_engine_status(orc_engine *e, wlx_run_status *st)
     return WLX_OK;
 }
 
+int orc_engine_set_walker_cadence(orc_engine *e, uint64_t k, uint64_t last_flatness_check,
+                                  double last_flatness)
+{
+    if (k >= e->walkers) return fail(WLX_OUT_OF_RANGE, "Walker index out of range");
+    e->w[k].last_check = last_flatness_check;
+    e->w[k].last_flatness = last_flatness;
+    return WLX_OK;
+}
+
+int orc_engine_set_run_state(orc_engine *e, uint64_t epoch, uint64_t exchange_attempted,
+                             uint64_t exchange_accepted)
+{
+    e->epoch = epoch;
+    e->done = 0;
+    e->exchange_attempted = exchange_attempted;
+    e->exchange_accepted = exchange_accepted;
+    return WLX_OK;
+}
+
 int orc_engine_run(orc_engine *e, uint64_t max_epochs, wlx_run_status *status)
 { /* parallel.cpp:376-534 */
     const int unlimited = e->cfg.max_attempts == 0;

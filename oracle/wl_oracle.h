@@ -70,6 +70,10 @@ int orc_engine_exchange(orc_engine *engine, uint64_t epoch, uint64_t *attempted,
                         uint64_t *accepted);
 int orc_engine_run(orc_engine *engine, uint64_t max_epochs, wlx_run_status *status);
 int orc_engine_status(orc_engine *engine, wlx_run_status *status);
+int orc_engine_set_walker_cadence(orc_engine *engine, uint64_t walker,
+                                  uint64_t last_flatness_check, double last_flatness);
+int orc_engine_set_run_state(orc_engine *engine, uint64_t epoch, uint64_t exchange_attempted,
+                             uint64_t exchange_accepted);
 int orc_engine_walkers(const orc_engine *engine, uint64_t *walkers, uint64_t *spins,
                        uint64_t *bins);
 int orc_engine_read_walker(orc_engine *engine, uint64_t walker, wlx_walker_info *info,

@@ -140,10 +140,12 @@ EXPORTS = [
     "wlx_engine_exchange",
     "wlx_engine_run",
     "wlx_engine_status",
+    "wlx_engine_set_run_state",
     "wlx_engine_walkers",
     "wlx_engine_read_walker",
     "wlx_engine_read_infos",
     "wlx_engine_write_walker",
+    "wlx_engine_set_walker_cadence",
     "wlx_engine_histogram_statistics",
     "wlx_engine_ready",
     "wlx_engine_next_iteration",

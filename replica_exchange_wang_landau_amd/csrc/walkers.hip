@@ -1801,6 +1801,37 @@ int wlx_engine_exchange(wlx_engine *e, uint64_t epoch, uint64_t *attempted, uint
     return WLX_OK;
 }
 
+int wlx_engine_set_walker_cadence(wlx_engine *e, uint64_t w, uint64_t last_flatness_check,
+                                  double last_flatness)
+{
+    if (int status = require_walker(e, w)) return status;
+    WalkerState st{};
+    REWL_HIP(hipMemcpyAsync(&st, e->d_state + w, sizeof st, hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    st.last_check = last_flatness_check;
+    st.last_flatness = last_flatness;
+    REWL_HIP(hipMemcpyAsync(e->d_state + w, &st, sizeof st, hipMemcpyHostToDevice, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    return WLX_OK;
+}
+
+int wlx_engine_set_run_state(wlx_engine *e, uint64_t epoch, uint64_t exchange_attempted,
+                             uint64_t exchange_accepted)
+{
+    if (int status = require_engine(e, false)) return status;
+    Globals g{};
+    REWL_HIP(hipMemcpyAsync(&g, e->d_glob, sizeof g, hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    g.epoch = epoch;
+    g.done = 0;
+    g.exchange_attempted = exchange_attempted;
+    g.exchange_accepted = exchange_accepted;
+    REWL_HIP(hipMemcpyAsync(e->d_glob, &g, sizeof g, hipMemcpyHostToDevice, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    e->chunk = 1;
+    return WLX_OK;
+}
+
 int wlx_engine_status(wlx_engine *e, wlx_run_status *st)
 {
     if (int status = require_engine(e, false)) return status;

@@ -323,6 +323,61 @@ class Engine:
         self.backend.call("engine_status", self.handle, ctypes.byref(st))
         return st
 
+    def set_walker_cadence(self, k: int, last_flatness_check: int, last_flatness: float):
+        """The driver's flatness-check bookkeeping of walker k (parallel.cpp:364-365)."""
+        self.backend.call("engine_set_walker_cadence", self.handle, ctypes.c_uint64(k),
+                          ctypes.c_uint64(last_flatness_check), ctypes.c_double(last_flatness))
+
+    def set_run_state(self, epoch: int, exchange_attempted: int, exchange_accepted: int):
+        """Run counters of a resumed run (exchange epoch and tallies)."""
+        self.backend.call("engine_set_run_state", self.handle, ctypes.c_uint64(epoch),
+                          ctypes.c_uint64(exchange_attempted), ctypes.c_uint64(exchange_accepted))
+
+    # ---- multi-walker checkpoint: every walker's snapshot plus the run counters -----------
+    _SNAP_FIELDS = ("walker_id", "window", "window_begin", "window_end", "energy", "factor",
+                    "attempted", "accepted", "forced_accepted", "last_accepted_attempt",
+                    "stage", "active_bins", "refinement_active_bins", "last_flatness_check",
+                    "last_flatness")
+
+    def save_checkpoint(self, path) -> None:
+        """Writes every walker's WalkerSnapshot (wang_landau.hpp:74-89) and the run counters
+        to an .npz file. The reference checkpoints only 1x1 runs (parallel.cpp:354,522)."""
+        snaps = [self.read_walker(k) for k in range(self.walkers)]
+        st = self.status()
+        data = {name: np.array([getattr(s.info, name) for s in snaps])
+                for name in self._SNAP_FIELDS}
+        data["rng_state"] = np.array([list(s.info.rng_state) for s in snaps], dtype=np.uint64)
+        for name in ("spins", "fields", "log_g", "histogram", "active"):
+            data[name] = np.stack([getattr(s, name) for s in snaps])
+        data["run"] = np.array([st.epoch, st.exchange_attempted, st.exchange_accepted],
+                               dtype=np.uint64)
+        data["shape"] = np.array([self.walkers, self.n, self.bins], dtype=np.uint64)
+        np.savez_compressed(path, **data)
+
+    def load_checkpoint(self, path) -> None:
+        """Restores a save_checkpoint file into this (initialized) engine, walker by walker
+        through WangLandauWalker::restore's checks (wang_landau.cpp:420-466)."""
+        with np.load(path, allow_pickle=False) as z:
+            data = {k: z[k] for k in z.files}
+        if tuple(int(x) for x in data["shape"]) != (self.walkers, self.n, self.bins):
+            raise InvalidArgument("Checkpoint does not match the engine's walkers, spins or bins")
+        for k in range(self.walkers):
+            snap = self.read_walker(k)
+            for name in self._SNAP_FIELDS:
+                setattr(snap.info, name, type(getattr(snap.info, name))(data[name][k]))
+            for j in range(4):
+                snap.info.rng_state[j] = int(data["rng_state"][k][j])
+            snap.spins = data["spins"][k]
+            snap.fields = data["fields"][k]
+            snap.log_g = data["log_g"][k]
+            snap.histogram = data["histogram"][k]
+            snap.active = data["active"][k]
+            self.write_walker(k, snap)
+            self.set_walker_cadence(k, int(data["last_flatness_check"][k]),
+                                    float(data["last_flatness"][k]))
+        epoch, ex_att, ex_acc = (int(x) for x in data["run"])
+        self.set_run_state(epoch, ex_att, ex_acc)
+
     def read_walker(self, k: int) -> WalkerSnapshot:
         info = abi.WalkerInfo()
         spins = np.empty(self.n, dtype=np.int8)

@@ -384,3 +384,23 @@ def test_lookahead_first_visits_in_inverse_time_bit_exact(gpu, orc):
         for k in range(eg.walkers):
             assert_walkers_equal(eg, eo, k)
     assert any(eg.read_walker(k).active_bins > 0 for k in range(eg.walkers))
+
+
+@pytest.mark.parametrize("rng_mode", [abi.RNG_XOSHIRO, abi.RNG_PHILOX])
+def test_multi_walker_checkpoint_resume_bit_exact(gpu, orc, tmp_path, rng_mode):
+    """All walkers saved after 150 C2 epochs, restored into a fresh device engine and run
+    150 more: identical to the oracle's uninterrupted 300 epochs."""
+    cfg = c2_config(rng=rng_mode)
+    geo = simple_cubic(8, 8, 8, 1.0, (0, 0, 1), True)
+    eg, eo = pair_engines(gpu, orc, geo, cfg)
+    eg.run(150)
+    eg.save_checkpoint(tmp_path / "c2.npz")
+    resumed = Engine(System(*geo, backend=gpu), cfg)
+    resumed.initialize()
+    resumed.load_checkpoint(tmp_path / "c2.npz")
+    sg = resumed.run(150)
+    so = eo.run(300)
+    assert (sg.epoch, sg.attempted, sg.accepted, sg.exchange_attempted, sg.exchange_accepted) == (
+        so.epoch, so.attempted, so.accepted, so.exchange_attempted, so.exchange_accepted)
+    for k in range(eo.walkers):
+        assert_walkers_equal(resumed, eo, k)
