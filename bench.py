@@ -92,6 +92,55 @@ def gpu_leg(steps: int, warmup: int, rng: int):
     }
 
 
+REFERENCE_DOS = ROOT / "tests" / "golden" / "c2_reference_dos.json"
+
+
+def tta_leg(budget_seconds: float, interval_epochs: int, rng: int = 1):
+    """Time to accuracy: a fresh C2 run (seed 1234) against the converged C2 reference DOS
+    (an independent long run, tests/golden/make_c2_reference_dos.py). Every interval_epochs
+    the walkers are read back, summarised per window and stitched (summarize_walkers +
+    stitch_dos). The clock (device time inside Engine.run only, snapshots excluded) stops at
+    the first snapshot that has found every bin the reference holds and lies within
+    max|dlog g| < 0.01 of it after mean-offset alignment."""
+    from replica_exchange_wang_landau_amd import Engine, System, analysis
+
+    if not REFERENCE_DOS.exists():
+        return None
+    import numpy as np
+
+    ref = json.loads(REFERENCE_DOS.read_text())
+    ref_lg = np.array([float.fromhex(v) for v in ref["log_g"]])
+    ref_valid = np.array(ref["valid"], dtype=np.uint8)
+    engine = Engine(System.simple_cubic(8, 8, 8, 1.0, (0.0, 0.0, 1.0), True), c2_config(rng=rng))
+    engine.initialize()
+    device, reached, best, snaps = 0.0, None, float("inf"), 0
+    attempted = 0
+    while device < budget_seconds:
+        t0 = time.perf_counter()
+        st = engine.run(interval_epochs)
+        device += time.perf_counter() - t0
+        attempted = st.attempted
+        try:
+            lg, valid = analysis.engine_dos(engine)
+        except RuntimeError:  # a window without a common valid bin yet
+            continue
+        snaps += 1
+        if np.any((ref_valid != 0) & (valid == 0)):
+            continue
+        dev = analysis.max_deviation(lg, valid, ref_lg, ref_valid)
+        best = min(best, dev)
+        if dev < 0.01:
+            reached = device
+            break
+        if st.done:
+            break
+    return {"threshold": 0.01, "seconds": reached, "attempted": int(attempted),
+            "best_max_dev": best if best != float("inf") else None, "snapshots": snaps,
+            "interval_epochs": interval_epochs, "budget_seconds": budget_seconds,
+            "reference": f"converged plain REWL, seed {ref['seed']}, "
+                         f"{ref['attempted']} attempts (tests/golden/c2_reference_dos.json)"}
+
+
 def cpu_leg(mcs_per_walker: int):
     """The compiled reference (oracle/_ref) if present, else the C oracle port."""
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
@@ -127,6 +176,63 @@ def cpu_leg(mcs_per_walker: int):
                       f"({attempted} attempts incl. walker init) in {seconds:.2f} s"}
 
 
+def c4_system():
+    """C4 geometry: simple-cubic 16x16x16 (N = 4096), spacing 1, periodic box 16, each site's
+    axis drawn uniformly from {+x, -x, +y, -y, +z, -z} by numpy's default_rng(1234)."""
+    import numpy as np
+
+    from replica_exchange_wang_landau_amd import System
+    from replica_exchange_wang_landau_amd.engine import simple_cubic
+
+    pos, _, box, per = simple_cubic(16, 16, 16, 1.0, (0.0, 0.0, 1.0), True)
+    choice = np.random.default_rng(1234).integers(0, 6, size=len(pos))
+    unit = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], float)
+    return System(pos, unit[choice], box, per)
+
+
+def c4_leg(epochs: int, warmup: int, resyncs: int):
+    """C4 precision study: dense J (FP32 copy streamed by the walkers, FP64 kept for the
+    resync), FP32 Kahan fields, FP64 H = J.S resync every 10 MCS, 32 windows x 8 walkers,
+    bin 2.0 over the seed-1234 10^4-MCS pilot range padded by 5 %, Philox stream."""
+    import math
+
+    from replica_exchange_wang_landau_amd import Engine, RunConfig, WlParameters, abi
+
+    system = c4_system()
+    n = system.n
+    lo, hi = system.pilot(1234, 10_000 * n)
+    pad = 0.05 * (hi - lo)
+    emin, emax = 2.0 * math.floor((lo - pad) / 2.0), 2.0 * math.ceil((hi + pad) / 2.0)
+    cfg = RunConfig(emin=emin, emax=emax, bin_width=2.0, windows=32, walkers_per_window=8,
+                    overlap=0.75, seed=1234, exchange_interval_attempts=n, rng=abi.RNG_PHILOX,
+                    fields=abi.FIELDS_FP32_KAHAN, resync_interval_attempts=10 * n,
+                    wl=WlParameters(final_factor=1e-7, check_interval_attempts=100 * n))
+    engine = Engine(system, cfg)
+    engine.initialize()
+    engine.run(warmup)
+    before = engine.status()
+    t0 = time.perf_counter()
+    after = engine.run(epochs)
+    seconds = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    for _ in range(resyncs):
+        engine.resync()
+    resync_s = (time.perf_counter() - t1) / max(1, resyncs)
+    f, e, count = engine.drift()
+    flops = 2.0 * n * n * engine.walkers
+    return {
+        "workload": f"C4: simple-cubic 16x16x16 periodic, random +-x/y/z axes (seed 1234), "
+                    f"N={n}, grid [{emin},{emax}] bin 2.0, 32 windows x 8 walkers, FP32 Kahan "
+                    f"fields, FP64 resync every 10 MCS",
+        "attempted_flips_per_s": (after.attempted - before.attempted) / seconds,
+        "epochs_timed": epochs,
+        "resync": {"ms": resync_s * 1e3, "flop": flops, "achieved_tflops": flops / resync_s / 1e12,
+                   "peak_tflops": 78.6, "frac": flops / resync_s / 1e12 / 78.6,
+                   "note": "three launches (spin matrix, MFMA J.S, finish) timed together"},
+        "drift": {"max_field_abs": f, "max_energy_rel": e, "resyncs": count},
+    }
+
+
 def calibrate_copy(mib: int):
     """A plain device copy of known size (read mib MiB, write mib MiB), five times: the
     reference point that scales FETCH_SIZE / WRITE_SIZE counter readings to bytes."""
@@ -149,9 +255,17 @@ def main():
     ap.add_argument("--rng", choices=["xoshiro", "philox"], default="philox")
     ap.add_argument("--cpu-mcs", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tta-seconds", type=float, default=150.0,
+                    help="device-time budget of the time-to-accuracy leg (0 skips it)")
+    ap.add_argument("--tta-interval", type=int, default=4000, help="epochs between snapshots")
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
                     help="only run a plain copy of MIB MiB (counter calibration)")
+    ap.add_argument("--c4", action="store_true",
+                    help="only run the C4 FP32-field precision study (resync kernel, drift)")
     args = ap.parse_args()
+    if args.c4:
+        print(json.dumps(c4_leg(args.steps if args.steps != 3000 else 200, 20, 10)))
+        return
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
@@ -166,6 +280,10 @@ def main():
     g = gpu_leg(args.steps, args.warmup, rng)
     other = gpu_leg(args.steps, args.warmup, 1 - rng)
     cpu = None if args.no_cpu else cpu_leg(args.cpu_mcs)
+    tta = tta_leg(args.tta_seconds, args.tta_interval, rng) if args.tta_seconds > 0 else None
+    if tta and cpu and tta["seconds"] is not None:
+        # the reference CPU path at its measured rate over the same number of attempts
+        tta["cpu_estimate_seconds"] = tta["attempted"] / cpu["value"]
     line = {
         "metric": "attempted flips/s (all walkers)",
         "value": g["rate"],
@@ -203,6 +321,7 @@ def main():
             "bytes_per_launch": g["per_launch_bytes"],
         },
         "cpu_baseline": cpu,
+        "time_to_accuracy": tta,
     }
     print(json.dumps(line))
 

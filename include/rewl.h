@@ -42,8 +42,9 @@ enum wlx_status {
 enum wlx_stage { WLX_STAGE_WANG_LANDAU = 0, WLX_STAGE_INVERSE_TIME = 1, WLX_STAGE_FROZEN = 2 };
 /* proposal/uniform stream: reference-compatible Xoshiro256** or counter-based Philox4x64-10 */
 enum wlx_rng { WLX_RNG_XOSHIRO = 0, WLX_RNG_PHILOX = 1 };
-/* local-field precision: FP64 (reference arithmetic) */
-enum wlx_fields { WLX_FIELDS_FP64 = 0 };
+/* local-field precision: FP64 (reference arithmetic), or FP32 fields with Kahan-compensated
+ * incremental updates over FP32 couplings, resynchronised periodically from an FP64 H = J.S */
+enum wlx_fields { WLX_FIELDS_FP64 = 0, WLX_FIELDS_FP32_KAHAN = 1 };
 
 /* trace flags, one byte per recorded attempt */
 enum wlx_trace_flag {
@@ -91,6 +92,7 @@ typedef struct wlx_config {
     uint64_t fields;                     /* enum wlx_fields */
     const uint64_t *explicit_windows;    /* NULL or 2*windows words (begin, end) */
     wlx_params wl;
+    uint64_t resync_interval_attempts;   /* FP32 fields: FP64 resync cadence; 0 = never */
 } wlx_config;
 
 /* Per-walker scalars (WalkerSnapshot, wang_landau.hpp:74-89, plus driver bookkeeping). */
@@ -220,6 +222,16 @@ int wlx_engine_write_walker(wlx_engine *engine, uint64_t walker, const wlx_walke
  * reference's 1x1 resume); a multi-walker checkpoint restores them for an exact resume. */
 int wlx_engine_set_walker_cadence(wlx_engine *engine, uint64_t walker,
                                   uint64_t last_flatness_check, double last_flatness);
+/* FP32 fields: recomputes every configuration's fields H = J.S in FP64 (one matrix product on
+ * the matrix cores over all walkers), measures the drift of the running FP32 fields and
+ * energies against it, and restarts them from the FP64 values. Replaces local_fields and
+ * total_energy (physics.cpp:238-259) as a periodic correction; WLX_LOGIC_ERROR in FP64 mode.
+ * Also run by wlx_engine_run every resync_interval_attempts. */
+int wlx_engine_resync(wlx_engine *engine);
+/* Largest drift seen at any resync so far: max |h_fp32 - h_fp64| over all sites and
+ * configurations, max |E - E_fp64| / |E_fp64| over configurations, and the resync count. */
+int wlx_engine_drift(wlx_engine *engine, double *max_field_error, double *max_energy_relative,
+                     uint64_t *resyncs);
 /* WangLandauWalker::histogram_statistics (wang_landau.cpp:344-367), computed on the device. */
 int wlx_engine_histogram_statistics(wlx_engine *engine, uint64_t walker,
                                     wlx_histogram_stats *out);

@@ -754,6 +754,10 @@ orc_engine *orc_engine_create(const orc_system *s, const wlx_config *c)
         return NULL;
     }
     if (validate_parameters(&c->wl) != WLX_OK) return NULL;
+    if (c->fields != WLX_FIELDS_FP64) { /* the reference's arithmetic only */
+        fail(WLX_INVALID_ARGUMENT, "The oracle restates FP64 fields only");
+        return NULL;
+    }
     orc_engine *e = calloc(1, sizeof *e);
     e->sys = s;
     e->cfg = *c;

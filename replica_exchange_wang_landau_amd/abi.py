@@ -26,6 +26,7 @@ STAGE_FROZEN = 2
 RNG_XOSHIRO = 0
 RNG_PHILOX = 1
 FIELDS_FP64 = 0
+FIELDS_FP32_KAHAN = 1
 
 TRACE_ACCEPTED = 1
 TRACE_FORCED = 2
@@ -73,6 +74,7 @@ class Config(ctypes.Structure):
         ("fields", c_u64),
         ("explicit_windows", ctypes.POINTER(c_u64)),
         ("wl", Params),
+        ("resync_interval_attempts", c_u64),
     ]
 
 
@@ -146,6 +148,8 @@ EXPORTS = [
     "wlx_engine_read_infos",
     "wlx_engine_write_walker",
     "wlx_engine_set_walker_cadence",
+    "wlx_engine_resync",
+    "wlx_engine_drift",
     "wlx_engine_histogram_statistics",
     "wlx_engine_ready",
     "wlx_engine_next_iteration",

@@ -19,6 +19,27 @@
 
 namespace rewl {
 
+// FP32 copy of the dense couplings (round to nearest), for FP32-field sampling
+__global__ void narrow_couplings_kernel(const double *J64, float *J32, size_t count)
+{
+    for (size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x; k < count;
+         k += size_t(gridDim.x) * blockDim.x)
+        J32[k] = float(J64[k]);
+}
+
+int system_j32(wlx_system *s)
+{
+    if (s->d_J32) return WLX_OK;
+    const size_t count = size_t(s->n) * s->np;
+    REWL_HIP(hipMalloc(reinterpret_cast<void **>(&s->d_J32), count * sizeof(float)));
+    hipLaunchKernelGGL(narrow_couplings_kernel, dim3(1024), dim3(256), 0, s->stream, s->d_J64,
+                       s->d_J32, count);
+    REWL_HIP(hipGetLastError());
+    REWL_HIP(hipStreamSynchronize(s->stream));
+    return WLX_OK;
+}
+
+
 namespace {
 thread_local std::string g_error;
 }
@@ -309,6 +330,7 @@ int wlx_system_create(const double *positions, const double *axes, uint64_t n,
 int wlx_system_destroy(wlx_system *s)
 {
     if (!s) return WLX_OK;
+    (void)hipFree(s->d_J32);
     (void)hipFree(s->d_J64);
     (void)hipFree(s->d_offsets);
     (void)hipFree(s->d_columns);

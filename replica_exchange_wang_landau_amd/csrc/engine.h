@@ -48,6 +48,7 @@ struct wlx_system {
     uint64_t *d_offsets = nullptr; // CSR: n+1
     uint32_t *d_columns = nullptr; // CSR: nnz
     double *d_values = nullptr;    // CSR: nnz
+    float *d_J32 = nullptr;        // FP32 copy of d_J64 for FP32-field engines (made on demand)
     hipStream_t stream = nullptr;
 };
 
@@ -55,4 +56,6 @@ namespace rewl {
 // Runs local_fields + total_energy for `count` configurations (device); spins are np-strided.
 int system_fields_device(wlx_system *s, const int8_t *d_spins, uint64_t count, double *d_fields,
                          double *d_energy, hipStream_t stream);
+// Creates d_J32 (each coupling rounded to nearest FP32) if it does not exist yet.
+int system_j32(wlx_system *s);
 } // namespace rewl

@@ -66,6 +66,8 @@ struct InitParams {
 
 struct Args {
     const double *J;
+    const float *J32;   // FP32 couplings (FP32-field engines)
+    float2 *fields32;   // FP32 fields as (h, c) pairs, value h - c (FP32-field engines)
     uint32_t n, np;
     Bounded bounded;
     Grid grid;
@@ -358,11 +360,10 @@ __device__ inline uint64_t batch_count(const Args &a, const WalkerState &st, uin
     return count;
 }
 
-// flatness cadence after the batch (parallel.cpp:395-407), then the walker's state back to HBM
-template <int NPL>
-__device__ inline void finish_walker(const Args &a, uint32_t w, WalkerState &st, Bins v,
-                                     uint32_t ws, uint64_t neg, const double (&h)[NPL],
-                                     double energy, uint32_t flags)
+// flatness cadence after the batch (parallel.cpp:395-407), then the walker's bins, energy and
+// state back to HBM (the configuration's spins and fields are stored by the caller)
+__device__ inline void finish_walker_state(const Args &a, uint32_t w, WalkerState &st, Bins v,
+                                           uint32_t ws, double energy, uint32_t flags)
 {
     if ((flags & kCheck) && !st.escaped && st.stage == WLX_STAGE_WANG_LANDAU) {
         __syncthreads();
@@ -377,12 +378,20 @@ __device__ inline void finish_walker(const Args &a, uint32_t w, WalkerState &st,
     }
     __syncthreads();
     store_bins(a, w, ws, v);
-    store_config<NPL>(a, st.slot, neg, h);
     if (lane_id() == 0) {
         a.energy[st.slot] = energy;
         a.state[w] = st;
         if (st.escaped) atomicOr(reinterpret_cast<unsigned long long *>(&a.glob->escaped), 1ull);
     }
+}
+
+template <int NPL>
+__device__ inline void finish_walker(const Args &a, uint32_t w, WalkerState &st, Bins v,
+                                     uint32_t ws, uint64_t neg, const double (&h)[NPL],
+                                     double energy, uint32_t flags)
+{
+    store_config<NPL>(a, st.slot, neg, h);
+    finish_walker_state(a, w, st, v, ws, energy, flags);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -951,6 +960,379 @@ __global__ __launch_bounds__(kWave) void lookahead_fast_kernel(Args a, uint64_t 
     finish_walker<NPL>(a, w, st, v, ws, neg, h, energy, flags);
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// FP32 local fields (WLX_FIELDS_FP32_KAHAN): the field of site j is a pair (h_j, c_j) of FP32
+// numbers with value h_j - c_j. A flip with multiplier m = -2 sigma adds m J32[i][j] to every
+// field by Kahan's compensated step (Kahan 1965): y = m J32 - c; t = h + y; c = (t - h) - y;
+// h = t. m J32 is exact (a power-of-two scaling). Decisions use the FP64 value
+// (double)h - (double)c and energies stay FP64, as do log_g and the histogram. The lookahead
+// applies the same step to the pair gathered at each proposal site, in acceptance order, so
+// the result equals a one-flip-at-a-time FP32 Kahan walker on the same stream. The drift
+// against FP64 is removed by the periodic resync below.
+// ---------------------------------------------------------------------------------------------
+template <int NPL> __device__ inline float select_slot_f32(const float (&h)[NPL], uint32_t k)
+{
+    float level[NPL];
+#pragma unroll
+    for (int j = 0; j < NPL; ++j) level[j] = h[j];
+#pragma unroll
+    for (int width = NPL, bit = 0; width > 1; width >>= 1, ++bit) {
+        const bool upper = (k >> bit) & 1;
+#pragma unroll
+        for (int j = 0; j < width / 2; ++j) level[j] = upper ? level[2 * j + 1] : level[2 * j];
+    }
+    return level[0];
+}
+
+template <int NPL> __device__ inline float gather_f32(const float (&h)[NPL], uint32_t site)
+{
+    float hv[NPL];
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) hv[k] = __shfl(h[k], int(site & (kWave - 1)));
+    return select_slot_f32<NPL>(hv, site >> 6);
+}
+
+__device__ inline float readlane_f32(float v, uint32_t lane)
+{
+    return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), lane));
+}
+
+// one compensated addition of `add` to the pair (h, c)
+__device__ inline void kahan_add(float &h, float &c, float add)
+{
+    const float y = add - c;
+    const float t = h + y;
+    c = (t - h) - y;
+    h = t;
+}
+
+template <int NPL>
+__device__ inline void load_config32(const Args &a, uint32_t slot, uint64_t &neg,
+                                     float (&h)[NPL], float (&c)[NPL])
+{
+    const uint32_t lane = lane_id();
+    const size_t base = size_t(slot) * a.np;
+    neg = 0;
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) {
+        const float2 p = a.fields32[base + k * kWave + lane];
+        h[k] = p.x;
+        c[k] = p.y;
+        if (a.spins[base + k * kWave + lane] < 0) neg |= uint64_t(1) << k;
+    }
+}
+
+template <int NPL>
+__device__ inline void store_config32(const Args &a, uint32_t slot, uint64_t neg,
+                                      const float (&h)[NPL], const float (&c)[NPL])
+{
+    const uint32_t lane = lane_id();
+    const size_t base = size_t(slot) * a.np;
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) {
+        a.fields32[base + k * kWave + lane] = make_float2(h[k], c[k]);
+        a.spins[base + k * kWave + lane] = ((neg >> k) & 1) ? int8_t(-1) : int8_t(1);
+    }
+}
+
+// compensated row updates of the `count` proposals of a batch, in order; rejected ones skipped
+template <int NPL, int KB>
+__device__ inline void rows_kahan(const Args &a, uint32_t p_site, uint32_t m_hi, uint32_t first,
+                                  uint32_t count, float (&h)[NPL], float (&c)[NPL])
+{
+    const uint32_t lane = lane_id();
+    constexpr int kRows = (64 / NPL) < 1 ? 1 : ((64 / NPL) > 8 ? 8 : (64 / NPL));
+#pragma unroll
+    for (int g = 0; g < KB; g += kRows) {
+        if (uint32_t(g) >= count) break;
+        float r[kRows][NPL];
+        float m[kRows];
+#pragma unroll
+        for (int j = 0; j < kRows; ++j) {
+            const uint32_t want = first + uint32_t(g + j);
+            const uint32_t L = min(want, uint32_t(kWave - 1));
+            const uint32_t site = __builtin_amdgcn_readlane(p_site, L);
+            const uint32_t mh = __builtin_amdgcn_readlane(m_hi, L);
+            m[j] = float(hi_double(want < uint32_t(kWave) && uint32_t(g + j) < count ? mh : 0u));
+            const float *row = a.J32 + size_t(site) * a.np;
+#pragma unroll
+            for (int k = 0; k < NPL; ++k) r[j][k] = row[k * kWave + lane];
+        }
+#pragma unroll
+        for (int j = 0; j < kRows; ++j) {
+            if (m[j] == 0.0f) continue; // a zero increment still moves c into h: skip it
+#pragma unroll
+            for (int k = 0; k < NPL; ++k) kahan_add(h[k], c[k], m[j] * r[j][k]);
+        }
+    }
+}
+
+template <int NPL>
+__global__ __launch_bounds__(kWave) void lookahead_f32_kernel(Args a, uint64_t request,
+                                                              uint32_t flags)
+{
+    constexpr int KB = kLookaheadBatch;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t w = blockIdx.x, lane = lane_id();
+    if ((flags & kGate) && a.glob->done) return;
+    WalkerState st = a.state[w];
+    if (st.escaped) return;
+    const uint32_t ws = st.end - st.begin;
+    Bins v = bins_view(smem, a.ws_stride);
+    uint64_t *flipbox = reinterpret_cast<uint64_t *>(smem + bins_lds_bytes(a.ws_stride));
+    load_bins(a, w, ws, v);
+    flipbox[lane] = 0;
+    uint64_t neg;
+    float h[NPL], c[NPL];
+    load_config32<NPL>(a, st.slot, neg, h, c);
+    double energy = a.energy[st.slot];
+    __syncthreads();
+    const uint64_t count = batch_count(a, st, request, flags);
+    const int64_t start_bin = grid_index(a.grid, energy);
+    if (count > 0 && (start_bin < int64_t(st.begin) || start_bin >= int64_t(st.end))) {
+        st.escaped = 1; // wang_landau.cpp:247-248
+    } else if (count > 0) {
+        const uint32_t begin = st.begin;
+        const uint64_t att0 = st.attempted;
+        uint32_t cur = uint32_t(start_bin);
+        double lg_cur = v.lg[cur - begin];
+        bool cur_seen = v.act[cur - begin] != 0;
+        uint32_t stage = st.stage, active = st.active_count;
+        double factor = st.factor;
+        uint32_t done_total = 0, n_acc = 0, last_acc = 0;
+        uint32_t p_site = 0, used = kWave;
+        double p_lu = 0.0;
+        uint64_t remaining = count;
+        while (remaining > 0) {
+            if (used == kWave) {
+                const uint64_t t = att0 + done_total + lane;
+                const Philox4 blk = philox4x64(t >> 1, kPhiloxWalkTag, 0, 0, a.seed, uint64_t(w));
+                const bool odd = t & 1;
+                p_site = uint32_t(__umul64hi(odd ? blk.v[2] : blk.v[0], uint64_t(a.n)));
+                const double u = double((odd ? blk.v[3] : blk.v[1]) >> 11) * 0x1.0p-53;
+                p_lu = log(fmax(u, 0x1.0p-53));
+                used = 0;
+            }
+            const uint32_t rem32 = remaining < uint64_t(kWave) ? uint32_t(remaining) : kWave;
+            const uint32_t K = min(min(uint32_t(KB), rem32), kWave - used);
+            float hr = gather_f32<NPL>(h, p_site), cr = gather_f32<NPL>(c, p_site);
+            const uint32_t src = p_site & (kWave - 1);
+            const uint64_t bits =
+                (uint64_t(uint32_t(__shfl(int(uint32_t(neg >> 32)), int(src)))) << 32) |
+                uint32_t(__shfl(int(uint32_t(neg)), int(src)));
+            uint32_t dsig_hi = ((bits >> (p_site >> 6)) & 1) ? 0x40000000u : 0xC0000000u;
+            float jc[KB];
+#pragma unroll
+            for (int q = 0; q < KB; ++q) {
+                const uint32_t sq = __builtin_amdgcn_readlane(
+                    p_site, min(used + uint32_t(q), uint32_t(kWave - 1)));
+                jc[q] = a.J32[size_t(sq) * a.np + p_site];
+            }
+            double inc = batch_factors(stage, factor, active, att0 + done_total, used);
+            uint32_t m_hi = 0, cur_lane = 0;
+            double lr_lane = 0.0;
+#pragma unroll
+            for (int b = 0; b < KB; ++b) {
+                if (uint32_t(b) >= K) continue;
+                const uint32_t L = used + uint32_t(b);
+                const double hs =
+                    double(readlane_f32(hr, L)) - double(readlane_f32(cr, L)); // field value
+                const uint32_t ms_hi = __builtin_amdgcn_readlane(dsig_hi, L);
+                const double ms = hi_double(ms_hi); // -2 sigma
+                const double proposed = energy + ms * hs;
+                const bool in = proposed >= a.grid.minimum && proposed <= a.grid.maximum;
+                const double fb = bin_floor(a.grid, proposed);
+                int32_t idx = int32_t(in ? fb : 0.0);
+                if (proposed == a.grid.maximum) idx = min(idx, int32_t(a.grid.count) - 1);
+                const uint32_t rel = uint32_t(idx) - begin;
+                const bool inside = in && rel < ws;
+                const uint32_t rr = inside ? rel : 0u;
+                const double lgn = v.lg[rr];
+                const bool seen_n = v.act[rr] != 0;
+                const double log_ratio = lg_cur - lgn;
+                const double lu = readlane_f64(p_lu, L);
+                const bool up = log_ratio >= 0.0;
+                const bool accepted = inside && (up || lu < log_ratio);
+                lr_lane = lane == L ? (inside ? log_ratio : __builtin_huge_val()) : lr_lane;
+                m_hi = writelane_u32(m_hi, accepted ? ms_hi : 0u, L);
+                if (accepted) { // the flip's compensated step on every proposal's pair
+                    float hn = hr, cn = cr;
+                    kahan_add(hn, cn, float(ms) * jc[b]);
+                    hr = hn;
+                    cr = cn;
+                }
+                const uint32_t s = __builtin_amdgcn_readlane(p_site, L);
+                dsig_hi ^= (accepted && p_site == s) ? 0x80000000u : 0u;
+                energy = accepted ? proposed : energy;
+                cur = accepted ? uint32_t(idx) : cur;
+                lg_cur = accepted ? lgn : lg_cur;
+                cur_seen = accepted ? seen_n : cur_seen;
+                n_acc += accepted ? 1u : 0u;
+                last_acc = accepted ? done_total + uint32_t(b) + 1 : last_acc;
+                const uint32_t crel = cur - begin;
+                const double f = readlane_f64(inc, L);
+                if (stage == WLX_STAGE_INVERSE_TIME) {
+                    factor = f;
+                    if (factor <= a.final_factor) stage = WLX_STAGE_FROZEN;
+                }
+                if (!cur_seen) {
+                    ++active;
+                    cur_seen = true;
+                    if (lane == 0) v.act[crel] = 1;
+                    if (stage == WLX_STAGE_INVERSE_TIME)
+                        inc = lane > L ? batch_factors(stage, factor, active,
+                                                       att0 + done_total + b + 1, L + 1)
+                                       : inc;
+                }
+                lg_cur = lg_cur + (stage != WLX_STAGE_FROZEN ? f : 0.0);
+                if (lane == 0) v.lg[crel] = lg_cur;
+                cur_lane = writelane_u32(cur_lane, crel, L);
+            }
+            const uint32_t last = used + K - 1;
+            const bool mine = lane >= used && lane <= last;
+            if (mine)
+                __hip_atomic_fetch_add(&v.H[cur_lane], uint64_t(1), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            {
+                const bool amb = mine && lr_lane < 0.0 && near_threshold(p_lu, lr_lane);
+                st.ambiguous += __popcll(__ballot(amb));
+            }
+            if (mine && m_hi != 0)
+                __hip_atomic_fetch_xor(&flipbox[p_site & (kWave - 1)], uint64_t(1) << (p_site >> 6),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __syncthreads();
+            neg ^= flipbox[lane];
+            flipbox[lane] = 0;
+            rows_kahan<NPL, KB>(a, p_site, m_hi, used, K, h, c);
+            used += K;
+            remaining -= K;
+            done_total += K;
+        }
+        asm volatile("" ::: "memory");
+        const uint64_t accepted0 = st.accepted, ambiguous = st.ambiguous;
+        st = a.state[w];
+        st.attempted = att0 + done_total;
+        st.accepted = accepted0 + n_acc;
+        if (last_acc) st.last_accepted = att0 + last_acc;
+        st.stage = stage;
+        st.factor = factor;
+        st.active_count = active;
+        st.ambiguous = ambiguous;
+    }
+    __syncthreads();
+    store_config32<NPL>(a, st.slot, neg, h, c);
+    finish_walker_state(a, w, st, v, ws, energy, flags);
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP64 resync of FP32 fields (kernel c): H = J.S for every configuration slot on the matrix
+// cores, energies E = 1/2 sum_i s_i H_i, the drift of the running FP32 state against them, and
+// a restart of the pairs from H (h = fl32(H), c = fl32(h - H)).
+// ---------------------------------------------------------------------------------------------
+struct Drift {
+    unsigned long long max_field;  // bit pattern of a non-negative double
+    unsigned long long max_energy; // bit pattern of a non-negative double
+};
+
+// S[k][s] = spin of site k in slot s, as +-1.0 (zero for padding slots s >= slots)
+__global__ void resync_spins_kernel(const int8_t *spins, uint32_t np, uint32_t slots,
+                                    uint32_t slots_padded, double *S)
+{
+    const size_t total = size_t(np) * slots_padded;
+    for (size_t x = size_t(blockIdx.x) * blockDim.x + threadIdx.x; x < total;
+         x += size_t(gridDim.x) * blockDim.x) {
+        const uint32_t k = uint32_t(x / slots_padded), s = uint32_t(x % slots_padded);
+        S[x] = s < slots ? (spins[size_t(s) * np + k] < 0 ? -1.0 : 1.0) : 0.0;
+    }
+}
+
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+// H[s][i] = sum_k J[k][i] S[k][s] with v_mfma_f64_16x16x4_f64: one wave per 32 sites x 32
+// slots, four 16x16 accumulator blocks, K stepped by 4. J is bitwise symmetric, so the A
+// operand J[i][k] is read as J[k][i] (consecutive lanes, consecutive addresses). Operand
+// layout of the instruction: lane l holds A[l % 16][l / 16], B[l / 16][l % 16] and
+// D[4 (l / 16) + r][l % 16] in accumulator register r.
+__global__ __launch_bounds__(kWave) void resync_mfma_kernel(const double *J, const double *S,
+                                                           uint32_t np, uint32_t slots_padded,
+                                                           uint32_t slots, double *H)
+{
+    const uint32_t lane = threadIdx.x;
+    const uint32_t tiles_i = np / 32;
+    const uint32_t ti = blockIdx.x % tiles_i, ts = blockIdx.x / tiles_i;
+    const uint32_t i0 = ti * 32, s0 = ts * 32;
+    const uint32_t row = lane % 16, kk = lane / 16;
+    mfma_d4 acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+    for (uint32_t k0 = 0; k0 < np; k0 += 4) {
+        const double *jrow = J + size_t(k0 + kk) * np + i0 + row;
+        const double *srow = S + size_t(k0 + kk) * slots_padded + s0 + row;
+        const double a0 = jrow[0], a1 = jrow[16];
+        const double b0 = srow[0], b1 = srow[16];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t i = i0 + 16 * x + 4 * kk + r, s = s0 + 16 * y + row;
+                if (s < slots) H[size_t(s) * np + i] = acc[x][y][r];
+            }
+}
+
+// one wave per slot: drift of the running pairs and energy, then restart from H
+__global__ __launch_bounds__(kWave) void resync_finish_kernel(Args a, const double *H,
+                                                             Drift *drift)
+{
+    __shared__ double part[kWave];
+    const uint32_t s = blockIdx.x, lane = lane_id();
+    const size_t base = size_t(s) * a.np;
+    double field_err = 0.0, e = 0.0;
+    for (uint32_t i = lane; i < a.n; i += kWave) {
+        const double hv = H[base + i];
+        const float2 p = a.fields32[base + i];
+        field_err = fmax(field_err, fabs((double(p.x) - double(p.y)) - hv));
+        const double sigma = a.spins[base + i] < 0 ? -1.0 : 1.0;
+        e += 0.5 * sigma * hv;
+        const float hf = float(hv);
+        a.fields32[base + i] = make_float2(hf, float(double(hf) - hv));
+    }
+    part[lane] = e;
+    __syncthreads();
+    for (int width = kWave / 2; width > 0; width >>= 1) {
+        if (lane < uint32_t(width)) part[lane] += part[lane + width];
+        __syncthreads();
+    }
+    for (int m = 1; m < kWave; m <<= 1) field_err = fmax(field_err, __shfl_xor(field_err, m));
+    if (lane == 0) {
+        const double e64 = part[0];
+        const double rel = e64 != 0.0 ? fabs(a.energy[s] - e64) / fabs(e64) : fabs(a.energy[s]);
+        atomicMax(&drift->max_field, (unsigned long long)__double_as_longlong(field_err));
+        atomicMax(&drift->max_energy, (unsigned long long)__double_as_longlong(rel));
+        a.energy[s] = e64;
+    }
+}
+
+// FP64 fields (from the initial local_fields) into (h, c) pairs
+__global__ void narrow_fields_kernel(const double *fields, float2 *fields32, size_t count)
+{
+    for (size_t x = size_t(blockIdx.x) * blockDim.x + threadIdx.x; x < count;
+         x += size_t(gridDim.x) * blockDim.x) {
+        const double hv = fields[x];
+        const float hf = float(hv);
+        fields32[x] = make_float2(hf, float(double(hf) - hv));
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // init: walker construction + target-Metropolis initialisation (wang_landau.cpp:91-242)
 // ---------------------------------------------------------------------------------------------
@@ -1278,7 +1660,7 @@ using FieldsFn = void (*)(Args, const int8_t *, double *, double *);
 using PilotFn = void (*)(Args, uint64_t, uint64_t, double *);
 
 struct KernelSet {
-    SampleFn sample_xo, sample_ph, sample_ph_fast;
+    SampleFn sample_xo, sample_ph, sample_ph_fast, sample_f32;
     InitFn init;
     FieldsFn fields;
     PilotFn pilot;
@@ -1287,6 +1669,7 @@ struct KernelSet {
 template <int NPL> KernelSet kernel_set()
 {
     return {sample_kernel<NPL, false>, lookahead_kernel<NPL>, lookahead_fast_kernel<NPL>,
+            lookahead_f32_kernel<NPL>,
             init_kernel<NPL>, fields_kernel<NPL>, pilot_kernel<NPL>};
 }
 
@@ -1392,11 +1775,20 @@ struct wlx_engine {
     uint64_t launches = 0;
     uint64_t epochs_run = 0;
     uint64_t chunk = 1;
+    // FP32 fields (WLX_FIELDS_FP32_KAHAN)
+    bool f32 = false;
+    float2 *d_fields32 = nullptr;
+    double *d_S = nullptr, *d_H = nullptr;
+    Drift *d_drift = nullptr;
+    uint32_t slots_padded = 0;
+    uint64_t resync_every = 0, since_resync = 0, resyncs = 0;
 
     Args args(uint32_t flags_trace = 0) const
     {
         Args a{};
         a.J = sys->d_J64;
+        a.J32 = sys->d_J32;
+        a.fields32 = d_fields32;
         a.n = uint32_t(sys->n);
         a.np = uint32_t(sys->np);
         a.bounded = make_bounded(sys->n);
@@ -1436,6 +1828,7 @@ struct wlx_engine {
 
     SampleFn sample_fn() const
     {
+        if (f32) return kernels.sample_f32;
         if (cfg.rng != WLX_RNG_PHILOX) return kernels.sample_xo;
         if (trace_cap == 0 && cfg.wl.force_accept_after_attempts == 0 && !cfg.wl.nalivaiko_mod)
             return kernels.sample_ph_fast;
@@ -1448,6 +1841,23 @@ struct wlx_engine {
         hipLaunchKernelGGL(sample_fn(), dim3(walkers), dim3(kWave), lds_sample, stream, args(),
                            request, flags);
         REWL_HIP(hipGetLastError());
+        return WLX_OK;
+    }
+
+    // kernel c: FP64 H = J.S over all configuration slots, drift, restart of the FP32 pairs
+    int launch_resync()
+    {
+        const uint32_t np = uint32_t(sys->np);
+        hipLaunchKernelGGL(resync_spins_kernel, dim3(256), dim3(256), 0, stream, d_spins, np,
+                           walkers, slots_padded, d_S);
+        REWL_HIP(hipGetLastError());
+        hipLaunchKernelGGL(resync_mfma_kernel, dim3((np / 32) * (slots_padded / 32)), dim3(kWave),
+                           0, stream, sys->d_J64, d_S, np, slots_padded, walkers, d_H);
+        REWL_HIP(hipGetLastError());
+        hipLaunchKernelGGL(resync_finish_kernel, dim3(walkers), dim3(kWave), 0, stream, args(),
+                           d_H, d_drift);
+        REWL_HIP(hipGetLastError());
+        ++resyncs;
         return WLX_OK;
     }
 
@@ -1586,8 +1996,13 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
     if (c->windows > bins) return set_error(WLX_INVALID_ARGUMENT, "More windows than energy bins");
     if (c->rng != WLX_RNG_XOSHIRO && c->rng != WLX_RNG_PHILOX)
         return set_error(WLX_INVALID_ARGUMENT, "Unknown RNG mode");
-    if (c->fields != WLX_FIELDS_FP64)
+    if (c->fields != WLX_FIELDS_FP64 && c->fields != WLX_FIELDS_FP32_KAHAN)
         return set_error(WLX_INVALID_ARGUMENT, "Unknown field precision mode");
+    if (c->fields == WLX_FIELDS_FP32_KAHAN &&
+        (c->rng != WLX_RNG_PHILOX || c->wl.force_accept_after_attempts != 0 || c->wl.nalivaiko_mod))
+        return set_error(WLX_INVALID_ARGUMENT,
+                         "FP32 fields run on the Philox stream without forced acceptance or the "
+                         "Nalivaiko mask");
     if (int status = validate_parameters(c->wl)) return status;
     auto e = std::make_unique<wlx_engine>();
     e->sys = sys;
@@ -1624,7 +2039,7 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
     if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.sample_xo),
                                    e->lds_sample))
         return status;
-    for (SampleFn fn : {e->kernels.sample_ph, e->kernels.sample_ph_fast})
+    for (SampleFn fn : {e->kernels.sample_ph, e->kernels.sample_ph_fast, e->kernels.sample_f32})
         if (int status = set_lds_limit(reinterpret_cast<const void *>(fn), e->lds_sample))
             return status;
     if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.init), e->lds_init))
@@ -1646,6 +2061,19 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
     if (int st = device_alloc(&e->d_glob, 1)) return st;
     if (int st = device_alloc(&e->d_report, W)) return st;
     if (int st = device_alloc(&e->d_op, 8)) return st;
+    if (c->fields == WLX_FIELDS_FP32_KAHAN) {
+        e->f32 = true;
+        if (int st = system_j32(sys)) return st;
+        e->slots_padded = (W + 31) / 32 * 32;
+        if (int st = device_alloc(&e->d_fields32, W * np)) return st;
+        if (int st = device_alloc(&e->d_S, size_t(np) * e->slots_padded)) return st;
+        if (int st = device_alloc(&e->d_H, W * np)) return st;
+        if (int st = device_alloc(&e->d_drift, 1)) return st;
+        REWL_HIP(hipMemsetAsync(e->d_fields32, 0, W * np * sizeof(float2), e->stream));
+        REWL_HIP(hipMemsetAsync(e->d_drift, 0, sizeof(Drift), e->stream));
+        const uint64_t x = c->exchange_interval_attempts;
+        e->resync_every = c->resync_interval_attempts ? (c->resync_interval_attempts + x - 1) / x : 0;
+    }
     std::vector<WalkerState> states(W);
     for (uint32_t k = 0; k < W; ++k) {
         WalkerState &s = states[k];
@@ -1672,7 +2100,8 @@ int wlx_engine_destroy(wlx_engine *e)
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     void *buffers[] = {e->d_state, e->d_spins,  e->d_fields, e->d_energy, e->d_logg,
                        e->d_hist,  e->d_active, e->d_refine, e->d_tsite,  e->d_tbin,
-                       e->d_tflag, e->d_glob,   e->d_report, e->d_op};
+                       e->d_tflag, e->d_glob,   e->d_report, e->d_op,     e->d_fields32,
+                       e->d_S,     e->d_H,      e->d_drift};
     for (void *p : buffers)
         if (p) (void)hipFree(p);
     for (hipEvent_t ev : e->events) (void)hipEventDestroy(ev);
@@ -1729,6 +2158,13 @@ int wlx_engine_initialize(wlx_engine *e, const int8_t *initial, const uint8_t *s
     if (d_supplied) (void)hipFree(d_supplied);
     e->epochs_run = 0;
     e->chunk = 1;
+    if (e->f32) { // the FP32 pairs start from the FP64 local fields
+        hipLaunchKernelGGL(narrow_fields_kernel, dim3(256), dim3(256), 0, e->stream, e->d_fields,
+                           e->d_fields32, size_t(W) * s->np);
+        REWL_HIP(hipGetLastError());
+        REWL_HIP(hipStreamSynchronize(e->stream));
+        e->since_resync = 0;
+    }
     for (uint32_t k = 0; k < W; ++k) {
         const InitReport &r = reports[k];
         if (r.status == 0) continue;
@@ -1798,6 +2234,33 @@ int wlx_engine_exchange(wlx_engine *e, uint64_t epoch, uint64_t *attempted, uint
     REWL_HIP(hipStreamSynchronize(e->stream));
     if (attempted) *attempted = after.exchange_attempted - before.exchange_attempted;
     if (accepted) *accepted = after.exchange_accepted - before.exchange_accepted;
+    return WLX_OK;
+}
+
+int wlx_engine_resync(wlx_engine *e)
+{
+    if (int status = require_engine(e)) return status;
+    if (!e->f32) return set_error(WLX_LOGIC_ERROR, "Resync applies to FP32 fields only");
+    if (int status = e->launch_resync()) return status;
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    return WLX_OK;
+}
+
+int wlx_engine_drift(wlx_engine *e, double *max_field_error, double *max_energy_relative,
+                     uint64_t *resyncs)
+{
+    if (int status = require_engine(e, false)) return status;
+    Drift d{};
+    if (e->d_drift) {
+        REWL_HIP(hipMemcpyAsync(&d, e->d_drift, sizeof d, hipMemcpyDeviceToHost, e->stream));
+        REWL_HIP(hipStreamSynchronize(e->stream));
+    }
+    double f, en;
+    std::memcpy(&f, &d.max_field, sizeof f);
+    std::memcpy(&en, &d.max_energy, sizeof en);
+    if (max_field_error) *max_field_error = f;
+    if (max_energy_relative) *max_energy_relative = en;
+    if (resyncs) *resyncs = e->resyncs;
     return WLX_OK;
 }
 
@@ -1878,6 +2341,10 @@ int wlx_engine_run(wlx_engine *e, uint64_t max_epochs, wlx_run_status *status)
             if (e->timing) REWL_HIP(hipEventRecord(e->events[2 * k], e->stream));
             if (int st = e->launch_sample(0, kGate | kCheck)) return st;
             if (e->timing) REWL_HIP(hipEventRecord(e->events[2 * k + 1], e->stream));
+            if (e->resync_every && ++e->since_resync >= e->resync_every) {
+                if (int st = e->launch_resync()) return st;
+                e->since_resync = 0;
+            }
             if (int st = e->launch_exchange(1, 0)) return st;
         }
         Globals g{};
@@ -1927,6 +2394,13 @@ int wlx_engine_read_walker(wlx_engine *e, uint64_t w, wlx_walker_info *info, int
                             hipMemcpyDeviceToHost, e->stream));
     REWL_HIP(hipMemcpyAsync(ac.data(), e->d_active + wb, ws, hipMemcpyDeviceToHost, e->stream));
     REWL_HIP(hipMemcpyAsync(rf.data(), e->d_refine + wb, ws, hipMemcpyDeviceToHost, e->stream));
+    if (e->f32) { // FP32 fields: the value h - c of each pair
+        std::vector<float2> pairs(s->np);
+        REWL_HIP(hipMemcpyAsync(pairs.data(), e->d_fields32 + size_t(st.slot) * s->np,
+                                s->np * sizeof(float2), hipMemcpyDeviceToHost, e->stream));
+        REWL_HIP(hipStreamSynchronize(e->stream));
+        for (uint64_t i = 0; i < s->np; ++i) fl[i] = double(pairs[i].x) - double(pairs[i].y);
+    }
     REWL_HIP(hipStreamSynchronize(e->stream));
     if (info) {
         std::memset(info, 0, sizeof *info);
@@ -2070,6 +2544,19 @@ int wlx_engine_write_walker(wlx_engine *e, uint64_t w, const wlx_walker_info *in
     std::vector<double> exact(s->n);
     double exact_energy = 0.0;
     if (int status = wlx_system_fields(s, spins, 1, exact.data(), &exact_energy)) return status;
+    if (e->f32) { // FP32 fields restart from the FP64 recompute of the restored spins
+        std::vector<float2> pairs(s->np, make_float2(0.0f, 0.0f));
+        for (uint64_t i = 0; i < s->n; ++i) {
+            const float hf = float(exact[i]);
+            pairs[i] = make_float2(hf, float(double(hf) - exact[i]));
+        }
+        REWL_HIP(hipMemcpyAsync(e->d_fields32 + size_t(st.slot) * s->np, pairs.data(),
+                                s->np * sizeof(float2), hipMemcpyHostToDevice, e->stream));
+        REWL_HIP(hipMemcpyAsync(e->d_energy + st.slot, &exact_energy, sizeof exact_energy,
+                                hipMemcpyHostToDevice, e->stream));
+        REWL_HIP(hipStreamSynchronize(e->stream));
+        return WLX_OK;
+    }
     if (std::abs(exact_energy - energy) > 1e-9 * std::max(1.0, std::abs(exact_energy)))
         return set_error(WLX_RUNTIME_ERROR, "Checkpoint energy does not match spin configuration");
     for (uint64_t i = 0; i < s->n; ++i)
@@ -2124,6 +2611,8 @@ int wlx_engine_next_iteration(wlx_engine *e, uint64_t w)
 int wlx_engine_trace(wlx_engine *e, uint64_t capacity)
 {
     if (int status = require_engine(e, false)) return status;
+    if (e->f32 && capacity)
+        return set_error(WLX_INVALID_ARGUMENT, "Attempt traces are not recorded with FP32 fields");
     (void)hipFree(e->d_tsite);
     (void)hipFree(e->d_tbin);
     (void)hipFree(e->d_tflag);

@@ -149,12 +149,14 @@ class RunConfig:
     rng: int = abi.RNG_XOSHIRO
     fields: int = abi.FIELDS_FP64
     explicit_windows: Sequence[tuple[int, int]] | None = None
+    resync_interval_attempts: int = 0
     wl: WlParameters = field(default_factory=WlParameters)
 
     def to_c(self) -> tuple[abi.Config, Any]:
         c = abi.Config()
         for name in ("emin", "emax", "bin_width", "windows", "walkers_per_window", "overlap",
-                     "seed", "exchange_interval_attempts", "max_attempts", "rng", "fields"):
+                     "seed", "exchange_interval_attempts", "max_attempts", "rng", "fields",
+                     "resync_interval_attempts"):
             setattr(c, name, getattr(self, name))
         keep = None
         if self.explicit_windows is not None:
@@ -235,6 +237,13 @@ class System:
         self.backend.call("system_read_csr", self.handle, _ptr(offsets, ctypes.c_uint64),
                           _ptr(columns, ctypes.c_uint64), _ptr(values, ctypes.c_double))
         return offsets, columns[: self.nnz], values[: self.nnz]
+
+    def pilot(self, seed: int, attempts: int) -> tuple[float, float]:
+        """The wl_run --pilot sweep (wl_run.cpp:84-104): sampled energy range."""
+        lo, hi = ctypes.c_double(), ctypes.c_double()
+        self.backend.call("system_pilot", self.handle, ctypes.c_uint64(seed),
+                          ctypes.c_uint64(attempts), ctypes.byref(lo), ctypes.byref(hi))
+        return lo.value, hi.value
 
     def fields(self, spins) -> tuple[np.ndarray, np.ndarray]:
         """local_fields + total_energy (physics.cpp:238-259) per configuration."""
@@ -322,6 +331,17 @@ class Engine:
         st = abi.RunStatus()
         self.backend.call("engine_status", self.handle, ctypes.byref(st))
         return st
+
+    def resync(self):
+        """FP64 recompute of every configuration's fields and energy (FP32 fields mode)."""
+        self.backend.call("engine_resync", self.handle)
+
+    def drift(self) -> tuple[float, float, int]:
+        """(max field error, max relative energy error, resyncs) over all resyncs so far."""
+        f, e, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_uint64()
+        self.backend.call("engine_drift", self.handle, ctypes.byref(f), ctypes.byref(e),
+                          ctypes.byref(n))
+        return f.value, e.value, n.value
 
     def set_walker_cadence(self, k: int, last_flatness_check: int, last_flatness: float):
         """The driver's flatness-check bookkeeping of walker k (parallel.cpp:364-365)."""

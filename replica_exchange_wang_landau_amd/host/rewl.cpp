@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <iostream>
 #include <sstream>
@@ -151,7 +152,13 @@ RewlResult run_rewl(std::shared_ptr<const Couplings> couplings, const RunConfig 
     cfg.exchange_interval_attempts = c.exchange.attempts;
     cfg.max_attempts = c.max_limit.attempts;
     cfg.rng = std::uint64_t(options.rng);
-    cfg.fields = WLX_FIELDS_FP64;
+    cfg.fields = std::uint64_t(options.fields);
+    if (options.fields == WLX_FIELDS_FP32_KAHAN) // attempts_from_mcs (io.cpp:75-87)
+        cfg.resync_interval_attempts = std::max<std::uint64_t>(
+            1, static_cast<std::uint64_t>(std::floor(
+                   static_cast<long double>(options.resync_interval_mcs) *
+                       static_cast<long double>(n) +
+                   0.5L)));
     cfg.explicit_windows = be.data();
     const auto &p = c.wl;
     cfg.wl = {p.flatness,
@@ -246,7 +253,7 @@ RewlResult run_rewl(std::shared_ptr<const Couplings> couplings, const RunConfig 
     std::uint64_t cus = 0;
     if (wlx_device_info(device, sizeof device, &cus) == WLX_OK) result.engine.device = device;
     result.engine.rng = options.rng == WLX_RNG_PHILOX ? "philox4x64-10" : "xoshiro256**";
-    result.engine.fields = "fp64";
+    result.engine.fields = options.fields == WLX_FIELDS_FP32_KAHAN ? "fp32-kahan" : "fp64";
     result.engine.walkers = walkers;
     result.engine.epochs = st.epoch;
     result.engine.ambiguous_decisions = st.ambiguous_decisions;

@@ -44,6 +44,8 @@ struct RewlResult { // parallel.hpp:23-31
 // Engine choices that have no INI/CLI key in the reference.
 struct EngineOptions {
     int rng = WLX_RNG_XOSHIRO;
+    int fields = WLX_FIELDS_FP64;
+    double resync_interval_mcs = 10.0; // FP32 fields: FP64 resync cadence
 };
 
 RewlResult run_rewl(std::shared_ptr<const Couplings> couplings, const RunConfig &config,

@@ -41,6 +41,15 @@ int engine_rng_from_environment()
     return (v && std::string(v) == "philox") ? WLX_RNG_PHILOX : WLX_RNG_XOSHIRO;
 }
 
+// REWL_FIELDS=fp32: FP32 Kahan fields (needs REWL_RNG=philox); REWL_RESYNC_MCS: its FP64
+// resync cadence in MCS (default 10)
+void engine_fields_from_environment(EngineOptions &options)
+{
+    const char *v = std::getenv("REWL_FIELDS");
+    if (v && std::string(v) == "fp32") options.fields = WLX_FIELDS_FP32_KAHAN;
+    if (const char *r = std::getenv("REWL_RESYNC_MCS")) options.resync_interval_mcs = std::atof(r);
+}
+
 } // namespace
 
 int main(int argc, char **argv)
@@ -100,6 +109,7 @@ int main(int argc, char **argv)
         }
         EngineOptions options;
         options.rng = engine_rng_from_environment();
+        engine_fields_from_environment(options);
         const auto start = std::chrono::steady_clock::now();
         const RewlResult result = run_rewl(couplings, config, options);
         const double elapsed =

@@ -1,0 +1,200 @@
+"""FP32 local fields with Kahan-compensated updates (WLX_FIELDS_FP32_KAHAN) and the FP64
+matrix-core resync.
+
+This mode is not the reference's arithmetic, so it is checked three ways:
+- exactly, against a sequential FP32 Kahan walker written here in numpy on the same Philox
+  stream (the lookahead must reproduce it bit for bit);
+- against the FP64 recompute within stated tolerances;
+- statistically, against the exact C1 density of states.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import math
+
+import numpy as np
+import pytest
+
+from harness import GOLDEN, c1_geometry
+from replica_exchange_wang_landau_amd import abi
+from replica_exchange_wang_landau_amd.engine import (Engine, InvalidArgument, RunConfig, System,
+                                                     WlParameters, simple_cubic)
+
+pytestmark = pytest.mark.gpu
+
+WALK = 0x57414C4B
+
+
+def philox(orc, counter0, key0, key1):
+    ctr = (ctypes.c_uint64 * 4)(counter0, WALK, 0, 0)
+    key = (ctypes.c_uint64 * 2)(key0, key1)
+    out = (ctypes.c_uint64 * 4)()
+    orc.lib.orc_philox4x64(ctr, key, out)
+    return list(out)
+
+
+def sequential_f32_walk(orc, J32, spins, fields64, energy, log_g, hist, active, cfg, walker,
+                        attempted, count, factor):
+    """One-flip-at-a-time FP32 Kahan walker in the WL stage (wang_landau.cpp:244-303 with the
+    FP32 field update), numpy float32 arithmetic, the engine's Philox stream."""
+    n = len(spins)
+    h = fields64.astype(np.float32)
+    c = (h.astype(np.float64) - fields64).astype(np.float32)
+    lib = orc.lib
+    grid_index = lambda e: lib.orc_grid_index(ctypes.c_double(cfg.emin), ctypes.c_double(cfg.emax),
+                                              ctypes.c_double(cfg.bin_width), ctypes.c_double(e))
+    begin, end = walker["begin"], walker["end"]
+    for t in range(attempted, attempted + count):
+        blk = philox(orc, t >> 1, cfg.seed, walker["id"])
+        odd = t & 1
+        site = (blk[2 if odd else 0] * n) >> 64
+        u = (blk[3 if odd else 1] >> 11) * 2.0 ** -53
+        sigma = float(spins[site])
+        hs = float(h[site]) - float(c[site])
+        proposed = energy + (-2.0 * sigma) * hs
+        cur, nxt = grid_index(energy), grid_index(proposed)
+        accepted = False
+        if begin <= nxt < end:
+            lr = log_g[cur] - log_g[nxt]
+            accepted = lr >= 0.0 or math.log(max(u, 2.0 ** -53)) < lr
+        if accepted:
+            m = np.float32(-2.0 * sigma)
+            y = m * J32[site] - c
+            tt = h + y
+            c = (tt - h) - y
+            h = tt
+            spins[site] = -spins[site]
+            energy = proposed
+            cur = nxt
+        if not active[cur]:
+            active[cur] = 1
+        hist[cur] += 1
+        log_g[cur] += factor
+    return spins, h.astype(np.float64) - c.astype(np.float64), energy, log_g, hist
+
+
+def f32_config(**kw):
+    base = dict(emin=-60.0, emax=80.0, bin_width=0.5, windows=1, walkers_per_window=1, seed=13,
+                exchange_interval_attempts=1000, rng=abi.RNG_PHILOX, fields=abi.FIELDS_FP32_KAHAN,
+                wl=WlParameters(check_interval_attempts=10**9))
+    base.update(kw)
+    return RunConfig(**base)
+
+
+@pytest.mark.parametrize("geometry", [(3, 3, 3, 1.0, (0, 0, 1), True),
+                                      (5, 7, 3, 1.0, (0.6, 0.0, 0.8), True)])
+def test_fp32_lookahead_equals_sequential_kahan_walker(gpu, orc, geometry):
+    pos, axes, box, per = simple_cubic(*geometry)
+    n = len(pos)
+    cfg = f32_config(emin=-6.0 * n, emax=6.0 * n, bin_width=0.5)
+    system = System(pos, axes, box, per, backend=gpu)
+    e = Engine(system, cfg)
+    e.initialize()
+    snap = e.read_walker(0)
+    spins = snap.spins.copy()
+    f64, _ = system.fields(spins[None, :])
+    J32 = system.dense().astype(np.float32)
+    walker = {"id": 0, "begin": int(snap.window_begin), "end": int(snap.window_end)}
+    ref = sequential_f32_walk(orc, J32, spins.copy(), f64[0], snap.energy, snap.log_g.copy(),
+                              snap.histogram.copy(), snap.active.copy(), cfg, walker,
+                              int(snap.attempted), 3000, snap.factor)
+    for chunk in (1, 63, 936, 2000):
+        e.sample(chunk)
+    got = e.read_walker(0)
+    assert np.array_equal(got.spins, ref[0])
+    assert got.fields.tobytes() == ref[1].tobytes()
+    assert got.energy.hex() == float(ref[2]).hex()
+    assert got.log_g.tobytes() == ref[3].tobytes()
+    assert np.array_equal(got.histogram, ref[4])
+
+
+def test_resync_recomputes_fp64_fields_and_energies(gpu):
+    """After resync every configuration holds J.S in FP64 (matrix cores) to within 1e-12
+    relative of the sequential local_fields, and its energy to within 1e-12 relative."""
+    geo = simple_cubic(8, 8, 8, 1.0, (0, 0, 1), True)
+    system = System(*geo, backend=gpu)
+    cfg = RunConfig(emin=-273.0, emax=354.5, bin_width=0.5, windows=8, walkers_per_window=16,
+                    overlap=0.75, seed=1234, exchange_interval_attempts=512, rng=abi.RNG_PHILOX,
+                    fields=abi.FIELDS_FP32_KAHAN,
+                    wl=WlParameters(final_factor=1e-7, check_interval_attempts=512 * 100))
+    e = Engine(system, cfg)
+    e.initialize()
+    e.run(50)
+    e.resync()
+    spins = np.stack([e.read_walker(k).spins for k in range(e.walkers)])
+    fields, energies = system.fields(spins)
+    for k in range(e.walkers):
+        w = e.read_walker(k)
+        scale = np.maximum(1.0, np.abs(fields[k]))
+        assert np.max(np.abs(w.fields - fields[k]) / scale) <= 1e-12, k
+        assert abs(w.energy - energies[k]) <= 1e-12 * max(1.0, abs(energies[k])), k
+    f, en, count = e.drift()
+    assert count == 1 and 0.0 < f < 1e-4 and en < 1e-6
+
+
+def test_fp32_drift_bounds_with_periodic_resync(gpu):
+    """C2 geometry, FP32 fields resynced every 10 MCS: field drift stays below 1e-4 absolute
+    (fields are O(10)) and energy drift below 1e-7 relative between resyncs."""
+    geo = simple_cubic(8, 8, 8, 1.0, (0, 0, 1), True)
+    cfg = RunConfig(emin=-273.0, emax=354.5, bin_width=0.5, windows=8, walkers_per_window=16,
+                    overlap=0.75, seed=7, exchange_interval_attempts=512, rng=abi.RNG_PHILOX,
+                    fields=abi.FIELDS_FP32_KAHAN, resync_interval_attempts=5120,
+                    wl=WlParameters(final_factor=1e-7, check_interval_attempts=512 * 100))
+    e = Engine(System(*geo, backend=gpu), cfg)
+    e.initialize()
+    e.run(400)
+    f, en, count = e.drift()
+    assert count == 40
+    assert f < 1e-4 and en < 1e-7, (f, en)
+
+
+def test_fp32_c1_converges_to_exact_dos(gpu):
+    """C1 (2x2x2 open) plain WL in FP32 mode to ln f = 1e-6 meets the same exact-DOS bound
+    as the FP64 path."""
+    from replica_exchange_wang_landau_amd import analysis
+    cfg = RunConfig(emin=-20.0, emax=20.0, bin_width=0.25, windows=1, walkers_per_window=2,
+                    seed=1234, exchange_interval_attempts=80, rng=abi.RNG_PHILOX,
+                    fields=abi.FIELDS_FP32_KAHAN, resync_interval_attempts=8000,
+                    wl=WlParameters(final_factor=1e-6, check_interval_attempts=800))
+    e = Engine(System(*c1_geometry(), backend=gpu), cfg)
+    e.initialize()
+    st = e.run()
+    assert st.converged
+    f = analysis.summarize_window([e.read_walker(0), e.read_walker(1)], e.bins)
+    exact = json.loads((GOLDEN / "exact_c1.json").read_text())["counts"]
+    occupied = sorted(int(b) for b in exact)
+    d = np.array([f.log_g[b] - np.log(exact[str(b)]) for b in occupied])
+    assert np.max(np.abs(d - d.mean())) < 0.03
+
+
+def test_fp32_mode_rejects_unsupported_combinations(gpu):
+    geo = simple_cubic(2, 2, 2, 1.0, (0, 0, 1), False)
+    for bad in (dict(rng=abi.RNG_XOSHIRO),
+                dict(wl=WlParameters(force_accept_after_attempts=5)),
+                dict(wl=WlParameters(nalivaiko_mod=True))):
+        with pytest.raises(InvalidArgument, match="FP32 fields run on the Philox stream"):
+            Engine(System(*geo, backend=gpu), f32_config(emin=-20, emax=20, **bad))
+    e = Engine(System(*geo, backend=gpu), f32_config(emin=-20, emax=20))
+    e.initialize()
+    with pytest.raises(InvalidArgument, match="not recorded with FP32 fields"):
+        e.trace(10)
+
+
+def test_wl_run_fp32_fields_from_environment(tmp_path):
+    """wl_run with REWL_RNG=philox REWL_FIELDS=fp32: C1 converges and the metadata's engine
+    object records the field mode."""
+    import os
+    import subprocess
+    from pathlib import Path
+    exe = Path(__file__).resolve().parents[1] / "replica_exchange_wang_landau_amd" / "bin" / "wl_run"
+    env = dict(os.environ, REWL_RNG="philox", REWL_FIELDS="fp32", REWL_RESYNC_MCS="100")
+    r = subprocess.run([str(exe), "--nx", "2", "--ny", "2", "--nz", "2", "--periodic", "false",
+                        "--emin", "-20", "--emax", "20", "--bin-width", "0.25", "--walkers", "2",
+                        "--seed", "1234", "--final-factor", "1e-4", "--max-mcs", "0",
+                        "--output", "f32"], cwd=tmp_path, env=env, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "converged=yes" in r.stdout
+    meta = json.loads((tmp_path / "f32_metadata.json").read_text())
+    assert meta["engine"]["fields"] == "fp32-kahan" and meta["engine"]["rng"] == "philox4x64-10"
