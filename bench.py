@@ -19,6 +19,13 @@ N * 8 bytes / mean launch duration (HIP events on the engine stream).
 
 cpu_baseline runs the reference's own run_rewl (compiled from its sources into oracle/_ref)
 on the same workload with OpenMP on the host cores, bounded to a fixed attempt budget.
+
+time_to_accuracy runs a fresh C2 engine and reports the device time until its stitched DOS
+is within max|dlog g| < 0.01 of a converged C2 reference DOS, after mean-offset alignment
+(tests/golden/c2_reference_dos.json, made by tests/golden/make_c2_reference_dos.py).
+
+`--c4` runs only the C4 precision study instead: a 16x16x16 dense system with FP32 Kahan
+fields, the FP64 matrix-core resync every 10 MCS, and the drift it measures.
 """
 from __future__ import annotations
 
