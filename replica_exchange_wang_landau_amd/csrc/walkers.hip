@@ -962,6 +962,204 @@ __global__ __launch_bounds__(kWave) void lookahead_fast_kernel(Args a, uint64_t 
 
 
 // ---------------------------------------------------------------------------------------------
+// lookahead sampling on the reference's Xoshiro256** stream (common configuration: no trace,
+// forcing or Nalivaiko mask). The walker consumes its stream word by word: a site draw (one
+// word, repeated while bounded() rejects it, rng.hpp:29-36), then a uniform only when the
+// decision needs one (wang_landau.cpp:254-261). The words themselves are a fixed sequence, so
+// a wave generates the next 64 on the scalar unit and gives word l to lane l. A batch covers
+// 16 stream positions: every position is gathered as a potential site (field, spin and the
+// couplings among them), and the serial chain then walks the positions. A position is an
+// attempt when it is the next site draw, a uniform when the previous attempt drew one, or a
+// rejected draw. The words consumed set the walker's final Xoshiro state.
+// ---------------------------------------------------------------------------------------------
+__device__ inline void xoshiro_advance(Xoshiro &r, uint32_t steps)
+{
+    for (uint32_t k = 0; k < steps; ++k) (void)r.next();
+}
+
+template <int NPL>
+__global__ __launch_bounds__(kWave) void lookahead_xo_kernel(Args a, uint64_t request,
+                                                             uint32_t flags)
+{
+    constexpr int KB = kLookaheadBatch;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t w = blockIdx.x, lane = lane_id();
+    if ((flags & kGate) && a.glob->done) return;
+    WalkerState st = a.state[w];
+    if (st.escaped) return;
+    const uint32_t ws = st.end - st.begin;
+    Bins v = bins_view(smem, a.ws_stride);
+    uint64_t *flipbox = reinterpret_cast<uint64_t *>(smem + bins_lds_bytes(a.ws_stride));
+    load_bins(a, w, ws, v);
+    flipbox[lane] = 0;
+    uint64_t neg;
+    double h[NPL];
+    load_config<NPL>(a, st.slot, neg, h);
+    double energy = a.energy[st.slot];
+    __syncthreads();
+    const uint64_t count = batch_count(a, st, request, flags);
+    const int64_t start_bin = grid_index(a.grid, energy);
+    Xoshiro rng{st.rng[0], st.rng[1], st.rng[2], st.rng[3]};
+    if (count > 0 && (start_bin < int64_t(st.begin) || start_bin >= int64_t(st.end))) {
+        st.escaped = 1; // wang_landau.cpp:247-248
+    } else if (count > 0) {
+        const uint32_t begin = st.begin;
+        const uint64_t att0 = st.attempted;
+        uint32_t cur = uint32_t(start_bin);
+        double lg_cur = v.lg[cur - begin];
+        bool cur_seen = v.act[cur - begin] != 0;
+        uint32_t stage = st.stage, active = st.active_count;
+        double factor = st.factor;
+        uint32_t done_total = 0, n_acc = 0, last_acc = 0;
+        Xoshiro block_start = rng; // state before word 0 of the current block
+        uint32_t p_site = 0, p_rej = 0, next = 0; // next: block position of the next site draw
+        bool have_block = false;
+        double p_lu = 0.0;
+        uint64_t remaining = count;
+        while (remaining > 0) {
+            if (!have_block || next >= kWave - 1) { // 64 words from the current position
+                if (have_block) xoshiro_advance(block_start, next);
+                have_block = true;
+                Xoshiro gen = block_start;
+                uint64_t word = 0;
+                for (uint32_t l = 0; l < kWave; ++l) {
+                    const uint64_t x = gen.next();
+                    word = lane == l ? x : word;
+                }
+                const bool rejected = word < a.bounded.threshold;
+                const uint64_t site = a.bounded.mask ? (word & a.bounded.mask)
+                                      : (a.bounded.n == 1 ? 0 : word % a.bounded.n);
+                p_site = uint32_t(site);
+                p_rej = rejected ? 1u : 0u;
+                const double u = double(word >> 11) * 0x1.0p-53;
+                p_lu = log(fmax(u, 0x1.0p-53));
+                next = 0;
+            }
+            // positions next .. next + B - 1 of the block are site candidates; word 63 only
+            // ever serves as a uniform of the block's last attempt
+            const uint32_t pos = next;
+            const uint32_t B = min(uint32_t(KB), kWave - 1 - pos);
+            double h_run = gather_field<NPL>(h, p_site);
+            const uint32_t src = p_site & (kWave - 1);
+            const uint64_t bits =
+                (uint64_t(uint32_t(__shfl(int(uint32_t(neg >> 32)), int(src)))) << 32) |
+                uint32_t(__shfl(int(uint32_t(neg)), int(src)));
+            uint32_t dsig_hi = ((bits >> (p_site >> 6)) & 1) ? 0x40000000u : 0xC0000000u;
+            double jc[KB];
+#pragma unroll
+            for (int q = 0; q < KB; ++q) {
+                const uint32_t sq = __builtin_amdgcn_readlane(
+                    p_site, min(pos + uint32_t(q), uint32_t(kWave - 1)));
+                jc[q] = a.J[size_t(sq) * a.np + p_site];
+            }
+            // lane j: the log_g increment of the batch's j-th attempt
+            double inc = batch_factors(stage, factor, active, att0 + done_total, 0);
+            uint32_t m_hi = 0, cur_lane = 0, attempts = 0;
+            double lr_lane = 0.0;
+            uint64_t attempt_lanes = 0;
+#pragma unroll
+            for (int b = 0; b < KB; ++b) {
+                const uint32_t L = pos + uint32_t(b);
+                if (uint32_t(b) >= B || L != next || remaining == attempts) continue;
+                if (__builtin_amdgcn_readlane(p_rej, L)) { // bounded() draws again
+                    next = L + 1;
+                    continue;
+                }
+                const double hs = readlane_f64(h_run, L);
+                const uint32_t ms_hi = __builtin_amdgcn_readlane(dsig_hi, L);
+                const double ms = hi_double(ms_hi); // -2 sigma
+                const double proposed = energy + ms * hs; // flip_delta, physics.cpp:261-266
+                const bool in = proposed >= a.grid.minimum && proposed <= a.grid.maximum;
+                const double fb = bin_floor(a.grid, proposed);
+                int32_t idx = int32_t(in ? fb : 0.0);
+                if (proposed == a.grid.maximum) idx = min(idx, int32_t(a.grid.count) - 1);
+                const uint32_t rel = uint32_t(idx) - begin;
+                const bool inside = in && rel < ws;
+                const uint32_t rr = inside ? rel : 0u;
+                const double lgn = v.lg[rr];
+                const bool seen_n = v.act[rr] != 0;
+                const double log_ratio = lg_cur - lgn;
+                const bool up = log_ratio >= 0.0;
+                const bool drew = inside && !up; // a uniform is drawn: the next word
+                const double lu = readlane_f64(p_lu, L + 1);
+                const bool accepted = inside && (up || lu < log_ratio);
+                lr_lane = lane == L ? (drew ? log_ratio : __builtin_huge_val()) : lr_lane;
+                lr_lane = lane == L + 1 && drew ? lu : lr_lane; // keep log(u) beside it
+                next = L + 1 + (drew ? 1u : 0u);
+                m_hi = writelane_u32(m_hi, accepted ? ms_hi : 0u, L);
+                h_run = fma(accepted ? ms : 0.0, jc[b], h_run);
+                const uint32_t s = __builtin_amdgcn_readlane(p_site, L);
+                dsig_hi ^= (accepted && p_site == s) ? 0x80000000u : 0u;
+                energy = accepted ? proposed : energy;
+                cur = accepted ? uint32_t(idx) : cur;
+                lg_cur = accepted ? lgn : lg_cur;
+                cur_seen = accepted ? seen_n : cur_seen;
+                n_acc += accepted ? 1u : 0u;
+                last_acc = accepted ? done_total + attempts + 1 : last_acc;
+                const uint32_t crel = cur - begin;
+                const double f = readlane_f64(inc, attempts);
+                if (stage == WLX_STAGE_INVERSE_TIME) {
+                    factor = f;
+                    if (factor <= a.final_factor) stage = WLX_STAGE_FROZEN;
+                }
+                if (!cur_seen) {
+                    ++active;
+                    cur_seen = true;
+                    if (lane == 0) v.act[crel] = 1;
+                    if (stage == WLX_STAGE_INVERSE_TIME)
+                        inc = lane > attempts
+                                  ? batch_factors(stage, factor, active,
+                                                  att0 + done_total + attempts + 1, attempts + 1)
+                                  : inc;
+                }
+                lg_cur = lg_cur + (stage != WLX_STAGE_FROZEN ? f : 0.0);
+                if (lane == 0) v.lg[crel] = lg_cur;
+                cur_lane = writelane_u32(cur_lane, crel, L);
+                attempt_lanes |= uint64_t(1) << L;
+                ++attempts;
+            }
+            // the batch consumed the words before `next`; from position 63 a new block follows
+            const bool mine = (attempt_lanes >> lane) & 1;
+            if (mine)
+                __hip_atomic_fetch_add(&v.H[cur_lane], uint64_t(1), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            { // drawn decisions whose log(u) (kept in the next lane) lay within 4 ulp
+                const double lu_next = __shfl_down(lr_lane, 1);
+                const bool amb = mine && lr_lane < 0.0 && near_threshold(lu_next, lr_lane);
+                st.ambiguous += __popcll(__ballot(amb));
+            }
+            if (mine && m_hi != 0)
+                __hip_atomic_fetch_xor(&flipbox[p_site & (kWave - 1)], uint64_t(1) << (p_site >> 6),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __syncthreads();
+            neg ^= flipbox[lane];
+            flipbox[lane] = 0;
+            rows_fma<NPL, KB>(a, p_site, m_hi, pos, B, h);
+            remaining -= attempts;
+            done_total += attempts;
+        }
+        rng = block_start;
+        xoshiro_advance(rng, next);
+        asm volatile("" ::: "memory");
+        const uint64_t accepted0 = st.accepted, ambiguous = st.ambiguous;
+        st = a.state[w];
+        st.attempted = att0 + done_total;
+        st.accepted = accepted0 + n_acc;
+        if (last_acc) st.last_accepted = att0 + last_acc;
+        st.stage = stage;
+        st.factor = factor;
+        st.active_count = active;
+        st.ambiguous = ambiguous;
+    }
+    st.rng[0] = rng.s0;
+    st.rng[1] = rng.s1;
+    st.rng[2] = rng.s2;
+    st.rng[3] = rng.s3;
+    __syncthreads();
+    finish_walker<NPL>(a, w, st, v, ws, neg, h, energy, flags);
+}
+
+// ---------------------------------------------------------------------------------------------
 // FP32 local fields (WLX_FIELDS_FP32_KAHAN): the field of site j is a pair (h_j, c_j) of FP32
 // numbers with value h_j - c_j. A flip with multiplier m = -2 sigma adds m J32[i][j] to every
 // field by Kahan's compensated step (Kahan 1965): y = m J32 - c; t = h + y; c = (t - h) - y;
@@ -1660,7 +1858,7 @@ using FieldsFn = void (*)(Args, const int8_t *, double *, double *);
 using PilotFn = void (*)(Args, uint64_t, uint64_t, double *);
 
 struct KernelSet {
-    SampleFn sample_xo, sample_ph, sample_ph_fast, sample_f32;
+    SampleFn sample_xo, sample_xo_fast, sample_ph, sample_ph_fast, sample_f32;
     InitFn init;
     FieldsFn fields;
     PilotFn pilot;
@@ -1668,7 +1866,8 @@ struct KernelSet {
 
 template <int NPL> KernelSet kernel_set()
 {
-    return {sample_kernel<NPL, false>, lookahead_kernel<NPL>, lookahead_fast_kernel<NPL>,
+    return {sample_kernel<NPL, false>, lookahead_xo_kernel<NPL>, lookahead_kernel<NPL>,
+            lookahead_fast_kernel<NPL>,
             lookahead_f32_kernel<NPL>,
             init_kernel<NPL>, fields_kernel<NPL>, pilot_kernel<NPL>};
 }
@@ -1829,7 +2028,9 @@ struct wlx_engine {
     SampleFn sample_fn() const
     {
         if (f32) return kernels.sample_f32;
-        if (cfg.rng != WLX_RNG_PHILOX) return kernels.sample_xo;
+        const bool common =
+            trace_cap == 0 && cfg.wl.force_accept_after_attempts == 0 && !cfg.wl.nalivaiko_mod;
+        if (cfg.rng != WLX_RNG_PHILOX) return common ? kernels.sample_xo_fast : kernels.sample_xo;
         if (trace_cap == 0 && cfg.wl.force_accept_after_attempts == 0 && !cfg.wl.nalivaiko_mod)
             return kernels.sample_ph_fast;
         return kernels.sample_ph;
@@ -2039,7 +2240,8 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
     if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.sample_xo),
                                    e->lds_sample))
         return status;
-    for (SampleFn fn : {e->kernels.sample_ph, e->kernels.sample_ph_fast, e->kernels.sample_f32})
+    for (SampleFn fn : {e->kernels.sample_xo_fast, e->kernels.sample_ph, e->kernels.sample_ph_fast,
+                        e->kernels.sample_f32})
         if (int status = set_lds_limit(reinterpret_cast<const void *>(fn), e->lds_sample))
             return status;
     if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.init), e->lds_init))

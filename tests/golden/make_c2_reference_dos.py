@@ -7,7 +7,7 @@ reference is a long plain-REWL run of this engine to convergence: every walker f
 ln f <= 1e-7. It uses its own seed, so it is independent of the runs it judges. The run takes
 roughly 6e11 attempts, which needs several GPU sessions. Each invocation runs for at most
 --seconds, checkpoints every walker (Engine.save_checkpoint) and resumes from that checkpoint
-on the next call. When all walkers are frozen, the stitched log g (stitch_dos normalisation:
+on the next call, restarting each configuration's energy and fields from an FP64 recompute. When all walkers are frozen, the stitched log g (stitch_dos normalisation:
 peak 0) and its valid mask are written.
 
     python tests/golden/make_c2_reference_dos.py --state STATE.npz --seconds 900
@@ -48,7 +48,7 @@ def main():
     engine.initialize()
     state = Path(args.state)
     if state.exists():
-        engine.load_checkpoint(state)
+        engine.load_checkpoint(state, recompute_fields=True)
         print(f"resumed from {state}", flush=True)
     t0 = time.perf_counter()
     last_print = t0

@@ -404,3 +404,44 @@ def test_multi_walker_checkpoint_resume_bit_exact(gpu, orc, tmp_path, rng_mode):
         so.epoch, so.attempted, so.accepted, so.exchange_attempted, so.exchange_accepted)
     for k in range(eo.walkers):
         assert_walkers_equal(resumed, eo, k)
+
+
+@pytest.mark.parametrize("geometry,grid", [
+    ((5, 7, 3, 1.0, (0.6, 0.0, 0.8), True), (-120.0, 120.0, 0.3)),   # N=105
+    ((16, 16, 4, 1.0, (0.0, 0.0, 1.0), True), (-700.0, 900.0, 2.0)),  # N=1024
+    ((15, 15, 15, 1.0, (0.0, 0.6, 0.8), True), (-2600.0, 3200.0, 5.0)),  # N=3375
+])
+def test_xoshiro_lookahead_uneven_sizes_bit_exact(gpu, orc, geometry, grid):
+    """The lookahead kernel on the reference's Xoshiro stream (words consumed one by one as
+    site draws and uniforms) against the sequential oracle."""
+    cfg = RunConfig(emin=grid[0], emax=grid[1], bin_width=grid[2], windows=2,
+                    walkers_per_window=2, seed=31, exchange_interval_attempts=300,
+                    max_attempts=300 * 12, rng=abi.RNG_XOSHIRO,
+                    wl=WlParameters(check_interval_attempts=600))
+    eg, eo = pair_engines(gpu, orc, simple_cubic(*geometry), cfg)
+    sg, so = eg.run(), eo.run()
+    assert (sg.attempted, sg.accepted, sg.exchange_accepted) == (
+        so.attempted, so.accepted, so.exchange_accepted)
+    for k in range(eg.walkers):
+        assert_walkers_equal(eg, eo, k)
+
+
+def test_xoshiro_lookahead_rejected_site_draw_bit_exact(gpu, orc):
+    """bounded(N) redraws a word below (2^64 - N) mod N (rng.hpp:29-36). A state with s1 = 0
+    makes the next Xoshiro256** output 0, so the walker's first site draw is rejected."""
+    geo = simple_cubic(5, 7, 3, 1.0, (0.6, 0.0, 0.8), True)  # N = 105
+    assert ((1 << 64) - 105) % 105 > 0
+    cfg = RunConfig(emin=-120.0, emax=120.0, bin_width=0.3, windows=1, walkers_per_window=2,
+                    seed=5, exchange_interval_attempts=1000, rng=abi.RNG_XOSHIRO,
+                    wl=WlParameters(check_interval_attempts=10**9))
+    eg, eo = pair_engines(gpu, orc, geo, cfg)
+    for k in range(2):
+        snap = eo.read_walker(k)
+        snap.info.rng_state[1] = 0
+        eg.write_walker(k, snap)
+        eo.write_walker(k, snap)
+    for chunk in (1, 200, 1000):
+        eg.sample(chunk)
+        eo.sample(chunk)
+        for k in range(2):
+            assert_walkers_equal(eg, eo, k)
