@@ -57,9 +57,9 @@ enum wlx_trace_flag {
 typedef struct wlx_system wlx_system;
 typedef struct wlx_engine wlx_engine;
 
-/* WlParameters, wang_landau.hpp:40-56 (booleans widened to uint64). The pilot-statistics
- * members (collect_window_statistics, round_trip_margin_fraction) belong to adaptive windows,
- * which this engine does not implement. */
+/* WlParameters, wang_landau.hpp:40-56 (booleans widened to uint64). The last two members
+ * drive the pilot statistics of adaptive windows (wang_landau.cpp:104-118, 272-276, 317-342);
+ * they sit at the end so that the reference's member order holds for everything before. */
 typedef struct wlx_params {
     double flatness;
     uint64_t minimum_visits;
@@ -74,6 +74,8 @@ typedef struct wlx_params {
     double initialization_temperature_multiplier;
     double initialization_max_temperature_fraction;
     uint64_t nalivaiko_mod;
+    uint64_t collect_window_statistics;
+    double round_trip_margin_fraction; /* in [0, 0.5) */
 } wlx_params;
 
 /* The part of RunConfig (io.hpp:12-58) that run_rewl (parallel.cpp:276-581) reads, with every
@@ -232,6 +234,16 @@ int wlx_engine_resync(wlx_engine *engine);
  * configurations, max |E - E_fp64| / |E_fp64| over configurations, and the resync count. */
 int wlx_engine_drift(wlx_engine *engine, double *max_field_error, double *max_energy_relative,
                      uint64_t *resyncs);
+/* The pilot statistics of a walker run with collect_window_statistics: squared accepted
+ * energy displacements and attempt counts per bin (full grid, zero outside the window;
+ * wang_landau.cpp:272-276), completed round trips (wang_landau.cpp:317-330), and the nine
+ * representative configurations closest to the window's energy targets
+ * (wang_landau.cpp:107-118, 332-342): energies, a present flag each, and N spins each. Any
+ * pointer may be NULL. WLX_LOGIC_ERROR when the engine does not collect them. */
+int wlx_engine_read_sampling(wlx_engine *engine, uint64_t walker, double *squared_displacement,
+                             uint64_t *displacement_samples, uint64_t *round_trips,
+                             double *representative_energies, uint8_t *representative_present,
+                             int8_t *representative_spins);
 /* WangLandauWalker::histogram_statistics (wang_landau.cpp:344-367), computed on the device. */
 int wlx_engine_histogram_statistics(wlx_engine *engine, uint64_t walker,
                                     wlx_histogram_stats *out);

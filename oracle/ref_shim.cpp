@@ -49,6 +49,8 @@ wl::WlParameters parameters(const wlx_params &p)
     q.initialization_temperature_multiplier = p.initialization_temperature_multiplier;
     q.initialization_max_temperature_fraction = p.initialization_max_temperature_fraction;
     q.nalivaiko_mod = p.nalivaiko_mod != 0;
+    q.collect_window_statistics = p.collect_window_statistics != 0;
+    q.round_trip_margin_fraction = p.round_trip_margin_fraction;
     return q;
 }
 
@@ -252,6 +254,39 @@ int ref_walker_next_iteration(void *walker)
     }
 }
 
+// the reference RunConfig of a wlx_config
+static wl::RunConfig run_config(const wlx_config *c)
+{
+    wl::RunConfig config;
+    config.grid = {c->emin, c->emax, c->bin_width};
+    config.energy_grid_explicit = true;
+    config.windows = c->windows;
+    config.walkers_per_rank = c->walkers_per_window;
+    config.overlap = c->overlap;
+    config.seed = c->seed;
+    config.seed_explicit = true;
+    config.exchange_interval_attempts = c->exchange_interval_attempts;
+    config.max_attempts = c->max_attempts;
+    config.wl = parameters(c->wl);
+    config.checkpoint_interval_attempts = 1;
+    if (c->explicit_windows)
+        for (uint64_t w = 0; w < c->windows; ++w)
+            config.explicit_windows.push_back(
+                {c->explicit_windows[2 * w], c->explicit_windows[2 * w + 1]});
+    return config;
+}
+
+static wl::RewlResult run(void *system, const wl::RunConfig &config)
+{
+    auto *s = static_cast<RefSystem *>(system);
+    int argc = 1;
+    char name[] = "ref";
+    char *args[] = {name, nullptr};
+    char **argv = args;
+    wl::ParallelContext context(argc, argv);
+    return wl::run_rewl(context, s->couplings, config);
+}
+
 // run_rewl on the system (shared-memory path). Outputs, all caller-allocated:
 //   fragments: log_g / standard_error [windows*bins], histogram [windows*bins], valid [...]
 //   per-walker: energy, factor, attempted, accepted, forced, since_last, active_bins, min_h,
@@ -263,29 +298,8 @@ int ref_run_rewl(void *system, const wlx_config *c, double *frag_log_g, double *
                  uint64_t *walker_integers, uint64_t *counters, uint64_t *converged)
 {
     try {
-        auto *s = static_cast<RefSystem *>(system);
-        int argc = 1;
-        char name[] = "ref";
-        char *args[] = {name, nullptr};
-        char **argv = args;
-        wl::ParallelContext context(argc, argv);
-        wl::RunConfig config;
-        config.grid = {c->emin, c->emax, c->bin_width};
-        config.energy_grid_explicit = true;
-        config.windows = c->windows;
-        config.walkers_per_rank = c->walkers_per_window;
-        config.overlap = c->overlap;
-        config.seed = c->seed;
-        config.seed_explicit = true;
-        config.exchange_interval_attempts = c->exchange_interval_attempts;
-        config.max_attempts = c->max_attempts;
-        config.wl = parameters(c->wl);
-        config.checkpoint_interval_attempts = 1;
-        if (c->explicit_windows)
-            for (uint64_t w = 0; w < c->windows; ++w)
-                config.explicit_windows.push_back(
-                    {c->explicit_windows[2 * w], c->explicit_windows[2 * w + 1]});
-        const auto result = wl::run_rewl(context, s->couplings, config);
+        const wl::RunConfig config = run_config(c);
+        const auto result = run(system, config);
         const auto bins = config.grid.bins();
         for (size_t w = 0; w < result.fragments.size(); ++w) {
             const auto &f = result.fragments[w];
@@ -322,6 +336,30 @@ int ref_run_rewl(void *system, const wlx_config *c, double *frag_log_g, double *
     }
 }
 
+// the pilot statistics of a reference walker built with collect_window_statistics
+int ref_walker_sampling(void *walker, double *squared, uint64_t *samples, uint64_t *round_trips,
+                        double *rep_energy, uint8_t *rep_present, int8_t *rep_spins, uint64_t n)
+{
+    try {
+        const auto *w = static_cast<const wl::WangLandauWalker *>(walker);
+        const auto &sq = w->squared_energy_displacement();
+        const auto &ds = w->displacement_samples();
+        std::copy(sq.begin(), sq.end(), squared);
+        std::copy(ds.begin(), ds.end(), samples);
+        *round_trips = w->round_trips();
+        const auto &reps = w->representatives();
+        for (size_t r = 0; r < reps.size(); ++r) {
+            rep_present[r] = reps[r].spins.empty() ? 0 : 1;
+            rep_energy[r] = reps[r].energy;
+            if (!reps[r].spins.empty())
+                std::copy(reps[r].spins.begin(), reps[r].spins.end(), rep_spins + r * n);
+        }
+        return WLX_OK;
+    } catch (const std::exception &e) {
+        return report(e);
+    }
+}
+
 // stitch_dos (analysis.cpp:45-130) over `count` fragments of `bins` bins each; fragment f
 // covers [begin_end[2f], begin_end[2f+1]). Writes the stitched log_g and valid mask.
 int ref_stitch(double emin, double emax, double width, uint64_t bins, uint64_t count,
@@ -344,6 +382,116 @@ int ref_stitch(double emin, double emax, double width, uint64_t bins, uint64_t c
                                         spins);
         std::copy(dos.log_g.begin(), dos.log_g.end(), out_log_g);
         std::copy(dos.valid.begin(), dos.valid.end(), out_valid);
+        return WLX_OK;
+    } catch (const std::exception &e) {
+        return report(e);
+    }
+}
+
+// run_rewl with collect_window_statistics: per window the summed squared displacements and
+// samples (windows x bins), round trips and walker counts; per walker its round trips; the
+// recorded representatives in result order (energies, n spins each) and their count.
+int ref_run_rewl_sampling(void *system, const wlx_config *c, double *squared, uint64_t *samples,
+                          uint64_t *round_trips, uint64_t *walkers, uint64_t *walker_round_trips,
+                          double *rep_energies, int8_t *rep_spins, uint64_t *rep_count)
+{
+    try {
+        wl::RunConfig config = run_config(c);
+        config.wl.collect_window_statistics = true;
+        const auto result = run(system, config);
+        const auto bins = config.grid.bins();
+        for (size_t w = 0; w < result.sampling_statistics.size(); ++w) {
+            const auto &s = result.sampling_statistics[w];
+            std::copy(s.squared_energy_displacement.begin(), s.squared_energy_displacement.end(),
+                      squared + w * bins);
+            std::copy(s.displacement_samples.begin(), s.displacement_samples.end(),
+                      samples + w * bins);
+            round_trips[w] = s.round_trips;
+            walkers[w] = s.walkers;
+        }
+        for (size_t k = 0; k < result.walker_statistics.size(); ++k)
+            walker_round_trips[k] = result.walker_statistics[k].round_trips;
+        size_t r = 0;
+        for (const auto &rep : result.representatives) {
+            rep_energies[r] = rep.energy;
+            std::copy(rep.spins.begin(), rep.spins.end(), rep_spins + r * rep.spins.size());
+            ++r;
+        }
+        *rep_count = r;
+        return WLX_OK;
+    } catch (const std::exception &e) {
+        return report(e);
+    }
+}
+
+// adapt_energy_windows (analysis.cpp:183-338). Fragment and sampling arrays are `count`
+// blocks of `bins` values; params = {smoothing_width, minimum_width,
+// diffusivity_floor_fraction, curvature_weight, round_trip_target, maximum_round_trip_penalty}.
+int ref_adapt_windows(double emin, double emax, double width, uint64_t bins, uint64_t count,
+                      double overlap, const double *params, const uint64_t *begin_end,
+                      const double *log_g, const uint8_t *valid, const double *squared,
+                      const uint64_t *samples, const uint64_t *round_trips,
+                      const uint64_t *walkers, uint64_t *out_begin_end)
+{
+    try {
+        std::vector<wl::DosFragment> fragments;
+        std::vector<wl::WindowSamplingStatistics> sampling;
+        for (uint64_t f = 0; f < count; ++f) {
+            const wl::EnergyWindow w{begin_end[2 * f], begin_end[2 * f + 1]};
+            fragments.push_back({w, std::vector<double>(log_g + f * bins, log_g + (f + 1) * bins),
+                                 std::vector<std::uint64_t>(bins, 0),
+                                 std::vector<double>(bins, 0.0),
+                                 std::vector<std::uint8_t>(valid + f * bins, valid + (f + 1) * bins)});
+            sampling.push_back({w, std::vector<double>(squared + f * bins, squared + (f + 1) * bins),
+                                std::vector<std::uint64_t>(samples + f * bins,
+                                                           samples + (f + 1) * bins),
+                                round_trips[f], walkers[f]});
+        }
+        wl::AdaptiveWindowParameters p;
+        p.smoothing_width = params[0];
+        p.minimum_width = params[1];
+        p.diffusivity_floor_fraction = params[2];
+        p.curvature_weight = params[3];
+        p.round_trip_target = params[4];
+        p.maximum_round_trip_penalty = params[5];
+        const auto windows =
+            wl::adapt_energy_windows({emin, emax, width}, fragments, sampling, count, overlap, p);
+        for (std::size_t k = 0; k < windows.size(); ++k) {
+            out_begin_end[2 * k] = windows[k].begin;
+            out_begin_end[2 * k + 1] = windows[k].end;
+        }
+        return WLX_OK;
+    } catch (const std::exception &e) {
+        return report(e);
+    }
+}
+
+// select_adaptive_initial_configurations (analysis.cpp:341-416) on one process: representative
+// r has energies[r] and spins[r*n..] (empty where present[r] == 0). Writes per walker its
+// spins and whether it got any, plus the missing and external warm-start counts.
+int ref_select_initial(double emin, double emax, double width, uint64_t count,
+                       const uint64_t *begin_end, uint64_t reps, const double *energies,
+                       const uint8_t *present, uint64_t n, const int8_t *spins,
+                       uint64_t per_window, int8_t *out_spins, uint8_t *out_present,
+                       uint64_t *missing, uint64_t *external)
+{
+    try {
+        std::vector<wl::EnergyWindow> windows;
+        for (uint64_t k = 0; k < count; ++k) windows.push_back({begin_end[2 * k], begin_end[2 * k + 1]});
+        std::vector<wl::EnergyRepresentative> representatives(reps);
+        for (uint64_t r = 0; r < reps; ++r) {
+            representatives[r].energy = energies[r];
+            if (present[r]) representatives[r].spins.assign(spins + r * n, spins + (r + 1) * n);
+        }
+        std::size_t m = 0, x = 0;
+        const auto out = wl::select_adaptive_initial_configurations(
+            {emin, emax, width}, windows, representatives, 1, per_window, m, x);
+        for (std::size_t k = 0; k < out.size(); ++k) {
+            out_present[k] = out[k].empty() ? 0 : 1;
+            if (!out[k].empty()) std::copy(out[k].begin(), out[k].end(), out_spins + k * n);
+        }
+        *missing = m;
+        *external = x;
         return WLX_OK;
     } catch (const std::exception &e) {
         return report(e);

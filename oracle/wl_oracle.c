@@ -426,7 +426,17 @@ typedef struct {
     uint32_t *trace_sites, *trace_bins;
     uint8_t *trace_flags;
     int escaped;
+    /* pilot statistics (collect_window_statistics), wang_landau.cpp:104-118 */
+    double *sq_disp;         /* full grid */
+    uint64_t *disp_samples;  /* full grid */
+    uint64_t round_trips;
+    int rt_state;
+    double rep_target[9], rep_dist[9], rep_energy[9];
+    uint8_t rep_present[9];
+    int8_t *rep_spins;       /* 9 x n */
 } walker_t;
+
+#define REPRESENTATIVES 9
 
 struct orc_engine {
     const orc_system *sys;
@@ -482,6 +492,48 @@ static void flip(orc_engine *e, walker_t *w, uint64_t site, double proposed)
 }
 
 /* attempt_flip (wang_landau.cpp:244-284); returns 1 accepted, 0 rejected, -1 escaped */
+/* update_round_trip_state (wang_landau.cpp:317-330) */
+static void update_round_trip_state(orc_engine *e, walker_t *w)
+{
+    if (!e->cfg.wl.collect_window_statistics) return;
+    const double lower = e->grid.minimum + (double)w->begin * e->grid.width;
+    const double upper = e->grid.minimum + (double)w->end * e->grid.width;
+    const double margin = e->cfg.wl.round_trip_margin_fraction * (upper - lower);
+    if (w->energy <= lower + margin) {
+        if (w->rt_state == 2) {
+            ++w->round_trips;
+            w->rt_state = 1;
+        } else if (w->rt_state == 0)
+            w->rt_state = 1;
+    } else if (w->energy >= upper - margin && w->rt_state == 1) {
+        w->rt_state = 2;
+    }
+}
+
+/* update_representatives (wang_landau.cpp:332-342) */
+static void update_representatives(orc_engine *e, walker_t *w)
+{
+    if (!e->cfg.wl.collect_window_statistics) return;
+    for (int r = 0; r < REPRESENTATIVES; ++r) {
+        const double distance = fabs(w->energy - w->rep_target[r]);
+        if (distance < w->rep_dist[r]) {
+            w->rep_dist[r] = distance;
+            w->rep_energy[r] = w->energy;
+            memcpy(w->rep_spins + (size_t)r * e->sys->n, w->spins, e->sys->n);
+            w->rep_present[r] = 1;
+        }
+    }
+}
+
+/* the end of every initialisation path (wang_landau.cpp:123-127, 177, 197, 219) */
+static int finish_initialization(orc_engine *e, walker_t *w)
+{
+    int status = update_current_bin(e, w);
+    update_round_trip_state(e, w);
+    update_representatives(e, w);
+    return status;
+}
+
 static int attempt_flip(orc_engine *e, walker_t *w)
 {
     const uint64_t n = e->sys->n;
@@ -530,8 +582,17 @@ static int attempt_flip(orc_engine *e, walker_t *w)
         if (forced) ++w->forced;
         w->last_accepted = w->attempted;
     }
+    if (e->cfg.wl.collect_window_statistics) { /* wang_landau.cpp:272-276 */
+        const double displacement = accepted ? delta : 0.0;
+        w->sq_disp[old_bin] += displacement * displacement;
+        ++w->disp_samples[old_bin];
+    }
     update_inverse_time_factor(e, w);
     update_current_bin(e, w);
+    if (accepted) { /* wang_landau.cpp:278-282 */
+        update_round_trip_state(e, w);
+        update_representatives(e, w);
+    }
     if (w->trace_count < e->trace_capacity) {
         uint64_t k = w->trace_count++;
         w->trace_sites[k] = (uint32_t)site;
@@ -627,9 +688,25 @@ static int initialize_walker(orc_engine *e, walker_t *w, const int8_t *initial)
     w->last_check = 0;
     w->last_flatness = 0.0;
     orc_xoshiro_seed(e->cfg.seed, w->id, w->rng);
+    if (p->collect_window_statistics) { /* wang_landau.cpp:104-118 */
+        memset(w->sq_disp, 0, e->grid.count * sizeof(double));
+        memset(w->disp_samples, 0, e->grid.count * sizeof(uint64_t));
+        w->round_trips = 0;
+        w->rt_state = 0;
+        const double lo = e->grid.minimum + (double)w->begin * e->grid.width;
+        const double hi = e->grid.minimum + (double)w->end * e->grid.width;
+        w->rep_target[0] = lo;
+        for (int r = 1; r < REPRESENTATIVES; ++r)
+            w->rep_target[r] = lo + (hi - lo) * ((double)r - 0.5) / (double)(REPRESENTATIVES - 1);
+        for (int r = 0; r < REPRESENTATIVES; ++r) {
+            w->rep_dist[r] = INFINITY;
+            w->rep_energy[r] = 0.0;
+            w->rep_present[r] = 0;
+        }
+    }
 
     int64_t initial_bin = grid_index(&e->grid, w->energy);
-    if (supplied && in_window(w, initial_bin)) return update_current_bin(e, w);
+    if (supplied && in_window(w, initial_bin)) return finish_initialization(e, w);
 
     const double lower = e->grid.minimum + (double)w->begin * e->grid.width;
     const double upper = e->grid.minimum + (double)w->end * e->grid.width;
@@ -667,7 +744,7 @@ static int initialize_walker(orc_engine *e, walker_t *w, const int8_t *initial)
     (in_window(w, grid_index(&e->grid, (c))) && fabs((c) - target) <= half)
 
     for (uint64_t step = 0; step < p->initialization_max_attempts; ++step) {
-        if (IN_BAND(w->energy)) return update_current_bin(e, w);
+        if (IN_BAND(w->energy)) return finish_initialization(e, w);
         if (since >= stall) {
             if (temperature < tmax) {
                 double raised = temperature * p->initialization_temperature_multiplier;
@@ -684,7 +761,7 @@ static int initialize_walker(orc_engine *e, walker_t *w, const int8_t *initial)
                 ++restarts;
             }
             since = 0;
-            if (IN_BAND(w->energy)) return update_current_bin(e, w);
+            if (IN_BAND(w->energy)) return finish_initialization(e, w);
         }
         uint64_t i = orc_xoshiro_bounded(w->rng, n);
         double delta = -2.0 * (double)w->spins[i] * w->fields[i];
@@ -702,7 +779,7 @@ static int initialize_walker(orc_engine *e, walker_t *w, const int8_t *initial)
         else if (since < UINT64_MAX)
             ++since;
     }
-    if (IN_BAND(w->energy)) return update_current_bin(e, w);
+    if (IN_BAND(w->energy)) return finish_initialization(e, w);
 #undef RECORD
 #undef IN_BAND
     char closest_bin[64];
@@ -740,7 +817,9 @@ static int validate_parameters(const wlx_params *p)
         !isfinite(p->initialization_temperature_multiplier) ||
         !(p->initialization_temperature_multiplier > 1.0) ||
         !isfinite(p->initialization_max_temperature_fraction) ||
-        p->initialization_max_temperature_fraction < p->initialization_temperature_fraction)
+        p->initialization_max_temperature_fraction < p->initialization_temperature_fraction ||
+        !isfinite(p->round_trip_margin_fraction) || p->round_trip_margin_fraction < 0.0 ||
+        p->round_trip_margin_fraction >= 0.5)
         return fail(WLX_INVALID_ARGUMENT, "Invalid WL parameters");
     return WLX_OK;
 }
@@ -793,6 +872,11 @@ orc_engine *orc_engine_create(const orc_system *s, const wlx_config *c)
         w->active = calloc(bins, 1);
         w->refine = calloc(bins, 1);
         w->factor = 1.0;
+        if (c->wl.collect_window_statistics) {
+            w->sq_disp = calloc(bins, sizeof(double));
+            w->disp_samples = calloc(bins, sizeof(uint64_t));
+            w->rep_spins = calloc(REPRESENTATIVES * s->n, 1);
+        }
     }
     return e;
 }
@@ -811,6 +895,9 @@ void orc_engine_destroy(orc_engine *e)
         free(w->trace_sites);
         free(w->trace_bins);
         free(w->trace_flags);
+        free(w->sq_disp);
+        free(w->disp_samples);
+        free(w->rep_spins);
     }
     free(e->w);
     free(e->window_be);
@@ -897,8 +984,10 @@ static double exchange_log_probability(const orc_engine *e, const walker_t *a,
     return a->log_g[x_here] - a->log_g[y_here] + b->log_g[y_here] - b->log_g[x_here];
 }
 
-/* swap_configuration (wang_landau.cpp:483-495): estimators stay with their walkers */
-static void swap_configuration(const orc_engine *e, walker_t *a, walker_t *b)
+/* swap_configuration (wang_landau.cpp:483-495): estimators stay with their walkers; with
+ * collect_window_statistics both walkers update their round-trip state and representatives
+ * from the configuration they received */
+static void swap_configuration(orc_engine *e, walker_t *a, walker_t *b)
 {
     int8_t *spins = a->spins;
     a->spins = b->spins;
@@ -909,7 +998,10 @@ static void swap_configuration(const orc_engine *e, walker_t *a, walker_t *b)
     double energy = a->energy;
     a->energy = b->energy;
     b->energy = energy;
-    (void)e;
+    update_round_trip_state(e, a);
+    update_round_trip_state(e, b);
+    update_representatives(e, a);
+    update_representatives(e, b);
 }
 
 /* local_exchange (parallel.cpp:157-166) */
@@ -971,6 +1063,28 @@ int orc_engine_status(orc_engine *e, wlx_run_status *st)
     st->converged = (uint64_t)converged;
     st->exchange_attempted = e->exchange_attempted;
     st->exchange_accepted = e->exchange_accepted;
+    return WLX_OK;
+}
+
+int orc_engine_read_sampling(orc_engine *e, uint64_t k, double *squared, uint64_t *samples,
+                             uint64_t *round_trips, double *rep_energy, uint8_t *rep_present,
+                             int8_t *rep_spins)
+{
+    if (k >= e->walkers) return fail(WLX_OUT_OF_RANGE, "Walker index out of range");
+    if (!e->cfg.wl.collect_window_statistics)
+        return fail(WLX_LOGIC_ERROR, "Window statistics are not collected");
+    const walker_t *w = &e->w[k];
+    const uint64_t bins = e->grid.count;
+    if (squared) memcpy(squared, w->sq_disp, bins * sizeof(double));
+    if (samples) memcpy(samples, w->disp_samples, bins * sizeof(uint64_t));
+    if (round_trips) *round_trips = w->round_trips;
+    for (int r = 0; r < REPRESENTATIVES; ++r) {
+        if (rep_energy) rep_energy[r] = w->rep_energy[r];
+        if (rep_present) rep_present[r] = w->rep_present[r];
+        if (rep_spins && w->rep_present[r])
+            memcpy(rep_spins + (size_t)r * e->sys->n, w->rep_spins + (size_t)r * e->sys->n,
+                   e->sys->n);
+    }
     return WLX_OK;
 }
 

@@ -54,6 +54,8 @@ class Params(ctypes.Structure):
         ("initialization_temperature_multiplier", c_f64),
         ("initialization_max_temperature_fraction", c_f64),
         ("nalivaiko_mod", c_u64),
+        ("collect_window_statistics", c_u64),
+        ("round_trip_margin_fraction", ctypes.c_double),
     ]
 
 
@@ -150,6 +152,7 @@ EXPORTS = [
     "wlx_engine_set_walker_cadence",
     "wlx_engine_resync",
     "wlx_engine_drift",
+    "wlx_engine_read_sampling",
     "wlx_engine_histogram_statistics",
     "wlx_engine_ready",
     "wlx_engine_next_iteration",

@@ -64,8 +64,17 @@ struct InitParams {
     double max_temperature_fraction;
 };
 
+struct SamplingState; // pilot statistics, defined with their kernels
+
 struct Args {
     const double *J;
+    // pilot statistics (collect_window_statistics); collect_offset: their LDS position
+    double *sq_disp;            // walkers x ws_stride
+    uint64_t *disp_samples;     // walkers x ws_stride
+    SamplingState *sampling;    // walkers
+    int8_t *rep_spins;          // walkers x 9 x np
+    uint32_t collect, collect_offset;
+    double rt_margin;
     const float *J32;   // FP32 couplings (FP32-field engines)
     float2 *fields32;   // FP32 fields as (h, c) pairs, value h - c (FP32-field engines)
     uint32_t n, np;
@@ -193,6 +202,137 @@ __device__ inline void update_current_bin(const Args &a, WalkerState &st, Bins v
         v.act[b] = 1;
         if (a.nalivaiko) v.ref[b] = 1;
         v.lg[b] = lg;
+    }
+}
+
+// ---- pilot statistics (collect_window_statistics, wang_landau.cpp:104-118, 272-282, 317-342)
+constexpr int kRepresentatives = 9;
+
+struct SamplingState {
+    uint64_t round_trips;
+    uint32_t rt_state, present; // present: bit r set once representative r is recorded
+    double target[kRepresentatives], dist[kRepresentatives], energy[kRepresentatives];
+};
+
+struct SamplingView { // the walker's statistics staged in LDS
+    double *sq;
+    uint64_t *cnt;
+    SamplingState *st;
+};
+
+__device__ inline SamplingView sampling_view(const Args &a, unsigned char *smem)
+{
+    double *sq = reinterpret_cast<double *>(smem + a.collect_offset);
+    uint64_t *cnt = reinterpret_cast<uint64_t *>(sq + a.ws_stride);
+    return {sq, cnt, reinterpret_cast<SamplingState *>(cnt + a.ws_stride)};
+}
+
+__device__ inline void load_sampling(const Args &a, uint32_t w, uint32_t ws, SamplingView c)
+{
+    const size_t base = size_t(w) * a.ws_stride;
+    for (uint32_t b = lane_id(); b < ws; b += kWave) {
+        c.sq[b] = a.sq_disp[base + b];
+        c.cnt[b] = a.disp_samples[base + b];
+    }
+    if (lane_id() == 0) *c.st = a.sampling[w];
+}
+
+__device__ inline void store_sampling(const Args &a, uint32_t w, uint32_t ws, SamplingView c)
+{
+    const size_t base = size_t(w) * a.ws_stride;
+    for (uint32_t b = lane_id(); b < ws; b += kWave) {
+        a.sq_disp[base + b] = c.sq[b];
+        a.disp_samples[base + b] = c.cnt[b];
+    }
+    if (lane_id() == 0) a.sampling[w] = *c.st;
+}
+
+// squared accepted displacement and one sample at the bin the attempt started from
+__device__ inline void record_displacement(SamplingView c, uint32_t old_rel, double displacement)
+{
+    if (lane_id() == 0) {
+        c.sq[old_rel] += displacement * displacement;
+        c.cnt[old_rel] += 1;
+    }
+}
+
+// update_round_trip_state (wang_landau.cpp:317-330): state 0 = not yet at the lower margin,
+// 1 = last at the lower margin, 2 = reached the upper margin since; a return to the lower
+// margin from 2 completes a round trip
+__device__ inline void round_trip_step(const Args &a, uint32_t begin, uint32_t end, double energy,
+                                       uint32_t &state, uint64_t &trips)
+{
+    const double lower = a.grid.minimum + double(begin) * a.grid.width;
+    const double upper = a.grid.minimum + double(end) * a.grid.width;
+    const double margin = a.rt_margin * (upper - lower);
+    if (energy <= lower + margin) {
+        if (state == 2) {
+            ++trips;
+            state = 1;
+        } else if (state == 0) {
+            state = 1;
+        }
+    } else if (energy >= upper - margin && state == 1) {
+        state = 2;
+    }
+}
+
+// update_round_trip_state and update_representatives (wang_landau.cpp:332-342) of a walker
+// wave after an accepted flip or its placement
+template <int NPL>
+__device__ inline void round_trip_and_representatives(const Args &a, SamplingView c, uint32_t w,
+                                                      uint32_t begin, uint32_t end,
+                                                      double energy, uint64_t neg)
+{
+    const uint32_t lane = lane_id();
+    uint32_t state = c.st->rt_state;
+    uint64_t trips = c.st->round_trips;
+    round_trip_step(a, begin, end, energy, state, trips);
+    const uint32_t present = c.st->present;
+    uint32_t improved = 0;
+    for (int r = 0; r < kRepresentatives; ++r) {
+        const double d = fabs(energy - c.st->target[r]);
+        if (d < c.st->dist[r]) {
+            improved |= 1u << r;
+            int8_t *out = a.rep_spins + (size_t(w) * kRepresentatives + r) * a.np;
+#pragma unroll
+            for (int k = 0; k < NPL; ++k)
+                out[k * kWave + lane] = ((neg >> k) & 1) ? int8_t(-1) : int8_t(1);
+        }
+    }
+    if (lane == 0) {
+        c.st->rt_state = state;
+        c.st->round_trips = trips;
+        for (int r = 0; r < kRepresentatives; ++r)
+            if ((improved >> r) & 1) {
+                c.st->dist[r] = fabs(energy - c.st->target[r]);
+                c.st->energy[r] = energy;
+            }
+        c.st->present = present | improved;
+    }
+}
+
+// the same two updates by one thread for a walker that received configuration `slot` in an
+// accepted replica exchange (swap_configuration, wang_landau.cpp:483-495)
+__device__ inline void exchange_statistics(const Args &a, uint32_t w, const WalkerState &s,
+                                           uint32_t slot, double energy)
+{
+    SamplingState &z = a.sampling[w];
+    uint32_t state = z.rt_state;
+    uint64_t trips = z.round_trips;
+    round_trip_step(a, s.begin, s.end, energy, state, trips);
+    z.rt_state = state;
+    z.round_trips = trips;
+    for (int r = 0; r < kRepresentatives; ++r) {
+        const double d = fabs(energy - z.target[r]);
+        if (d < z.dist[r]) {
+            z.dist[r] = d;
+            z.energy[r] = energy;
+            z.present |= 1u << r;
+            const int8_t *from = a.spins + size_t(slot) * a.np;
+            int8_t *to = a.rep_spins + (size_t(w) * kRepresentatives + r) * a.np;
+            for (uint32_t i = 0; i < a.np; ++i) to[i] = from[i];
+        }
     }
 }
 
@@ -421,6 +561,11 @@ __global__ __launch_bounds__(kWave) void sample_kernel(Args a, uint64_t request,
     const uint64_t trace_base = uint64_t(w) * a.trace_cap;
     int64_t cur = grid_index(a.grid, energy);
     Philox4 block{};
+    const SamplingView col = sampling_view(a, smem);
+    if (a.collect) {
+        load_sampling(a, w, ws, col);
+        __syncthreads();
+    }
     for (uint64_t t = 0; t < count; ++t) {
         uint64_t site;
         if (PHILOX) {
@@ -464,6 +609,7 @@ __global__ __launch_bounds__(kWave) void sample_kernel(Args a, uint64_t request,
             }
         }
         ++st.attempted;
+        const uint32_t old_rel = uint32_t(cur - st.begin);
         if (accepted) {
             apply_flip<NPL>(a, s, sigma, neg, h);
             energy = proposed;
@@ -472,12 +618,15 @@ __global__ __launch_bounds__(kWave) void sample_kernel(Args a, uint64_t request,
             st.last_accepted = st.attempted;
             cur = next;
         }
+        if (a.collect) record_displacement(col, old_rel, accepted ? delta : 0.0);
         if (st.stage == WLX_STAGE_INVERSE_TIME) { // update_inverse_time_factor, 309-315
             if (st.active_count != 0 && st.attempted != 0)
                 st.factor = double(st.active_count) / double(st.attempted);
             if (st.factor <= a.final_factor) st.stage = WLX_STAGE_FROZEN;
         }
         update_current_bin(a, st, v, uint32_t(cur - st.begin));
+        if (a.collect && accepted)
+            round_trip_and_representatives<NPL>(a, col, w, st.begin, st.end, energy, neg);
         if ((flags & kTrace) && st.trace_count < a.trace_cap) {
             if (lane == 0) {
                 const uint64_t k = trace_base + st.trace_count;
@@ -495,6 +644,10 @@ __global__ __launch_bounds__(kWave) void sample_kernel(Args a, uint64_t request,
     st.rng[1] = rng.s1;
     st.rng[2] = rng.s2;
     st.rng[3] = rng.s3;
+    if (a.collect) {
+        __syncthreads();
+        store_sampling(a, w, ws, col);
+    }
     finish_walker<NPL>(a, w, st, v, ws, neg, h, energy, flags);
 }
 
@@ -1450,11 +1603,14 @@ typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 // H[s][i] = sum_k J[k][i] S[k][s] with v_mfma_f64_16x16x4_f64: one wave per 32 sites x 32
 // slots, four 16x16 accumulator blocks, K stepped by 4. J is bitwise symmetric, so the A
 // operand J[i][k] is read as J[k][i] (consecutive lanes, consecutive addresses). Operand
-// layout of the instruction: lane l holds A[l % 16][l / 16], B[l / 16][l % 16] and
-// D[4 (l / 16) + r][l % 16] in accumulator register r.
+// layout of the instruction on gfx950: lane l holds A[l % 16][l / 16], B[l / 16][l % 16] and
+// D[4 r + l / 16][l % 16] in accumulator register r (rows interleaved across the four lane
+// groups; checked with constant-row and constant-column operands). J has n rows of np
+// columns: rows k >= n (the padding of the last K step) enter as zero and are never read.
 __global__ __launch_bounds__(kWave) void resync_mfma_kernel(const double *J, const double *S,
-                                                           uint32_t np, uint32_t slots_padded,
-                                                           uint32_t slots, double *H)
+                                                           uint32_t n, uint32_t np,
+                                                           uint32_t slots_padded, uint32_t slots,
+                                                           double *H)
 {
     const uint32_t lane = threadIdx.x;
     const uint32_t tiles_i = np / 32;
@@ -1466,10 +1622,12 @@ __global__ __launch_bounds__(kWave) void resync_mfma_kernel(const double *J, con
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) acc[x][y] = mfma_d4{0.0, 0.0, 0.0, 0.0};
-    for (uint32_t k0 = 0; k0 < np; k0 += 4) {
-        const double *jrow = J + size_t(k0 + kk) * np + i0 + row;
-        const double *srow = S + size_t(k0 + kk) * slots_padded + s0 + row;
-        const double a0 = jrow[0], a1 = jrow[16];
+    for (uint32_t k0 = 0; k0 < n; k0 += 4) {
+        const uint32_t k = k0 + kk; // < np: n <= np and np is a multiple of 64
+        const bool real = k < n;
+        const double *jrow = J + size_t(real ? k : 0) * np + i0 + row;
+        const double *srow = S + size_t(k) * slots_padded + s0 + row;
+        const double a0 = real ? jrow[0] : 0.0, a1 = real ? jrow[16] : 0.0;
         const double b0 = srow[0], b1 = srow[16];
         acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
         acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
@@ -1482,7 +1640,7 @@ __global__ __launch_bounds__(kWave) void resync_mfma_kernel(const double *J, con
         for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const uint32_t i = i0 + 16 * x + 4 * kk + r, s = s0 + 16 * y + row;
+                const uint32_t i = i0 + 16 * x + 4 * r + kk, s = s0 + 16 * y + row;
                 if (s < slots) H[size_t(s) * np + i] = acc[x][y][r];
             }
 }
@@ -1675,6 +1833,14 @@ __global__ __launch_bounds__(kWave) void init_kernel(Args a, const int8_t *initi
     if (placed) {
         __syncthreads();
         update_current_bin(a, st, v, uint32_t(grid_index(a.grid, energy) - begin));
+        if (a.collect) { // wang_landau.cpp:123-127, 177, 197, 219
+            const SamplingView col = sampling_view(a, smem);
+            load_sampling(a, w, ws, col);
+            __syncthreads();
+            round_trip_and_representatives<NPL>(a, col, w, st.begin, st.end, energy, neg);
+            __syncthreads();
+            store_sampling(a, w, ws, col);
+        }
     } else {
         rep.status = 1;
     }
@@ -1790,6 +1956,10 @@ __global__ __launch_bounds__(kExchangeBlock) void exchange_kernel(Args a, uint32
             l.slot = r.slot;
             r.slot = slot;
             ++taken;
+            if (a.collect) { // the walkers now hold energies er (L) and el (R)
+                exchange_statistics(a, L, l, l.slot, er);
+                exchange_statistics(a, R, r, r.slot, el);
+            }
         }
     }
     // termination flags (parallel.cpp:391-393, 527-530)
@@ -1858,7 +2028,7 @@ using FieldsFn = void (*)(Args, const int8_t *, double *, double *);
 using PilotFn = void (*)(Args, uint64_t, uint64_t, double *);
 
 struct KernelSet {
-    SampleFn sample_xo, sample_xo_fast, sample_ph, sample_ph_fast, sample_f32;
+    SampleFn sample_xo, sample_xo_fast, sample_ph, sample_ph_fast, sample_f32, sample_ph_seq;
     InitFn init;
     FieldsFn fields;
     PilotFn pilot;
@@ -1868,7 +2038,7 @@ template <int NPL> KernelSet kernel_set()
 {
     return {sample_kernel<NPL, false>, lookahead_xo_kernel<NPL>, lookahead_kernel<NPL>,
             lookahead_fast_kernel<NPL>,
-            lookahead_f32_kernel<NPL>,
+            lookahead_f32_kernel<NPL>, sample_kernel<NPL, true>,
             init_kernel<NPL>, fields_kernel<NPL>, pilot_kernel<NPL>};
 }
 
@@ -1981,6 +2151,13 @@ struct wlx_engine {
     Drift *d_drift = nullptr;
     uint32_t slots_padded = 0;
     uint64_t resync_every = 0, since_resync = 0, resyncs = 0;
+    // pilot statistics (collect_window_statistics)
+    bool collect = false;
+    uint32_t collect_offset = 0;
+    double *d_sq = nullptr;
+    uint64_t *d_cnt = nullptr;
+    SamplingState *d_sampling = nullptr;
+    int8_t *d_rep = nullptr;
 
     Args args(uint32_t flags_trace = 0) const
     {
@@ -2021,6 +2198,13 @@ struct wlx_engine {
         a.nalivaiko = cfg.wl.nalivaiko_mod ? 1 : 0;
         a.philox = cfg.rng == WLX_RNG_PHILOX ? 1 : 0;
         a.unlimited = cfg.max_attempts == 0 ? 1 : 0;
+        a.sq_disp = d_sq;
+        a.disp_samples = d_cnt;
+        a.sampling = d_sampling;
+        a.rep_spins = d_rep;
+        a.collect = collect ? 1 : 0;
+        a.collect_offset = collect_offset;
+        a.rt_margin = cfg.wl.round_trip_margin_fraction;
         (void)flags_trace;
         return a;
     }
@@ -2028,6 +2212,8 @@ struct wlx_engine {
     SampleFn sample_fn() const
     {
         if (f32) return kernels.sample_f32;
+        // the statistics are recorded by the one-flip-at-a-time kernels
+        if (collect) return cfg.rng == WLX_RNG_PHILOX ? kernels.sample_ph_seq : kernels.sample_xo;
         const bool common =
             trace_cap == 0 && cfg.wl.force_accept_after_attempts == 0 && !cfg.wl.nalivaiko_mod;
         if (cfg.rng != WLX_RNG_PHILOX) return common ? kernels.sample_xo_fast : kernels.sample_xo;
@@ -2053,7 +2239,8 @@ struct wlx_engine {
                            walkers, slots_padded, d_S);
         REWL_HIP(hipGetLastError());
         hipLaunchKernelGGL(resync_mfma_kernel, dim3((np / 32) * (slots_padded / 32)), dim3(kWave),
-                           0, stream, sys->d_J64, d_S, np, slots_padded, walkers, d_H);
+                           0, stream, sys->d_J64, d_S, uint32_t(sys->n), np, slots_padded,
+                           walkers, d_H);
         REWL_HIP(hipGetLastError());
         hipLaunchKernelGGL(resync_finish_kernel, dim3(walkers), dim3(kWave), 0, stream, args(),
                            d_H, d_drift);
@@ -2204,6 +2391,9 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
         return set_error(WLX_INVALID_ARGUMENT,
                          "FP32 fields run on the Philox stream without forced acceptance or the "
                          "Nalivaiko mask");
+    if (c->fields == WLX_FIELDS_FP32_KAHAN && c->wl.collect_window_statistics)
+        return set_error(WLX_INVALID_ARGUMENT,
+                         "Window statistics are collected with FP64 fields only");
     if (int status = validate_parameters(c->wl)) return status;
     auto e = std::make_unique<wlx_engine>();
     e->sys = sys;
@@ -2235,13 +2425,20 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
     e->kernels = kernels_for(sys->npl);
     e->lds_sample = bins_lds_bytes(e->ws_stride) + kFlipBoxBytes;
     e->lds_init = e->lds_sample + sys->np * sizeof(double);
+    if (c->wl.collect_window_statistics) { // per-bin sums and the walker's state after init's scratch
+        e->collect = true;
+        e->collect_offset = uint32_t((e->lds_init + 15) / 16 * 16);
+        const size_t end = e->collect_offset + 16 * size_t(e->ws_stride) + sizeof(SamplingState);
+        e->lds_sample = end;
+        e->lds_init = end;
+    }
     if (e->lds_init > 160 * 1024)
         return set_error(WLX_INVALID_ARGUMENT, "Energy windows too wide for LDS residency");
     if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.sample_xo),
                                    e->lds_sample))
         return status;
     for (SampleFn fn : {e->kernels.sample_xo_fast, e->kernels.sample_ph, e->kernels.sample_ph_fast,
-                        e->kernels.sample_f32})
+                        e->kernels.sample_f32, e->kernels.sample_ph_seq})
         if (int status = set_lds_limit(reinterpret_cast<const void *>(fn), e->lds_sample))
             return status;
     if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.init), e->lds_init))
@@ -2276,6 +2473,12 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
         const uint64_t x = c->exchange_interval_attempts;
         e->resync_every = c->resync_interval_attempts ? (c->resync_interval_attempts + x - 1) / x : 0;
     }
+    if (e->collect) {
+        if (int st = device_alloc(&e->d_sq, W * S)) return st;
+        if (int st = device_alloc(&e->d_cnt, W * S)) return st;
+        if (int st = device_alloc(&e->d_sampling, W)) return st;
+        if (int st = device_alloc(&e->d_rep, W * kRepresentatives * np)) return st;
+    }
     std::vector<WalkerState> states(W);
     for (uint32_t k = 0; k < W; ++k) {
         WalkerState &s = states[k];
@@ -2303,7 +2506,8 @@ int wlx_engine_destroy(wlx_engine *e)
     void *buffers[] = {e->d_state, e->d_spins,  e->d_fields, e->d_energy, e->d_logg,
                        e->d_hist,  e->d_active, e->d_refine, e->d_tsite,  e->d_tbin,
                        e->d_tflag, e->d_glob,   e->d_report, e->d_op,     e->d_fields32,
-                       e->d_S,     e->d_H,      e->d_drift};
+                       e->d_S,     e->d_H,      e->d_drift,  e->d_sq,     e->d_cnt,
+                       e->d_sampling, e->d_rep};
     for (void *p : buffers)
         if (p) (void)hipFree(p);
     for (hipEvent_t ev : e->events) (void)hipEventDestroy(ev);
@@ -2349,6 +2553,24 @@ int wlx_engine_initialize(wlx_engine *e, const int8_t *initial, const uint8_t *s
                   e->cfg.wl.initialization_stall_attempts_per_spin,
                   e->cfg.wl.initialization_temperature_multiplier,
                   e->cfg.wl.initialization_max_temperature_fraction};
+    if (e->collect) { // statistics restart with the walker (wang_landau.cpp:104-118)
+        REWL_HIP(hipMemsetAsync(e->d_sq, 0, W * e->ws_stride * sizeof(double), e->stream));
+        REWL_HIP(hipMemsetAsync(e->d_cnt, 0, W * e->ws_stride * sizeof(uint64_t), e->stream));
+        std::vector<SamplingState> sampling(W);
+        for (uint32_t k = 0; k < W; ++k) {
+            SamplingState &z = sampling[k];
+            std::memset(&z, 0, sizeof z);
+            const double lo = e->grid.minimum + double(states[k].begin) * e->grid.width;
+            const double hi = e->grid.minimum + double(states[k].end) * e->grid.width;
+            z.target[0] = lo;
+            for (int r = 1; r < kRepresentatives; ++r)
+                z.target[r] = lo + (hi - lo) * (double(r) - 0.5) / double(kRepresentatives - 1);
+            for (int r = 0; r < kRepresentatives; ++r) z.dist[r] = HUGE_VAL;
+        }
+        REWL_HIP(hipMemcpyAsync(e->d_sampling, sampling.data(), W * sizeof(SamplingState),
+                                hipMemcpyHostToDevice, e->stream));
+        REWL_HIP(hipStreamSynchronize(e->stream));
+    }
     hipLaunchKernelGGL(e->kernels.init, dim3(e->walkers), dim3(kWave), e->lds_init, e->stream,
                        e->args(), d_initial, d_supplied, ip, e->d_report);
     REWL_HIP(hipGetLastError());
@@ -2463,6 +2685,49 @@ int wlx_engine_drift(wlx_engine *e, double *max_field_error, double *max_energy_
     if (max_field_error) *max_field_error = f;
     if (max_energy_relative) *max_energy_relative = en;
     if (resyncs) *resyncs = e->resyncs;
+    return WLX_OK;
+}
+
+int wlx_engine_read_sampling(wlx_engine *e, uint64_t w, double *squared_displacement,
+                             uint64_t *displacement_samples, uint64_t *round_trips,
+                             double *representative_energies, uint8_t *representative_present,
+                             int8_t *representative_spins)
+{
+    if (int status = require_walker(e, w)) return status;
+    if (!e->collect) return set_error(WLX_LOGIC_ERROR, "Window statistics are not collected");
+    WalkerState st{};
+    SamplingState z{};
+    REWL_HIP(hipMemcpyAsync(&st, e->d_state + w, sizeof st, hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipMemcpyAsync(&z, e->d_sampling + w, sizeof z, hipMemcpyDeviceToHost, e->stream));
+    const size_t S = e->ws_stride, np = e->sys->np, n = e->sys->n;
+    std::vector<double> sq(S);
+    std::vector<uint64_t> cnt(S);
+    std::vector<int8_t> spins(kRepresentatives * np);
+    REWL_HIP(hipMemcpyAsync(sq.data(), e->d_sq + w * S, S * sizeof(double), hipMemcpyDeviceToHost,
+                            e->stream));
+    REWL_HIP(hipMemcpyAsync(cnt.data(), e->d_cnt + w * S, S * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, e->stream));
+    if (representative_spins)
+        REWL_HIP(hipMemcpyAsync(spins.data(), e->d_rep + w * kRepresentatives * np, spins.size(),
+                                hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    const uint32_t ws = st.end - st.begin;
+    if (squared_displacement) {
+        std::fill(squared_displacement, squared_displacement + e->bins, 0.0);
+        std::copy(sq.begin(), sq.begin() + ws, squared_displacement + st.begin);
+    }
+    if (displacement_samples) {
+        std::fill(displacement_samples, displacement_samples + e->bins, uint64_t(0));
+        std::copy(cnt.begin(), cnt.begin() + ws, displacement_samples + st.begin);
+    }
+    if (round_trips) *round_trips = z.round_trips;
+    for (int r = 0; r < kRepresentatives; ++r) {
+        const bool present = (z.present >> r) & 1;
+        if (representative_energies) representative_energies[r] = present ? z.energy[r] : 0.0;
+        if (representative_present) representative_present[r] = present ? 1 : 0;
+        if (representative_spins && present)
+            std::memcpy(representative_spins + r * n, spins.data() + r * np, n);
+    }
     return WLX_OK;
 }
 

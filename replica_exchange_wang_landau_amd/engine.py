@@ -127,6 +127,8 @@ class WlParameters:
     initialization_temperature_multiplier: float = 2.0
     initialization_max_temperature_fraction: float = 0.5
     nalivaiko_mod: bool = False
+    collect_window_statistics: bool = False
+    round_trip_margin_fraction: float = 0.1
 
     def to_c(self) -> abi.Params:
         p = abi.Params()
@@ -434,6 +436,24 @@ class Engine:
                           ctypes.byref(snap.info), _ptr(arrays[0], ctypes.c_int8),
                           _ptr(arrays[1], ctypes.c_double), _ptr(arrays[2], ctypes.c_double),
                           _ptr(arrays[3], ctypes.c_uint64), _ptr(arrays[4], ctypes.c_uint8))
+
+    def read_sampling(self, k: int) -> dict:
+        """Pilot statistics of walker k (wlx_engine_read_sampling): squared displacements and
+        samples per bin (full grid), round trips, and the nine representatives as
+        (energy, spins) pairs, None where none was recorded."""
+        sq = np.empty(self.bins)
+        samples = np.empty(self.bins, dtype=np.uint64)
+        trips = ctypes.c_uint64()
+        energies = np.empty(9)
+        present = np.empty(9, dtype=np.uint8)
+        spins = np.empty((9, self.n), dtype=np.int8)
+        self.backend.call("engine_read_sampling", self.handle, ctypes.c_uint64(k),
+                          _ptr(sq, ctypes.c_double), _ptr(samples, ctypes.c_uint64),
+                          ctypes.byref(trips), _ptr(energies, ctypes.c_double),
+                          _ptr(present, ctypes.c_uint8), _ptr(spins, ctypes.c_int8))
+        reps = [(float(energies[r]), spins[r].copy()) if present[r] else None for r in range(9)]
+        return {"squared_displacement": sq, "displacement_samples": samples,
+                "round_trips": trips.value, "representatives": reps}
 
     def statistics(self, k: int) -> abi.HistogramStats:
         out = abi.HistogramStats()

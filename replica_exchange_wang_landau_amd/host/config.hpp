@@ -43,6 +43,7 @@ struct WlParameters { // wang_landau.hpp:40-56
     std::uint64_t initialization_stall_attempts_per_spin{1'000};
     double initialization_temperature_multiplier{2.0};
     double initialization_max_temperature_fraction{0.5};
+    bool collect_window_statistics{false};
     double round_trip_margin_fraction{0.1};
     bool nalivaiko_mod{false};
 };
@@ -82,6 +83,9 @@ struct RunConfig {
     std::size_t walkers_per_rank{1};
     double overlap{0.75};
     std::vector<Window> explicit_windows;
+    // starting configurations per walker (window-major; empty = generated), set between
+    // adaptive pilots (io.hpp:28)
+    std::vector<std::vector<std::int8_t>> initial_spins_by_walker;
     AdaptiveWindowParameters adaptive_windows;
     std::uint64_t seed{1};
     bool seed_explicit{false};

@@ -83,6 +83,26 @@ WalkerState download(wlx_engine *e, std::uint64_t k, std::size_t n, std::size_t 
     return w;
 }
 
+// the pilot statistics of walker k (accessors of wang_landau.hpp:118-127)
+WalkerSampling download_sampling(wlx_engine *e, std::uint64_t k, std::size_t n, std::size_t bins)
+{
+    WalkerSampling s;
+    s.squared_energy_displacement.resize(bins);
+    s.displacement_samples.resize(bins);
+    double energies[9];
+    std::uint8_t present[9];
+    std::vector<std::int8_t> spins(9 * n);
+    check(wlx_engine_read_sampling(e, k, s.squared_energy_displacement.data(),
+                                   s.displacement_samples.data(), &s.round_trips, energies,
+                                   present, spins.data()));
+    for (int r = 0; r < 9; ++r)
+        if (present[r])
+            s.representatives.push_back(
+                {energies[r], std::vector<std::int8_t>(spins.begin() + r * n,
+                                                       spins.begin() + (r + 1) * n)});
+    return s;
+}
+
 // WangLandauWalker::restore of a WLCHKP record into walker 0 (wang_landau.cpp:420-466)
 void restore(wlx_engine *e, const WalkerState &s, std::size_t n, std::size_t bins)
 {
@@ -132,12 +152,28 @@ RewlResult run_rewl(std::shared_ptr<const Couplings> couplings, const RunConfig 
 { // parallel.cpp:276-581, shared-memory path on one GPU
     if (!couplings) throw std::invalid_argument("Null couplings");
     c.validate(!c.smoke_test);
-    if (c.adaptive_windows.enabled)
-        throw std::invalid_argument("Adaptive energy windows are not supported by this engine");
     const std::size_t n = couplings->size(), bins = c.grid.bins();
     const auto windows =
         c.explicit_windows.empty() ? partition_windows(bins, c.windows, c.overlap)
                                    : c.explicit_windows;
+    const std::size_t walkers = c.windows * c.walkers_per_rank;
+    std::vector<std::int8_t> initial;
+    std::vector<std::uint8_t> supplied;
+    if (!c.initial_spins_by_walker.empty()) { // parallel.cpp:286-293
+        if (c.initial_spins_by_walker.size() != walkers)
+            throw std::invalid_argument("Adaptive initial-configuration count mismatch");
+        for (const auto &spins : c.initial_spins_by_walker)
+            if (!spins.empty() && spins.size() != n)
+                throw std::invalid_argument("Adaptive initial spin count mismatch");
+        initial.assign(walkers * n, 1);
+        supplied.assign(walkers, 0);
+        for (std::size_t k = 0; k < walkers; ++k)
+            if (!c.initial_spins_by_walker[k].empty()) {
+                std::copy(c.initial_spins_by_walker[k].begin(), c.initial_spins_by_walker[k].end(),
+                          initial.begin() + static_cast<std::ptrdiff_t>(k * n));
+                supplied[k] = 1;
+            }
+    }
     std::vector<std::uint64_t> be;
     for (const auto &w : windows) be.insert(be.end(), {w.begin, w.end});
 
@@ -173,10 +209,13 @@ RewlResult run_rewl(std::shared_ptr<const Couplings> couplings, const RunConfig 
               p.initialization_stall_attempts_per_spin,
               p.initialization_temperature_multiplier,
               p.initialization_max_temperature_fraction,
-              p.nalivaiko_mod ? 1u : 0u};
+              p.nalivaiko_mod ? 1u : 0u,
+              p.collect_window_statistics ? 1u : 0u,
+              p.round_trip_margin_fraction};
     Engine engine(couplings->handle(), cfg);
     wlx_engine *e = engine.get();
-    check(wlx_engine_initialize(e, nullptr, nullptr));
+    check(wlx_engine_initialize(e, initial.empty() ? nullptr : initial.data(),
+                                supplied.empty() ? nullptr : supplied.data()));
 
     const bool checkpointing = !c.checkpoint_path.empty() && c.windows == 1 &&
                                c.walkers_per_rank == 1;
@@ -196,7 +235,6 @@ RewlResult run_rewl(std::shared_ptr<const Couplings> couplings, const RunConfig 
     std::uint64_t last_checkpoint = 0;
     wlx_run_status st{};
     std::uint64_t chunk = 1;
-    const std::size_t walkers = c.windows * c.walkers_per_rank;
     std::vector<wlx_walker_info> infos(walkers);
     for (;;) {
         // one epoch at a time while a checkpoint may fall due, otherwise growing chunks
@@ -227,6 +265,9 @@ RewlResult run_rewl(std::shared_ptr<const Couplings> couplings, const RunConfig 
     std::vector<WalkerState> states;
     states.reserve(walkers);
     for (std::size_t k = 0; k < walkers; ++k) states.push_back(download(e, k, n, bins));
+    const bool collect = c.wl.collect_window_statistics;
+    std::vector<WalkerSampling> sampling(collect ? walkers : 0);
+    for (std::size_t k = 0; k < sampling.size(); ++k) sampling[k] = download_sampling(e, k, n, bins);
     bool converged = true;
     for (std::size_t w = 0; w < c.windows; ++w) { // parallel.cpp:541-579
         std::vector<const WalkerState *> group;
@@ -235,16 +276,28 @@ RewlResult run_rewl(std::shared_ptr<const Couplings> couplings, const RunConfig 
             group.push_back(&s);
             converged &= s.stage == WLX_STAGE_FROZEN;
             const auto h = histogram_statistics(s, c.wl.nalivaiko_mod);
+            const std::size_t k = w * c.walkers_per_rank + l;
             result.walker_statistics.push_back({s.id, w, 0, s.attempted, s.accepted,
                                                 s.forced_accepted,
                                                 s.attempted - s.last_accepted_attempt, s.energy,
                                                 s.factor, h.active_bins, h.minimum, h.mean,
-                                                h.min_over_mean, 0});
+                                                h.min_over_mean,
+                                                collect ? sampling[k].round_trips : 0});
+            if (collect)
+                for (const EnergyRepresentative &r : sampling[k].representatives)
+                    result.representatives.push_back(r);
             result.attempted += s.attempted;
             result.accepted += s.accepted;
             result.forced_accepted += s.forced_accepted;
         }
-        result.fragments.push_back(summarize_walkers(windows[w], group, bins, false));
+        // a pilot window without a common active bin still contributes its statistics
+        result.fragments.push_back(summarize_walkers(windows[w], group, bins, collect));
+        if (collect) {
+            std::vector<const WalkerSampling *> members;
+            for (std::size_t l = 0; l < c.walkers_per_rank; ++l)
+                members.push_back(&sampling[w * c.walkers_per_rank + l]);
+            result.sampling_statistics.push_back(summarize_sampling(windows[w], members, bins));
+        }
     }
     result.converged = converged;
     result.exchange_attempted = st.exchange_attempted;

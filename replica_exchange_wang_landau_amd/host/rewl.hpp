@@ -4,6 +4,7 @@
 // restores/writes the 1x1 checkpoint, prints progress and summarises the downloaded walkers.
 #pragma once
 
+#include "adaptive.hpp"
 #include "geometry.hpp"
 #include "output.hpp"
 
@@ -34,7 +35,9 @@ class Couplings {
 
 struct RewlResult { // parallel.hpp:23-31
     std::vector<DosFragment> fragments;
+    std::vector<WindowSamplingStatistics> sampling_statistics; // collect_window_statistics
     std::vector<WalkerStatistics> walker_statistics;
+    std::vector<EnergyRepresentative> representatives;         // collect_window_statistics
     std::uint64_t attempted{}, accepted{}, forced_accepted{};
     std::uint64_t exchange_attempted{}, exchange_accepted{};
     bool converged{false};

@@ -1,11 +1,16 @@
 // CPU self-test of the host layer (no GPU): configuration parsing and MCS resolution,
 // windows, WLCHKP3 layout, summaries, stitching, thermodynamics and output formats. The
 // expectations are those of the reference's tests (tests/test_main.cpp, cited per case).
+//
+// `host_selftest adapt FILE` instead runs the adaptive-window planner on the pilot data in
+// FILE and prints its result, for comparison with the reference's planner.
+#include "adaptive.hpp"
 #include "geometry.hpp"
 #include "output.hpp"
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <filesystem>
 #include <fstream>
 #include <functional>
@@ -339,8 +344,88 @@ void stitching_and_thermodynamics()
 
 } // namespace
 
-int main()
+// FILE: whitespace-separated numbers (strtod syntax, so nan/inf and hex floats work):
+//   bins count overlap emin emax width per_window n
+//   smoothing_width minimum_width diffusivity_floor_fraction curvature_weight
+//   round_trip_target maximum_round_trip_penalty
+//   per window: begin end round_trips walkers, squared[bins] samples[bins] log_g[bins] valid[bins]
+//   R, then per representative: present energy spins[n]
+// Prints "windows b e ...", "missing M external X" and one line of spins (or "-") per walker,
+// or "error <what>" when the planner throws.
+int adapt_main(const char *path)
 {
+    std::ifstream in(path);
+    std::vector<std::string> tokens{std::istream_iterator<std::string>(in),
+                                    std::istream_iterator<std::string>()};
+    std::size_t at = 0;
+    const auto num = [&] {
+        if (at >= tokens.size()) throw std::runtime_error("truncated input");
+        return std::strtod(tokens[at++].c_str(), nullptr);
+    };
+    const auto count_of = [&] { return static_cast<std::size_t>(num()); };
+    const std::size_t bins = count_of(), count = count_of();
+    const double overlap = num();
+    Grid grid;
+    grid.minimum = num();
+    grid.maximum = num();
+    grid.width = num();
+    const std::size_t per_window = count_of(), n = count_of();
+    AdaptiveWindowParameters p;
+    p.smoothing_width = num();
+    p.minimum_width = num();
+    p.diffusivity_floor_fraction = num();
+    p.curvature_weight = num();
+    p.round_trip_target = num();
+    p.maximum_round_trip_penalty = num();
+    std::vector<DosFragment> fragments;
+    std::vector<WindowSamplingStatistics> sampling;
+    for (std::size_t w = 0; w < count; ++w) {
+        WindowSamplingStatistics s;
+        s.window.begin = count_of();
+        s.window.end = count_of();
+        s.round_trips = static_cast<std::uint64_t>(num());
+        s.walkers = static_cast<std::uint64_t>(num());
+        DosFragment f{s.window, {}, std::vector<std::uint64_t>(bins, 0),
+                      std::vector<double>(bins, 0.0), {}};
+        for (std::size_t i = 0; i < bins; ++i) s.squared_energy_displacement.push_back(num());
+        for (std::size_t i = 0; i < bins; ++i)
+            s.displacement_samples.push_back(static_cast<std::uint64_t>(num()));
+        for (std::size_t i = 0; i < bins; ++i) f.log_g.push_back(num());
+        for (std::size_t i = 0; i < bins; ++i) f.valid.push_back(static_cast<std::uint8_t>(num()));
+        fragments.push_back(std::move(f));
+        sampling.push_back(std::move(s));
+    }
+    std::vector<EnergyRepresentative> reps(count_of());
+    for (auto &r : reps) {
+        const bool present = num() != 0.0;
+        r.energy = num();
+        for (std::size_t i = 0; i < n; ++i) {
+            const auto v = static_cast<std::int8_t>(num());
+            if (present) r.spins.push_back(v);
+        }
+    }
+    try {
+        const auto windows = adapt_energy_windows(grid, fragments, sampling, count, overlap, p);
+        std::cout << "windows";
+        for (const auto &w : windows) std::cout << ' ' << w.begin << ' ' << w.end;
+        std::cout << '\n';
+        const auto starts = select_initial_configurations(grid, windows, reps, per_window);
+        std::cout << "missing " << starts.missing << " external " << starts.external_warm_starts
+                  << '\n';
+        for (const auto &s : starts.spins) {
+            if (s.empty()) std::cout << '-';
+            for (std::size_t i = 0; i < s.size(); ++i) std::cout << (i ? " " : "") << int(s[i]);
+            std::cout << '\n';
+        }
+    } catch (const std::exception &e) {
+        std::cout << "error " << e.what() << '\n';
+    }
+    return 0;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc == 3 && std::string(argv[1]) == "adapt") return adapt_main(argv[2]);
     const std::vector<std::pair<const char *, std::function<void()>>> tests{
         {"csv_geometry_ini", csv_geometry_and_ini},
         {"grid_windows", grid_and_windows},

@@ -4,9 +4,11 @@
 #include "rewl.hpp"
 
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <exception>
 #include <iostream>
+#include <limits>
 #include <random>
 #include <string>
 
@@ -48,6 +50,67 @@ void engine_fields_from_environment(EngineOptions &options)
     const char *v = std::getenv("REWL_FIELDS");
     if (v && std::string(v) == "fp32") options.fields = WLX_FIELDS_FP32_KAHAN;
     if (const char *r = std::getenv("REWL_RESYNC_MCS")) options.resync_interval_mcs = std::atof(r);
+}
+
+// The adaptive pilots (wl_run.cpp:105-174): each pilot runs REWL for pilot_mcs on the current
+// windows with statistics collection and its own seed; the planner then moves the windows and
+// picks the next starting configurations. On return config holds the final windows and starts.
+void plan_adaptive_windows(RunConfig &config, std::shared_ptr<const Couplings> couplings,
+                           std::size_t spins, EngineOptions options)
+{
+    options.fields = WLX_FIELDS_FP64; // the statistics are collected with FP64 fields
+    auto windows = partition_windows(config.grid.bins(), config.windows, config.overlap);
+    std::vector<std::vector<std::int8_t>> starts;
+    const auto t0 = std::chrono::steady_clock::now();
+    const std::size_t iterations = config.adaptive_windows.iterations;
+    for (std::size_t it = 0; it < iterations; ++it) {
+        std::cout << "adaptive_window_pilot=" << (it + 1) << '/' << iterations
+                  << " pilot_mcs=" << config.adaptive_windows.pilot_mcs << '\n';
+        RunConfig pilot = config;
+        pilot.adaptive_windows.enabled = false;
+        pilot.explicit_windows = windows;
+        pilot.initial_spins_by_walker = starts;
+        pilot.wl.collect_window_statistics = true;
+        pilot.max_limit.mcs = config.adaptive_windows.pilot_mcs;
+        pilot.max_limit.uses_mcs = true;
+        pilot.checkpoint_path.clear();
+        pilot.resolved_spin_count = 0;
+        pilot.seed = config.seed ^ ((static_cast<std::uint64_t>(it) + 1) * 0x9e3779b97f4a7c15ULL);
+        pilot.resolve_mcs(spins);
+        const RewlResult r = run_rewl(couplings, pilot, options);
+
+        windows = adapt_energy_windows(config.grid, r.fragments, r.sampling_statistics,
+                                       config.windows, config.overlap, config.adaptive_windows);
+        std::uint64_t round_trips = 0;
+        for (const auto &s : r.sampling_statistics) round_trips += s.round_trips;
+        double lowest = std::numeric_limits<double>::infinity();
+        for (const auto &rep : r.representatives)
+            if (!rep.spins.empty()) lowest = std::min(lowest, rep.energy);
+        InitialConfigurations next = select_initial_configurations(
+            config.grid, windows, r.representatives, config.walkers_per_rank);
+        std::cout << "adaptive_window_round_trips=" << round_trips
+                  << " seeded_walkers=" << next.spins.size() - next.missing << '/'
+                  << next.spins.size() << " external_warm_starts=" << next.external_warm_starts;
+        if (std::isfinite(lowest)) std::cout << " minimum_seed_energy=" << lowest;
+        std::cout << " energy_ranges=";
+        for (std::size_t w = 0; w < windows.size(); ++w) {
+            if (w != 0) std::cout << ';';
+            std::cout << '['
+                      << config.grid.minimum + static_cast<double>(windows[w].begin) * config.grid.width
+                      << ','
+                      << config.grid.minimum + static_cast<double>(windows[w].end) * config.grid.width
+                      << ')';
+        }
+        std::cout << '\n';
+        starts = std::move(next.spins);
+    }
+    config.explicit_windows = std::move(windows);
+    config.initial_spins_by_walker = std::move(starts);
+    config.wl.collect_window_statistics = false;
+    config.validate(!config.smoke_test);
+    std::cout << "adaptive_window_elapsed_seconds="
+              << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count()
+              << '\n';
 }
 
 } // namespace
@@ -110,6 +173,8 @@ int main(int argc, char **argv)
         EngineOptions options;
         options.rng = engine_rng_from_environment();
         engine_fields_from_environment(options);
+        if (config.adaptive_windows.enabled)
+            plan_adaptive_windows(config, couplings, geometry.size(), options);
         const auto start = std::chrono::steady_clock::now();
         const RewlResult result = run_rewl(couplings, config, options);
         const double elapsed =

@@ -156,6 +156,33 @@ class Reference:
         return out
 
 
+    def run_rewl_sampling(self, system: "RefSystem", cfg: RunConfig):
+        """run_rewl with collect_window_statistics: per-window sampling sums, per-walker round
+        trips and the representatives in result order."""
+        c, keep = cfg.to_c()
+        bins, W, n = cfg.bins, cfg.windows * cfg.walkers_per_window, system.n
+        out = dict(squared=np.empty((cfg.windows, bins)),
+                   samples=np.empty((cfg.windows, bins), dtype=np.uint64),
+                   round_trips=np.empty(cfg.windows, dtype=np.uint64),
+                   walkers=np.empty(cfg.windows, dtype=np.uint64),
+                   walker_round_trips=np.empty(W, dtype=np.uint64),
+                   rep_energies=np.empty(9 * W), rep_spins=np.empty((9 * W, n), dtype=np.int8))
+        count = ctypes.c_uint64()
+        st = self.lib.ref_run_rewl_sampling(
+            system.handle, ctypes.byref(c), _ptr(out["squared"], ctypes.c_double),
+            _ptr(out["samples"], ctypes.c_uint64), _ptr(out["round_trips"], ctypes.c_uint64),
+            _ptr(out["walkers"], ctypes.c_uint64),
+            _ptr(out["walker_round_trips"], ctypes.c_uint64),
+            _ptr(out["rep_energies"], ctypes.c_double), _ptr(out["rep_spins"], ctypes.c_int8),
+            ctypes.byref(count))
+        if st:
+            raise RuntimeError(self.error())
+        k = count.value
+        out["representatives"] = [(float(out["rep_energies"][r]), out["rep_spins"][r].copy())
+                                  for r in range(k)]
+        return out
+
+
 class RefSystem:
     def __init__(self, ref: Reference, handle, n):
         self.ref, self.handle, self.n = ref, handle, n
@@ -202,6 +229,26 @@ class RefWalker:
         if st:
             raise RuntimeError(self.system.ref.error())
         return acc, bins
+
+    def sampling(self):
+        """Pilot statistics (collect_window_statistics) in Engine.read_sampling's layout."""
+        n = self.system.n
+        sq = np.empty(self.bins)
+        samples = np.empty(self.bins, dtype=np.uint64)
+        trips = ctypes.c_uint64()
+        energies = np.empty(9)
+        present = np.zeros(9, dtype=np.uint8)
+        spins = np.zeros((9, n), dtype=np.int8)
+        st = self.lib.ref_walker_sampling(self.handle, _ptr(sq, ctypes.c_double),
+                                          _ptr(samples, ctypes.c_uint64), ctypes.byref(trips),
+                                          _ptr(energies, ctypes.c_double),
+                                          _ptr(present, ctypes.c_uint8), _ptr(spins, ctypes.c_int8),
+                                          ctypes.c_uint64(n))
+        if st:
+            raise RuntimeError(self.system.ref.error())
+        reps = [(float(energies[r]), spins[r].copy()) if present[r] else None for r in range(9)]
+        return {"squared_displacement": sq, "displacement_samples": samples,
+                "round_trips": trips.value, "representatives": reps}
 
     def read(self):
         info = abi.WalkerInfo()

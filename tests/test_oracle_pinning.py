@@ -531,3 +531,84 @@ def test_target_metropolis_initialization(orc):
         assert abs(w.energy - target) <= half
         assert int(w.histogram.sum()) == 1 and w.attempted == 0 and w.accepted == 0
         assert_walkers_equal(a, b, k)
+
+
+def assert_sampling_equal(a, b):
+    assert a["squared_displacement"].tobytes() == b["squared_displacement"].tobytes()
+    assert np.array_equal(a["displacement_samples"], b["displacement_samples"])
+    assert a["round_trips"] == b["round_trips"]
+    for x, y in zip(a["representatives"], b["representatives"]):
+        assert (x is None) == (y is None)
+        if x is not None:
+            assert x[0].hex() == y[0].hex() and np.array_equal(x[1], y[1])
+
+
+@pytest.mark.parametrize("window", [(0, 160), (40, 100)])
+def test_window_statistics_match_compiled_reference(orc, ref, window):
+    """collect_window_statistics (wang_landau.cpp:104-118, 272-282, 317-342): squared
+    displacements, samples, round trips and representatives after initialisation and 40000
+    attempts, against the compiled reference walker. A narrow window makes round trips
+    frequent."""
+    pos, axes, box, per = simple_cubic(2, 2, 2, 1.0, (0, 0, 1), False)
+    cfg = c1_config(explicit_windows=[window])
+    cfg.wl.collect_window_statistics = True
+    cfg.wl.round_trip_margin_fraction = 0.15
+    rw = ref.system(pos, axes, box, per).walker(0, cfg, *window)
+    e = Engine(System(pos, axes, box, per, backend=orc), cfg)
+    e.initialize()
+    assert_sampling_equal(e.read_sampling(0), rw.sampling())
+    rw.attempts(40000)
+    e.sample(40000)
+    a, b = e.read_sampling(0), rw.sampling()
+    assert_sampling_equal(a, b)
+    assert sum(r is not None for r in a["representatives"]) >= 5
+    if window == (40, 100):  # C1 reaches both margins of this window, not of the full grid
+        assert a["round_trips"] > 0
+
+
+def test_rewl_window_statistics_match_compiled_reference(orc, ref):
+    """run_rewl with collect_window_statistics on 3x3x3 periodic, 3 windows x 2 walkers,
+    exchanges every 2 MCS: an accepted swap_configuration updates both walkers' round-trip
+    state and representatives (wang_landau.cpp:483-495). Per-window sums of squared
+    displacements and samples (summarize_sampling_impl, parallel.cpp:64-79), round trips per
+    window and walker, and the representatives in result order, against the compiled
+    reference."""
+    pos, axes, box, per = simple_cubic(3, 3, 3, 1.0, (0, 0, 1), True)
+    cfg = RunConfig(emin=-60.0, emax=80.0, bin_width=0.5, windows=3, walkers_per_window=2,
+                    seed=21, exchange_interval_attempts=54, max_attempts=54 * 100,
+                    wl=WlParameters(check_interval_attempts=2700, collect_window_statistics=True,
+                                    round_trip_margin_fraction=0.1))
+    theirs = ref.run_rewl_sampling(ref.system(pos, axes, box, per), cfg)
+    e = Engine(System(pos, axes, box, per, backend=orc), cfg)
+    e.initialize()
+    st = e.run()
+    assert st.exchange_accepted > 0
+    per_window = cfg.walkers_per_window
+    reps = []
+    for w in range(cfg.windows):
+        sq = np.zeros(e.bins)
+        samples = np.zeros(e.bins, dtype=np.uint64)
+        trips = 0
+        for k in range(w * per_window, (w + 1) * per_window):
+            s = e.read_sampling(k)
+            sq += s["squared_displacement"]
+            samples += s["displacement_samples"]
+            trips += s["round_trips"]
+            assert s["round_trips"] == theirs["walker_round_trips"][k]
+            reps += [r for r in s["representatives"] if r is not None]
+        assert sq.tobytes() == theirs["squared"][w].tobytes()
+        assert np.array_equal(samples, theirs["samples"][w])
+        assert trips == theirs["round_trips"][w] and theirs["walkers"][w] == per_window
+    assert sum(int(t) for t in theirs["round_trips"]) > 0
+    assert len(reps) == len(theirs["representatives"])
+    for (ea, sa), (eb, sb) in zip(reps, theirs["representatives"]):
+        assert ea.hex() == eb.hex() and np.array_equal(sa, sb)
+
+
+def test_round_trip_margin_validation(orc):
+    pos, axes, box, per = simple_cubic(2, 2, 2, 1.0, (0, 0, 1), False)
+    for bad in (0.5, -0.1, float("nan")):
+        cfg = c1_config()
+        cfg.wl.round_trip_margin_fraction = bad
+        with pytest.raises(InvalidArgument, match="Invalid WL parameters"):
+            Engine(System(pos, axes, box, per, backend=orc), cfg)

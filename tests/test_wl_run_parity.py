@@ -27,6 +27,7 @@ def run(exe: Path, args, cwd: Path):
 
 
 def normalise_stdout(text: str) -> str:
+    text = re.sub(r"adaptive_window_elapsed_seconds=\S+", "", text)
     return re.sub(r" elapsed_seconds=\S+", "", text)
 
 
@@ -113,3 +114,22 @@ def test_checkpoint_write_and_resume(tmp_path):
     compare(tmp_path, args)
     resumed = [900 if (i > 0 and args[i - 1] == "--max-mcs") else a for i, a in enumerate(args)]
     compare(tmp_path, resumed)
+
+
+@pytest.mark.parametrize("args", [
+    ["--nx", 3, "--ny", 3, "--nz", 3, "--periodic", "true", "--emin", -60, "--emax", 80,
+     "--bin-width", 0.5, "--windows", 3, "--walkers", 2, "--seed", 21,
+     "--adaptive-iterations", 2, "--adaptive-pilot-mcs", 200, "--max-mcs", 2000,
+     "--exchange-interval-mcs", 2],
+    ["--nx", 4, "--ny", 4, "--nz", 4, "--periodic", "true", "--emin", -120, "--emax", 160,
+     "--bin-width", 1.0, "--windows", 4, "--walkers", 3, "--overlap", 0.5, "--seed", 8,
+     "--adaptive-iterations", 3, "--adaptive-pilot-mcs", 300, "--adaptive-round-trip-margin",
+     0.2, "--max-mcs", 1500],
+])
+def test_adaptive_windows_identical(tmp_path, args):
+    """--adaptive-windows: pilots whose statistics are collected by the device walkers, the
+    window planner and warm-start selection on the host. Identical pilot lines (round trips,
+    seeded walkers, warm starts, energy ranges), final windows and outputs."""
+    ra, rb = compare(tmp_path, args + ["--adaptive-windows", "true", "--output", "ad"])
+    assert rb.stdout.count("adaptive_window_round_trips=") == args[args.index(
+        "--adaptive-iterations") + 1]
