@@ -236,7 +236,7 @@ def c4_leg(epochs: int, warmup: int, resyncs: int):
         "resync": {"ms": resync_s * 1e3, "flop": flops, "achieved_tflops": flops / resync_s / 1e12,
                    "peak_tflops": 78.6, "frac": flops / resync_s / 1e12 / 78.6,
                    "note": "three launches (spin matrix, MFMA J.S, finish) timed together"},
-        "drift": {"max_field_abs": f, "max_energy_rel": e, "resyncs": count},
+        "drift": {"max_field_abs": f, "max_energy_bins": e, "resyncs": count},
     }
 
 
@@ -281,9 +281,9 @@ def main():
               "(the walkers would shard by energy window across GPUs)", file=sys.stderr)
         sys.exit(2)
     rng = 0 if args.rng == "xoshiro" else 1
-    # N = 512 sites: 8 register slots per lane; Philox without trace, forcing or the
-    # Nalivaiko mask runs the common-configuration lookahead kernel
-    kernel = "sample_kernel<8,false>" if rng == 0 else "lookahead_fast_kernel<8>"
+    # N = 512 sites: 8 register slots per lane; without trace, forcing or the Nalivaiko mask
+    # each stream runs its common-configuration lookahead kernel
+    kernel = "lookahead_xo_kernel<8>" if rng == 0 else "lookahead_fast_kernel<8>"
     g = gpu_leg(args.steps, args.warmup, rng)
     other = gpu_leg(args.steps, args.warmup, 1 - rng)
     cpu = None if args.no_cpu else cpu_leg(args.cpu_mcs)

@@ -231,8 +231,10 @@ int wlx_engine_set_walker_cadence(wlx_engine *engine, uint64_t walker,
  * Also run by wlx_engine_run every resync_interval_attempts. */
 int wlx_engine_resync(wlx_engine *engine);
 /* Largest drift seen at any resync so far: max |h_fp32 - h_fp64| over all sites and
- * configurations, max |E - E_fp64| / |E_fp64| over configurations, and the resync count. */
-int wlx_engine_drift(wlx_engine *engine, double *max_field_error, double *max_energy_relative,
+ * configurations, max |E - E_fp64| / bin_width over configurations (the energy drift in
+ * bins: what decides whether a walker could be booked into a wrong bin), and the resync
+ * count. */
+int wlx_engine_drift(wlx_engine *engine, double *max_field_error, double *max_energy_bins,
                      uint64_t *resyncs);
 /* The pilot statistics of a walker run with collect_window_statistics: squared accepted
  * energy displacements and attempt counts per bin (full grid, zero outside the window;

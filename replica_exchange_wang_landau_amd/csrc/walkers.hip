@@ -1583,7 +1583,7 @@ __global__ __launch_bounds__(kWave) void lookahead_f32_kernel(Args a, uint64_t r
 // ---------------------------------------------------------------------------------------------
 struct Drift {
     unsigned long long max_field;  // bit pattern of a non-negative double
-    unsigned long long max_energy; // bit pattern of a non-negative double
+    unsigned long long max_energy; // bit pattern of a non-negative double (energy drift in bins)
 };
 
 // S[k][s] = spin of site k in slot s, as +-1.0 (zero for padding slots s >= slots)
@@ -1671,7 +1671,7 @@ __global__ __launch_bounds__(kWave) void resync_finish_kernel(Args a, const doub
     for (int m = 1; m < kWave; m <<= 1) field_err = fmax(field_err, __shfl_xor(field_err, m));
     if (lane == 0) {
         const double e64 = part[0];
-        const double rel = e64 != 0.0 ? fabs(a.energy[s] - e64) / fabs(e64) : fabs(a.energy[s]);
+        const double rel = fabs(a.energy[s] - e64) / a.grid.width; // drift in bins
         atomicMax(&drift->max_field, (unsigned long long)__double_as_longlong(field_err));
         atomicMax(&drift->max_energy, (unsigned long long)__double_as_longlong(rel));
         a.energy[s] = e64;
@@ -2670,7 +2670,7 @@ int wlx_engine_resync(wlx_engine *e)
     return WLX_OK;
 }
 
-int wlx_engine_drift(wlx_engine *e, double *max_field_error, double *max_energy_relative,
+int wlx_engine_drift(wlx_engine *e, double *max_field_error, double *max_energy_bins,
                      uint64_t *resyncs)
 {
     if (int status = require_engine(e, false)) return status;
@@ -2683,7 +2683,7 @@ int wlx_engine_drift(wlx_engine *e, double *max_field_error, double *max_energy_
     std::memcpy(&f, &d.max_field, sizeof f);
     std::memcpy(&en, &d.max_energy, sizeof en);
     if (max_field_error) *max_field_error = f;
-    if (max_energy_relative) *max_energy_relative = en;
+    if (max_energy_bins) *max_energy_bins = en;
     if (resyncs) *resyncs = e->resyncs;
     return WLX_OK;
 }

@@ -339,7 +339,7 @@ class Engine:
         self.backend.call("engine_resync", self.handle)
 
     def drift(self) -> tuple[float, float, int]:
-        """(max field error, max relative energy error, resyncs) over all resyncs so far."""
+        """(max field error, max energy error in bins, resyncs) over all resyncs so far."""
         f, e, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_uint64()
         self.backend.call("engine_drift", self.handle, ctypes.byref(f), ctypes.byref(e),
                           ctypes.byref(n))
