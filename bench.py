@@ -133,9 +133,13 @@ def tta_leg(budget_seconds: float, interval_epochs: int, rng: int = 1):
             continue
         snaps += 1
         if np.any((ref_valid != 0) & (valid == 0)):
+            print(f"tta snapshot {snaps}: {device:.1f} s, reference bins not all found yet",
+                  file=sys.stderr, flush=True)
             continue
         dev = analysis.max_deviation(lg, valid, ref_lg, ref_valid)
         best = min(best, dev)
+        print(f"tta snapshot {snaps}: {device:.1f} s, max|dlog g| {dev:.4f}", file=sys.stderr,
+              flush=True)
         if dev < 0.01:
             reached = device
             break
@@ -285,8 +289,11 @@ def main():
     # each stream runs its common-configuration lookahead kernel
     kernel = "lookahead_xo_kernel<8>" if rng == 0 else "lookahead_fast_kernel<8>"
     g = gpu_leg(args.steps, args.warmup, rng)
+    print(f"bench: {args.rng} leg done", file=sys.stderr, flush=True)
     other = gpu_leg(args.steps, args.warmup, 1 - rng)
+    print("bench: second-stream leg done", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_leg(args.cpu_mcs)
+    print("bench: cpu leg done", file=sys.stderr, flush=True)
     tta = tta_leg(args.tta_seconds, args.tta_interval, rng) if args.tta_seconds > 0 else None
     if tta and cpu and tta["seconds"] is not None:
         # the reference CPU path at its measured rate over the same number of attempts

@@ -27,6 +27,7 @@ struct WalkerState {
     int stage{};
     std::uint64_t active_bins{}, refinement_active_bins{};
     std::uint64_t rng_state[4]{};
+    std::uint64_t last_flatness_check{}; // driver cadence (parallel.cpp:364-365)
     double last_flatness{};
 };
 

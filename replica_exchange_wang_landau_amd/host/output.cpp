@@ -284,48 +284,34 @@ template <class T> void get_vector(std::istream &in, std::vector<T> &v)
     if (!in) throw std::runtime_error("Truncated checkpoint vector");
 }
 
-} // namespace
-
-void save_checkpoint(const std::string &path, const WalkerState &w)
-{ // io.cpp:539-556: written to path.tmp, then renamed over path
-    const std::string temporary = path + ".tmp";
-    {
-        std::ofstream out(temporary, std::ios::binary | std::ios::trunc);
-        if (!out) throw std::runtime_error("Cannot write checkpoint " + temporary);
-        const std::array<char, 8> magic{'W', 'L', 'C', 'H', 'K', 'P', '3', '\0'};
-        out.write(magic.data(), magic.size());
-        put(out, w.id);
-        put_vector(out, w.spins);
-        put_vector(out, w.fields);
-        put(out, w.energy);
-        put_vector(out, w.log_g);
-        put_vector(out, w.histogram);
-        put_vector(out, w.active);
-        put(out, w.factor);
-        put(out, w.attempted);
-        put(out, w.accepted);
-        put(out, w.forced_accepted);
-        put(out, w.last_accepted_attempt);
-        put(out, static_cast<std::uint8_t>(w.stage));
-        for (std::uint64_t word : w.rng_state) put(out, word);
-        out.close();
-        if (!out) throw std::runtime_error("Failed writing checkpoint");
-    }
-    std::error_code error;
-    std::filesystem::remove(path, error);
-    error.clear();
-    std::filesystem::rename(temporary, path, error);
-    if (error) throw std::runtime_error("Cannot atomically replace checkpoint: " + error.message());
+// one WLCHKP3 record (io.cpp:543-551)
+void put_record(std::ostream &out, const WalkerState &w)
+{
+    const std::array<char, 8> magic{'W', 'L', 'C', 'H', 'K', 'P', '3', '\0'};
+    out.write(magic.data(), magic.size());
+    put(out, w.id);
+    put_vector(out, w.spins);
+    put_vector(out, w.fields);
+    put(out, w.energy);
+    put_vector(out, w.log_g);
+    put_vector(out, w.histogram);
+    put_vector(out, w.active);
+    put(out, w.factor);
+    put(out, w.attempted);
+    put(out, w.accepted);
+    put(out, w.forced_accepted);
+    put(out, w.last_accepted_attempt);
+    put(out, static_cast<std::uint8_t>(w.stage));
+    for (std::uint64_t word : w.rng_state) put(out, word);
 }
 
-WalkerState load_checkpoint(const std::string &path)
+// one WLCHKP1/2/3 record (io.cpp:558-572)
+WalkerState get_record(std::istream &in)
 {
-    std::ifstream in(path, std::ios::binary);
-    if (!in) throw std::runtime_error("Cannot open checkpoint " + path);
     std::array<char, 8> magic{};
     in.read(magic.data(), magic.size());
     const std::string_view version(magic.data(), 7);
-    if (version != "WLCHKP1" && version != "WLCHKP2" && version != "WLCHKP3")
+    if (!in || (version != "WLCHKP1" && version != "WLCHKP2" && version != "WLCHKP3"))
         throw std::runtime_error("Invalid checkpoint format");
     WalkerState w;
     get(in, w.id);
@@ -348,6 +334,84 @@ WalkerState load_checkpoint(const std::string &path)
     w.stage = stage;
     for (auto &word : w.rng_state) get(in, word);
     return w;
+}
+
+// writes to path.tmp, then renames it over path (io.cpp:552-555)
+template <class Write> void write_atomically(const std::string &path, Write &&write)
+{
+    const std::string temporary = path + ".tmp";
+    {
+        std::ofstream out(temporary, std::ios::binary | std::ios::trunc);
+        if (!out) throw std::runtime_error("Cannot write checkpoint " + temporary);
+        write(out);
+        out.close();
+        if (!out) throw std::runtime_error("Failed writing checkpoint");
+    }
+    std::error_code error;
+    std::filesystem::remove(path, error);
+    error.clear();
+    std::filesystem::rename(temporary, path, error);
+    if (error) throw std::runtime_error("Cannot atomically replace checkpoint: " + error.message());
+}
+
+} // namespace
+
+void save_checkpoint(const std::string &path, const WalkerState &w)
+{ // io.cpp:539-556
+    write_atomically(path, [&](std::ostream &out) { put_record(out, w); });
+}
+
+WalkerState load_checkpoint(const std::string &path)
+{
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("Cannot open checkpoint " + path);
+    return get_record(in);
+}
+
+void save_run_checkpoint(const std::string &path, const RunCheckpoint &run)
+{
+    write_atomically(path, [&](std::ostream &out) {
+        const std::array<char, 8> magic{'W', 'L', 'C', 'H', 'K', 'M', '1', '\0'};
+        out.write(magic.data(), magic.size());
+        put(out, static_cast<std::uint64_t>(run.walkers.size()));
+        put(out, run.epoch);
+        put(out, run.exchange_attempted);
+        put(out, run.exchange_accepted);
+        for (const WalkerState &w : run.walkers) {
+            put(out, w.last_flatness_check);
+            put(out, w.last_flatness);
+        }
+        for (const WalkerState &w : run.walkers) put_record(out, w);
+    });
+}
+
+RunCheckpoint load_run_checkpoint(const std::string &path)
+{
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("Cannot open checkpoint " + path);
+    std::array<char, 8> magic{};
+    in.read(magic.data(), magic.size());
+    if (!in || std::string_view(magic.data(), 7) != "WLCHKM1")
+        throw std::runtime_error("Invalid multi-walker checkpoint format");
+    RunCheckpoint run;
+    std::uint64_t count{};
+    get(in, count);
+    if (count > (std::uint64_t(1) << 24)) throw std::runtime_error("Unreasonable walker count");
+    get(in, run.epoch);
+    get(in, run.exchange_attempted);
+    get(in, run.exchange_accepted);
+    std::vector<std::uint64_t> checks(count);
+    std::vector<double> flatness(count);
+    for (std::uint64_t k = 0; k < count; ++k) {
+        get(in, checks[k]);
+        get(in, flatness[k]);
+    }
+    for (std::uint64_t k = 0; k < count; ++k) {
+        run.walkers.push_back(get_record(in));
+        run.walkers.back().last_flatness_check = checks[k];
+        run.walkers.back().last_flatness = flatness[k];
+    }
+    return run;
 }
 
 } // namespace rewl::host

@@ -50,4 +50,15 @@ void save_checkpoint(const std::string &path, const WalkerState &walker);
 // Accepts WLCHKP1/2/3 (io.cpp:558-572).
 WalkerState load_checkpoint(const std::string &path);
 
+// Checkpoint of a whole run with any window x walker layout (the reference checkpoints 1x1 runs
+// only): "WLCHKM1\0", u64 walker count, u64 epoch, u64 exchange attempts, u64 accepted
+// exchanges, then per walker u64 last flatness check and f64 last flatness (the driver's
+// cadence, parallel.cpp:364-365), then one WLCHKP3 record per walker in walker order.
+struct RunCheckpoint {
+    std::uint64_t epoch{}, exchange_attempted{}, exchange_accepted{};
+    std::vector<WalkerState> walkers; // last_flatness_check / last_flatness filled
+};
+void save_run_checkpoint(const std::string &path, const RunCheckpoint &run);
+RunCheckpoint load_run_checkpoint(const std::string &path);
+
 } // namespace rewl::host

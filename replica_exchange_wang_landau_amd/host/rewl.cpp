@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <filesystem>
 #include <iostream>
 #include <sstream>
 #include <stdexcept>
@@ -79,6 +80,7 @@ WalkerState download(wlx_engine *e, std::uint64_t k, std::size_t n, std::size_t 
     w.active_bins = info.active_bins;
     w.refinement_active_bins = info.refinement_active_bins;
     std::memcpy(w.rng_state, info.rng_state, sizeof w.rng_state);
+    w.last_flatness_check = info.last_flatness_check;
     w.last_flatness = info.last_flatness;
     return w;
 }
@@ -103,10 +105,11 @@ WalkerSampling download_sampling(wlx_engine *e, std::uint64_t k, std::size_t n, 
     return s;
 }
 
-// WangLandauWalker::restore of a WLCHKP record into walker 0 (wang_landau.cpp:420-466)
-void restore(wlx_engine *e, const WalkerState &s, std::size_t n, std::size_t bins)
+// WangLandauWalker::restore of a WLCHKP record into walker k (wang_landau.cpp:420-466)
+void restore(wlx_engine *e, const WalkerState &s, std::size_t n, std::size_t bins,
+             std::uint64_t k = 0)
 {
-    if (s.id != 0 || s.spins.size() != n || s.fields.size() != n || s.log_g.size() != bins ||
+    if (s.id != k || s.spins.size() != n || s.fields.size() != n || s.log_g.size() != bins ||
         s.histogram.size() != bins || s.active.size() != bins)
         throw std::invalid_argument("Checkpoint is incompatible with walker");
     wlx_walker_info info{};
@@ -119,8 +122,22 @@ void restore(wlx_engine *e, const WalkerState &s, std::size_t n, std::size_t bin
     info.last_accepted_attempt = s.last_accepted_attempt;
     info.stage = std::uint64_t(s.stage);
     std::memcpy(info.rng_state, s.rng_state, sizeof info.rng_state);
-    check(wlx_engine_write_walker(e, 0, &info, s.spins.data(), s.fields.data(), s.log_g.data(),
+    check(wlx_engine_write_walker(e, k, &info, s.spins.data(), s.fields.data(), s.log_g.data(),
                                   s.histogram.data(), s.active.data()));
+}
+
+// every walker, the driver's cadence and the run counters from a WLCHKM1 file
+void restore_run(wlx_engine *e, const RunCheckpoint &run, std::size_t walkers, std::size_t n,
+                 std::size_t bins)
+{
+    if (run.walkers.size() != walkers)
+        throw std::invalid_argument("Checkpoint holds a different number of walkers");
+    for (std::size_t k = 0; k < walkers; ++k) {
+        restore(e, run.walkers[k], n, bins, k);
+        check(wlx_engine_set_walker_cadence(e, k, run.walkers[k].last_flatness_check,
+                                            run.walkers[k].last_flatness));
+    }
+    check(wlx_engine_set_run_state(e, run.epoch, run.exchange_attempted, run.exchange_accepted));
 }
 
 std::string progress_line(const RunConfig &c, const std::vector<wlx_walker_info> &infos,
@@ -217,9 +234,11 @@ RewlResult run_rewl(std::shared_ptr<const Couplings> couplings, const RunConfig 
     check(wlx_engine_initialize(e, initial.empty() ? nullptr : initial.data(),
                                 supplied.empty() ? nullptr : supplied.data()));
 
-    const bool checkpointing = !c.checkpoint_path.empty() && c.windows == 1 &&
-                               c.walkers_per_rank == 1;
-    if (checkpointing) { // parallel.cpp:354-360
+    // 1x1 runs keep the reference's WLCHKP3 file (parallel.cpp:354-360, 522-526); other
+    // layouts write a WLCHKM1 file of every walker at the same cadence (walker 0's attempts)
+    const bool checkpointing = !c.checkpoint_path.empty() && walkers == 1;
+    const bool run_checkpointing = !c.checkpoint_path.empty() && walkers > 1;
+    if (checkpointing) {
         try {
             restore(e, load_checkpoint(c.checkpoint_path), n, bins);
         } catch (const std::exception &error) {
@@ -227,18 +246,40 @@ RewlResult run_rewl(std::shared_ptr<const Couplings> couplings, const RunConfig 
                       << "; starting a new run\n";
         }
     }
+    std::uint64_t last_checkpoint = 0;
+    if (run_checkpointing && std::filesystem::exists(c.checkpoint_path)) {
+        try {
+            const RunCheckpoint run = load_run_checkpoint(c.checkpoint_path);
+            restore_run(e, run, walkers, n, bins);
+            last_checkpoint = run.walkers[0].attempted;
+        } catch (const std::exception &error) {
+            std::cerr << "warning: checkpoint was not restored: " << error.what()
+                      << "; starting a new run\n";
+            check(wlx_engine_initialize(e, initial.empty() ? nullptr : initial.data(),
+                                        supplied.empty() ? nullptr : supplied.data()));
+        }
+    }
 
     const auto period = std::chrono::duration_cast<std::chrono::steady_clock::duration>(
         std::chrono::duration<double>(c.progress_interval_seconds));
     auto next_progress = std::chrono::steady_clock::now() + period;
     std::size_t progress_width = 0;
-    std::uint64_t last_checkpoint = 0;
     wlx_run_status st{};
     std::uint64_t chunk = 1;
     std::vector<wlx_walker_info> infos(walkers);
+    if (run_checkpointing) check(wlx_engine_read_infos(e, infos.data()));
     for (;;) {
-        // one epoch at a time while a checkpoint may fall due, otherwise growing chunks
-        check(wlx_engine_run(e, checkpointing ? 1 : chunk, &st));
+        // one epoch at a time while a 1x1 checkpoint may fall due, otherwise growing chunks;
+        // a run checkpoint falls due after a known number of epochs (walker 0 gains one
+        // exchange interval of attempts per epoch until its limit)
+        std::uint64_t epochs = checkpointing ? 1 : chunk;
+        if (run_checkpointing) {
+            const std::uint64_t since = infos[0].attempted - last_checkpoint;
+            const std::uint64_t due = since >= c.checkpoint.attempts ? 0 : c.checkpoint.attempts - since;
+            epochs = std::min(epochs, std::max<std::uint64_t>(
+                                          1, (due + c.exchange.attempts - 1) / c.exchange.attempts));
+        }
+        check(wlx_engine_run(e, epochs, &st));
         chunk = std::min<std::uint64_t>(chunk * 2, 4096);
         if (c.progress_interval_seconds > 0.0 && std::chrono::steady_clock::now() >= next_progress) {
             check(wlx_engine_read_infos(e, infos.data()));
@@ -254,6 +295,15 @@ RewlResult run_rewl(std::shared_ptr<const Couplings> couplings, const RunConfig 
             check(wlx_engine_read_infos(e, infos.data()));
             if (infos[0].attempted - last_checkpoint >= c.checkpoint.attempts) {
                 save_checkpoint(c.checkpoint_path, download(e, 0, n, bins));
+                last_checkpoint = infos[0].attempted;
+            }
+        }
+        if (run_checkpointing) {
+            check(wlx_engine_read_infos(e, infos.data()));
+            if (infos[0].attempted - last_checkpoint >= c.checkpoint.attempts) {
+                RunCheckpoint run{st.epoch, st.exchange_attempted, st.exchange_accepted, {}};
+                for (std::size_t k = 0; k < walkers; ++k) run.walkers.push_back(download(e, k, n, bins));
+                save_run_checkpoint(c.checkpoint_path, run);
                 last_checkpoint = infos[0].attempted;
             }
         }

@@ -228,6 +228,43 @@ void checkpoint_layout()
     fs::remove(path);
 }
 
+void run_checkpoint_layout()
+{ // WLCHKM1: 40-byte header, 16 bytes of cadence per walker, then WLCHKP3 records
+    RunCheckpoint run{12, 30, 7, {}};
+    for (std::uint64_t k = 0; k < 3; ++k) {
+        auto w = synthetic_walker(8, 160);
+        w.id = k;
+        w.attempted = 100 + k;
+        w.last_flatness_check = 50 + k;
+        w.last_flatness = 0.25 * double(k);
+        w.rng_state[2] = 0x1234 + k;
+        run.walkers.push_back(w);
+    }
+    const auto path = (fs::temp_directory_path() / "rewl_selftest_run.chk").string();
+    save_run_checkpoint(path, run);
+    require(fs::file_size(path) == 40 + 3 * 16 + 3 * 2929, "WLCHKM1 size");
+    const auto r = load_run_checkpoint(path);
+    require(r.epoch == 12 && r.exchange_attempted == 30 && r.exchange_accepted == 7 &&
+                r.walkers.size() == 3,
+            "WLCHKM1 header");
+    for (std::uint64_t k = 0; k < 3; ++k)
+        require(r.walkers[k].id == k && r.walkers[k].attempted == 100 + k &&
+                    r.walkers[k].last_flatness_check == 50 + k &&
+                    r.walkers[k].last_flatness == 0.25 * double(k) &&
+                    r.walkers[k].rng_state[2] == 0x1234 + k &&
+                    r.walkers[k].spins == run.walkers[k].spins,
+                "WLCHKM1 walker round trip");
+    bool rejected = false; // a single-walker file is not a run checkpoint
+    save_checkpoint(path, run.walkers[0]);
+    try {
+        (void)load_run_checkpoint(path);
+    } catch (const std::runtime_error &e) {
+        rejected = std::string(e.what()) == "Invalid multi-walker checkpoint format";
+    }
+    require(rejected, "WLCHKM1 magic");
+    fs::remove(path);
+}
+
 void summary()
 { // test_main.cpp:730-752
     auto a = synthetic_walker(1, 4), b = synthetic_walker(1, 4);
@@ -430,6 +467,7 @@ int main(int argc, char **argv)
         {"csv_geometry_ini", csv_geometry_and_ini},
         {"grid_windows", grid_and_windows},
         {"checkpoint_layout", checkpoint_layout},
+        {"run_checkpoint_layout", run_checkpoint_layout},
         {"summary", summary},
         {"statistics_and_stat_row", statistics_and_stat_row},
         {"stitching_and_thermodynamics", stitching_and_thermodynamics}};
