@@ -1622,17 +1622,29 @@ __global__ __launch_bounds__(kWave) void resync_mfma_kernel(const double *J, con
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) acc[x][y] = mfma_d4{0.0, 0.0, 0.0, 0.0};
-    for (uint32_t k0 = 0; k0 < n; k0 += 4) {
-        const uint32_t k = k0 + kk; // < np: n <= np and np is a multiple of 64
-        const bool real = k < n;
-        const double *jrow = J + size_t(real ? k : 0) * np + i0 + row;
-        const double *srow = S + size_t(k) * slots_padded + s0 + row;
-        const double a0 = real ? jrow[0] : 0.0, a1 = real ? jrow[16] : 0.0;
-        const double b0 = srow[0], b1 = srow[16];
-        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    // kResyncSteps K steps per round: all their loads are issued before their MFMAs, so one
+    // wave per SIMD keeps that many loads in flight
+    constexpr int kResyncSteps = 8;
+    for (uint32_t k0 = 0; k0 < n; k0 += 4 * kResyncSteps) {
+        double a0[kResyncSteps], a1[kResyncSteps], b0[kResyncSteps], b1[kResyncSteps];
+#pragma unroll
+        for (int u = 0; u < kResyncSteps; ++u) {
+            const uint32_t k = k0 + 4 * uint32_t(u) + kk;
+            const bool real = k < n;
+            const double *jrow = J + size_t(real ? k : 0) * np + i0 + row;
+            const double *srow = S + size_t(real ? k : 0) * slots_padded + s0 + row;
+            a0[u] = real ? jrow[0] : 0.0;
+            a1[u] = real ? jrow[16] : 0.0;
+            b0[u] = real ? srow[0] : 0.0;
+            b1[u] = real ? srow[16] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kResyncSteps; ++u) {
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b0[u], acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b1[u], acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b0[u], acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b1[u], acc[1][1], 0, 0, 0);
+        }
     }
 #pragma unroll
     for (int x = 0; x < 2; ++x)

@@ -153,7 +153,7 @@ def test_multi_walker_checkpoint_resume_identical(tmp_path):
     assert (a / "run.chk").read_bytes()[:8] == b"WLCHKM1\0"
     resumed = run(OURS, base + ["--max-mcs", 1000, "--checkpoint", "run.chk",
                                 "--checkpoint-interval-mcs", 100], a)
-    straight = run(OURS, base + ["--max-mcs", 1000], b)
+    straight = run(OURS, base + ["--max-mcs", 1000, "--checkpoint-interval-mcs", 100], b)
     assert "warning: checkpoint was not restored" not in resumed.stderr
     assert normalise_stdout(resumed.stdout) == normalise_stdout(straight.stdout)
     for name in ("run_dos.csv", "run_window_0.csv", "run_window_1.csv", "run_thermo.csv",
