@@ -45,6 +45,17 @@ enum wlx_rng { WLX_RNG_XOSHIRO = 0, WLX_RNG_PHILOX = 1 };
 /* local-field precision: FP64 (reference arithmetic), or FP32 fields with Kahan-compensated
  * incremental updates over FP32 couplings, resynchronised periodically from an FP64 H = J.S */
 enum wlx_fields { WLX_FIELDS_FP64 = 0, WLX_FIELDS_FP32_KAHAN = 1 };
+/* Walker kernel families (all give the same trajectory):
+ * AUTO       lookahead for runs without statistics collection;
+ * LOOKAHEAD  batches of proposals resolved together, fields in registers (N <= 4096);
+ * SEQUENTIAL one attempt at a time, fields in registers;
+ * CSR        one attempt at a time, fields and spins in LDS, CSR row scatter (CSR systems). */
+enum wlx_walker_kernel {
+    WLX_KERNEL_AUTO = 0,
+    WLX_KERNEL_LOOKAHEAD = 1,
+    WLX_KERNEL_SEQUENTIAL = 2,
+    WLX_KERNEL_CSR = 3
+};
 
 /* trace flags, one byte per recorded attempt */
 enum wlx_trace_flag {
@@ -246,6 +257,14 @@ int wlx_engine_read_sampling(wlx_engine *engine, uint64_t walker, double *square
                              uint64_t *displacement_samples, uint64_t *round_trips,
                              double *representative_energies, uint8_t *representative_present,
                              int8_t *representative_spins);
+/* Kernel selection. No reference counterpart: the reference has one CPU walker loop
+ * (wang_landau.cpp:244-288). set_kernel forces a family (enum wlx_walker_kernel); kernel
+ * reports the family and kernel name the next sampling launch uses; autotune runs `epochs`
+ * epochs of the live run with each eligible family (the trajectory does not depend on it),
+ * keeps the fastest and writes the seconds per family to seconds[4] (-1 = not tried). */
+int wlx_engine_set_kernel(wlx_engine *engine, uint64_t kernel);
+int wlx_engine_kernel(const wlx_engine *engine, uint64_t *kernel, char *name, size_t capacity);
+int wlx_engine_autotune(wlx_engine *engine, uint64_t epochs, uint64_t *chosen, double *seconds);
 /* WangLandauWalker::histogram_statistics (wang_landau.cpp:344-367), computed on the device. */
 int wlx_engine_histogram_statistics(wlx_engine *engine, uint64_t walker,
                                     wlx_histogram_stats *out);

@@ -28,6 +28,11 @@ RNG_PHILOX = 1
 FIELDS_FP64 = 0
 FIELDS_FP32_KAHAN = 1
 
+KERNEL_AUTO = 0
+KERNEL_LOOKAHEAD = 1
+KERNEL_SEQUENTIAL = 2
+KERNEL_CSR = 3
+
 TRACE_ACCEPTED = 1
 TRACE_FORCED = 2
 TRACE_UNIFORM = 4
@@ -153,6 +158,9 @@ EXPORTS = [
     "wlx_engine_resync",
     "wlx_engine_drift",
     "wlx_engine_read_sampling",
+    "wlx_engine_set_kernel",
+    "wlx_engine_kernel",
+    "wlx_engine_autotune",
     "wlx_engine_histogram_statistics",
     "wlx_engine_ready",
     "wlx_engine_next_iteration",

@@ -19,6 +19,7 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -75,6 +76,10 @@ struct Args {
     int8_t *rep_spins;          // walkers x 9 x np
     uint32_t collect, collect_offset;
     double rt_margin;
+    // CSR couplings of a CSR system (physics.cpp:197-223): row offsets, columns, values
+    const uint64_t *csr_offsets;
+    const uint32_t *csr_columns;
+    const double *csr_values;
     const float *J32;   // FP32 couplings (FP32-field engines)
     float2 *fields32;   // FP32 fields as (h, c) pairs, value h - c (FP32-field engines)
     uint32_t n, np;
@@ -649,6 +654,142 @@ __global__ __launch_bounds__(kWave) void sample_kernel(Args a, uint64_t request,
         store_sampling(a, w, ws, col);
     }
     finish_walker<NPL>(a, w, st, v, ws, neg, h, energy, flags);
+}
+
+// ---------------------------------------------------------------------------------------------
+// CSR walker (WLX_KERNEL_CSR_LDS): the same attempt sequence as sample_kernel for a CSR system,
+// with the configuration's fields and spins staged in LDS instead of registers, so one kernel
+// serves every N (no per-size register layout) and a batch of uneven systems shares a launch.
+// An accepted flip scatters its CSR row, h[col] += m val (physics.cpp:231-236), one column per
+// lane: the columns of a row are distinct, so the lanes never collide, and fma(m, val, h) is
+// the reference's rounding because m val is exact. LDS: bins | fields (np doubles) | spins
+// (np bytes).
+// ---------------------------------------------------------------------------------------------
+__host__ __device__ inline size_t csr_lds_bytes(uint32_t ws_stride, uint32_t np)
+{
+    return bins_lds_bytes(ws_stride) + size_t(np) * 9;
+}
+
+template <bool PHILOX>
+__device__ inline void csr_walk(const Args &a, uint32_t w, uint64_t request, uint32_t flags,
+                                unsigned char *smem)
+{
+    const uint32_t lane = lane_id();
+    if ((flags & kGate) && a.glob->done) return;
+    WalkerState st = a.state[w];
+    if (st.escaped) return;
+    const uint32_t ws = st.end - st.begin;
+    Bins v = bins_view(smem, a.ws_stride);
+    double *hf = reinterpret_cast<double *>(smem + bins_lds_bytes(a.ws_stride));
+    int8_t *sp = reinterpret_cast<int8_t *>(hf + a.np);
+    load_bins(a, w, ws, v);
+    const size_t base = size_t(st.slot) * a.np;
+    for (uint32_t i = lane; i < a.n; i += kWave) {
+        hf[i] = a.fields[base + i];
+        sp[i] = a.spins[base + i];
+    }
+    double energy = a.energy[st.slot];
+    __syncthreads();
+
+    const uint64_t count = batch_count(a, st, request, flags);
+    Xoshiro rng{st.rng[0], st.rng[1], st.rng[2], st.rng[3]};
+    const uint64_t trace_base = uint64_t(w) * a.trace_cap;
+    int64_t cur = grid_index(a.grid, energy);
+    Philox4 block{};
+    for (uint64_t t = 0; t < count; ++t) {
+        uint64_t site;
+        if (PHILOX) {
+            if (t == 0 || (st.attempted & 1) == 0)
+                block = philox4x64(st.attempted >> 1, kPhiloxWalkTag, 0, 0, a.seed,
+                                   uint64_t(w));
+            site = __umul64hi((st.attempted & 1) ? block.v[2] : block.v[0], uint64_t(a.n));
+        } else {
+            site = draw_bounded(rng, a.bounded);
+        }
+        if (cur < int64_t(st.begin) || cur >= int64_t(st.end)) { // wang_landau.cpp:247-248
+            st.escaped = 1;
+            break;
+        }
+        const uint32_t s = uint32_t(site);
+        const double hs = hf[s];
+        const double sigma = sp[s] < 0 ? -1.0 : 1.0;
+        const double delta = -2.0 * sigma * hs; // flip_delta, physics.cpp:261-266
+        const double proposed = energy + delta;
+        const int64_t next = grid_index(a.grid, proposed);
+        bool accepted = false, forced = false, drew = false, inside = false;
+        if (next >= int64_t(st.begin) && next < int64_t(st.end)) {
+            inside = true;
+            const double log_ratio = v.lg[cur - st.begin] - v.lg[next - st.begin];
+            const bool force_due =
+                a.force_after != 0 && st.attempted - st.last_accepted >= a.force_after;
+            if (log_ratio >= 0.0)
+                accepted = true;
+            else if (force_due)
+                accepted = forced = true;
+            else {
+                const double u =
+                    PHILOX ? double(((st.attempted & 1) ? block.v[3] : block.v[1]) >> 11) *
+                                 0x1.0p-53
+                           : rng.uniform();
+                drew = true;
+                const double lu = log(fmax(u, 0x1.0p-53));
+                accepted = lu < log_ratio;
+                if (near_threshold(lu, log_ratio)) ++st.ambiguous;
+            }
+        }
+        ++st.attempted;
+        if (accepted) {
+            const double m = -2.0 * sigma;
+            const uint64_t row_end = a.csr_offsets[s + 1];
+            for (uint64_t k = a.csr_offsets[s] + lane; k < row_end; k += kWave) {
+                const uint32_t c = a.csr_columns[k];
+                hf[c] = fma(m, a.csr_values[k], hf[c]);
+            }
+            if (lane == 0) sp[s] = sigma < 0.0 ? int8_t(1) : int8_t(-1);
+            __syncthreads(); // the scattered fields before the next attempt reads one
+            energy = proposed;
+            ++st.accepted;
+            if (forced) ++st.forced;
+            st.last_accepted = st.attempted;
+            cur = next;
+        }
+        if (st.stage == WLX_STAGE_INVERSE_TIME) { // update_inverse_time_factor, 309-315
+            if (st.active_count != 0 && st.attempted != 0)
+                st.factor = double(st.active_count) / double(st.attempted);
+            if (st.factor <= a.final_factor) st.stage = WLX_STAGE_FROZEN;
+        }
+        update_current_bin(a, st, v, uint32_t(cur - st.begin));
+        if ((flags & kTrace) && st.trace_count < a.trace_cap) {
+            if (lane == 0) {
+                const uint64_t k = trace_base + st.trace_count;
+                a.trace_site[k] = s;
+                a.trace_bin[k] = uint32_t(cur);
+                a.trace_flag[k] = uint8_t((accepted ? WLX_TRACE_ACCEPTED : 0) |
+                                          (forced ? WLX_TRACE_FORCED : 0) |
+                                          (drew ? WLX_TRACE_UNIFORM : 0) |
+                                          (inside ? WLX_TRACE_IN_WINDOW : 0));
+            }
+            ++st.trace_count;
+        }
+    }
+    st.rng[0] = rng.s0;
+    st.rng[1] = rng.s1;
+    st.rng[2] = rng.s2;
+    st.rng[3] = rng.s3;
+    __syncthreads();
+    for (uint32_t i = lane; i < a.n; i += kWave) {
+        a.fields[base + i] = hf[i];
+        a.spins[base + i] = sp[i];
+    }
+    finish_walker_state(a, w, st, v, ws, energy, flags);
+}
+
+template <bool PHILOX>
+__global__ __launch_bounds__(kWave) void csr_sample_kernel(Args a, uint64_t request,
+                                                           uint32_t flags)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    csr_walk<PHILOX>(a, blockIdx.x, request, flags, smem);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2041,6 +2182,7 @@ using PilotFn = void (*)(Args, uint64_t, uint64_t, double *);
 
 struct KernelSet {
     SampleFn sample_xo, sample_xo_fast, sample_ph, sample_ph_fast, sample_f32, sample_ph_seq;
+    SampleFn csr_xo, csr_ph; // any N
     InitFn init;
     FieldsFn fields;
     PilotFn pilot;
@@ -2051,6 +2193,7 @@ template <int NPL> KernelSet kernel_set()
     return {sample_kernel<NPL, false>, lookahead_xo_kernel<NPL>, lookahead_kernel<NPL>,
             lookahead_fast_kernel<NPL>,
             lookahead_f32_kernel<NPL>, sample_kernel<NPL, true>,
+            csr_sample_kernel<false>, csr_sample_kernel<true>,
             init_kernel<NPL>, fields_kernel<NPL>, pilot_kernel<NPL>};
 }
 
@@ -2149,7 +2292,8 @@ struct wlx_engine {
     uint64_t *d_op = nullptr;
     hipStream_t stream = nullptr;
     bool initialized = false;
-    size_t lds_sample = 0, lds_init = 0;
+    size_t lds_sample = 0, lds_init = 0, lds_csr = 0;
+    uint64_t kernel_choice = WLX_KERNEL_AUTO;
     bool timing = false;
     std::vector<hipEvent_t> events;
     double sample_ms = 0.0;
@@ -2210,6 +2354,9 @@ struct wlx_engine {
         a.nalivaiko = cfg.wl.nalivaiko_mod ? 1 : 0;
         a.philox = cfg.rng == WLX_RNG_PHILOX ? 1 : 0;
         a.unlimited = cfg.max_attempts == 0 ? 1 : 0;
+        a.csr_offsets = sys->d_offsets;
+        a.csr_columns = sys->d_columns;
+        a.csr_values = sys->d_values;
         a.sq_disp = d_sq;
         a.disp_samples = d_cnt;
         a.sampling = d_sampling;
@@ -2224,8 +2371,12 @@ struct wlx_engine {
     SampleFn sample_fn() const
     {
         if (f32) return kernels.sample_f32;
+        const bool philox = cfg.rng == WLX_RNG_PHILOX;
         // the statistics are recorded by the one-flip-at-a-time kernels
-        if (collect) return cfg.rng == WLX_RNG_PHILOX ? kernels.sample_ph_seq : kernels.sample_xo;
+        if (collect) return philox ? kernels.sample_ph_seq : kernels.sample_xo;
+        if (kernel_choice == WLX_KERNEL_CSR) return philox ? kernels.csr_ph : kernels.csr_xo;
+        if (kernel_choice == WLX_KERNEL_SEQUENTIAL)
+            return philox ? kernels.sample_ph_seq : kernels.sample_xo;
         const bool common =
             trace_cap == 0 && cfg.wl.force_accept_after_attempts == 0 && !cfg.wl.nalivaiko_mod;
         if (cfg.rng != WLX_RNG_PHILOX) return common ? kernels.sample_xo_fast : kernels.sample_xo;
@@ -2237,7 +2388,9 @@ struct wlx_engine {
     int launch_sample(uint64_t request, uint32_t flags)
     {
         if (trace_cap) flags |= kTrace;
-        hipLaunchKernelGGL(sample_fn(), dim3(walkers), dim3(kWave), lds_sample, stream, args(),
+        const SampleFn fn = sample_fn();
+        const bool csr_fn = fn == kernels.csr_xo || fn == kernels.csr_ph;
+        hipLaunchKernelGGL(fn, dim3(walkers), dim3(kWave), csr_fn ? lds_csr : lds_sample, stream, args(),
                            request, flags);
         REWL_HIP(hipGetLastError());
         return WLX_OK;
@@ -2455,6 +2608,13 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
             return status;
     if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.init), e->lds_init))
         return status;
+    if (sys->csr) { // the CSR walker stages the configuration in LDS beside the bins
+        e->lds_csr = csr_lds_bytes(e->ws_stride, uint32_t(sys->np));
+        if (e->lds_csr <= 160 * 1024)
+            for (SampleFn fn : {e->kernels.csr_xo, e->kernels.csr_ph})
+                if (int status = set_lds_limit(reinterpret_cast<const void *>(fn), e->lds_csr))
+                    return status;
+    }
     if (int status = set_lds_limit(reinterpret_cast<const void *>(walker_op_kernel),
                                    e->lds_sample))
         return status;
@@ -2679,6 +2839,83 @@ int wlx_engine_resync(wlx_engine *e)
     if (!e->f32) return set_error(WLX_LOGIC_ERROR, "Resync applies to FP32 fields only");
     if (int status = e->launch_resync()) return status;
     REWL_HIP(hipStreamSynchronize(e->stream));
+    return WLX_OK;
+}
+
+int wlx_engine_set_kernel(wlx_engine *e, uint64_t kernel)
+{
+    if (int status = require_engine(e, false)) return status;
+    if (kernel > WLX_KERNEL_CSR) return set_error(WLX_INVALID_ARGUMENT, "Unknown walker kernel");
+    if (kernel == WLX_KERNEL_CSR && !e->sys->csr)
+        return set_error(WLX_INVALID_ARGUMENT, "The CSR walker kernel needs a CSR system");
+    if (kernel == WLX_KERNEL_CSR && e->lds_csr > 160 * 1024)
+        return set_error(WLX_INVALID_ARGUMENT, "Fields and window bins exceed the LDS of a CU");
+    e->kernel_choice = kernel;
+    return WLX_OK;
+}
+
+int wlx_engine_kernel(const wlx_engine *e, uint64_t *kernel, char *name, size_t capacity)
+{
+    if (!e) return set_error(WLX_INVALID_ARGUMENT, "Null engine");
+    const SampleFn fn = e->sample_fn();
+    const KernelSet &k = e->kernels;
+    uint64_t family = WLX_KERNEL_LOOKAHEAD;
+    const char *base = "lookahead_kernel";
+    if (fn == k.sample_xo || fn == k.sample_ph_seq) {
+        family = WLX_KERNEL_SEQUENTIAL;
+        base = "sample_kernel";
+    } else if (fn == k.csr_xo || fn == k.csr_ph) {
+        family = WLX_KERNEL_CSR;
+        base = "csr_sample_kernel";
+    } else if (fn == k.sample_xo_fast) {
+        base = "lookahead_xo_kernel";
+    } else if (fn == k.sample_ph_fast) {
+        base = "lookahead_fast_kernel";
+    } else if (fn == k.sample_f32) {
+        base = "lookahead_f32_kernel";
+    }
+    if (kernel) *kernel = family;
+    if (name && capacity) {
+        if (family == WLX_KERNEL_CSR)
+            std::snprintf(name, capacity, "%s<%s>", base,
+                          e->cfg.rng == WLX_RNG_PHILOX ? "true" : "false");
+        else if (family == WLX_KERNEL_SEQUENTIAL)
+            std::snprintf(name, capacity, "%s<%u,%s>", base, unsigned(e->sys->npl),
+                          e->cfg.rng == WLX_RNG_PHILOX ? "true" : "false");
+        else
+            std::snprintf(name, capacity, "%s<%u>", base, unsigned(e->sys->npl));
+    }
+    return WLX_OK;
+}
+
+int wlx_engine_autotune(wlx_engine *e, uint64_t epochs, uint64_t *chosen, double *seconds)
+{
+    if (int status = require_engine(e)) return status;
+    if (epochs == 0) return set_error(WLX_INVALID_ARGUMENT, "Autotuning needs at least one epoch");
+    if (seconds)
+        for (int k = 0; k <= int(WLX_KERNEL_CSR); ++k) seconds[k] = -1.0;
+    std::vector<uint64_t> candidates{WLX_KERNEL_LOOKAHEAD, WLX_KERNEL_SEQUENTIAL};
+    if (e->sys->csr && e->lds_csr <= 160 * 1024) candidates.push_back(WLX_KERNEL_CSR);
+    uint64_t best = e->kernel_choice == WLX_KERNEL_AUTO ? WLX_KERNEL_LOOKAHEAD : e->kernel_choice;
+    double best_s = -1.0;
+    // every variant yields the same trajectory, so the trials are part of the run itself
+    for (uint64_t variant : candidates) {
+        e->kernel_choice = variant;
+        wlx_run_status st{};
+        if (int status = wlx_engine_run(e, 1, &st)) return status; // first launch: code load
+        if (st.done) break;
+        const auto t0 = std::chrono::steady_clock::now();
+        if (int status = wlx_engine_run(e, epochs, &st)) return status;
+        const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (seconds) seconds[variant] = s;
+        if (best_s < 0.0 || s < best_s) {
+            best_s = s;
+            best = variant;
+        }
+        if (st.done) break;
+    }
+    e->kernel_choice = best;
+    if (chosen) *chosen = best;
     return WLX_OK;
 }
 

@@ -338,6 +338,27 @@ class Engine:
         """FP64 recompute of every configuration's fields and energy (FP32 fields mode)."""
         self.backend.call("engine_resync", self.handle)
 
+    def set_kernel(self, kernel: int):
+        """Forces a walker kernel family (abi.KERNEL_*); all give the same trajectory."""
+        self.backend.call("engine_set_kernel", self.handle, ctypes.c_uint64(kernel))
+
+    def kernel(self) -> tuple[int, str]:
+        """(family, kernel name) of the next sampling launch."""
+        family = ctypes.c_uint64()
+        name = ctypes.create_string_buffer(128)
+        self.backend.call("engine_kernel", self.handle, ctypes.byref(family), name,
+                          ctypes.c_size_t(128))
+        return family.value, name.value.decode()
+
+    def autotune(self, epochs: int) -> tuple[int, list[float]]:
+        """Runs `epochs` epochs of the live run with each eligible kernel family and keeps the
+        fastest: (chosen family, seconds per family, -1 where not tried)."""
+        chosen = ctypes.c_uint64()
+        seconds = (ctypes.c_double * 4)()
+        self.backend.call("engine_autotune", self.handle, ctypes.c_uint64(epochs),
+                          ctypes.byref(chosen), seconds)
+        return chosen.value, list(seconds)
+
     def drift(self) -> tuple[float, float, int]:
         """(max field error, max energy error in bins, resyncs) over all resyncs so far."""
         f, e, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_uint64()
