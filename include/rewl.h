@@ -49,12 +49,13 @@ enum wlx_fields { WLX_FIELDS_FP64 = 0, WLX_FIELDS_FP32_KAHAN = 1 };
  * AUTO       lookahead for runs without statistics collection;
  * LOOKAHEAD  batches of proposals resolved together, fields in registers (N <= 4096);
  * SEQUENTIAL one attempt at a time, fields in registers;
- * CSR        one attempt at a time, fields and spins in LDS, CSR row scatter (CSR systems). */
+ * LDS        one attempt at a time, fields and spins in LDS; a flip scatters the CSR row of a
+ *            CSR system or streams the dense row otherwise. Serves any N in one kernel. */
 enum wlx_walker_kernel {
     WLX_KERNEL_AUTO = 0,
     WLX_KERNEL_LOOKAHEAD = 1,
     WLX_KERNEL_SEQUENTIAL = 2,
-    WLX_KERNEL_CSR = 3
+    WLX_KERNEL_LDS = 3
 };
 
 /* trace flags, one byte per recorded attempt */
@@ -265,6 +266,23 @@ int wlx_engine_read_sampling(wlx_engine *engine, uint64_t walker, double *square
 int wlx_engine_set_kernel(wlx_engine *engine, uint64_t kernel);
 int wlx_engine_kernel(const wlx_engine *engine, uint64_t *kernel, char *name, size_t capacity);
 int wlx_engine_autotune(wlx_engine *engine, uint64_t epochs, uint64_t *chosen, double *seconds);
+/* ---- batches of engines (uneven systems in one launch) -------------------------------------
+ * No reference counterpart (the reference runs one system per process). A batch runs the
+ * epochs of several initialized engines together: per epoch one sampling launch over every
+ * walker of every engine (the LDS walker kernel, one wave per walker) and one exchange launch
+ * (one workgroup per engine). Each engine keeps its own schedule, gate, counters and results,
+ * identical to running it alone with the same kernel family. The engines must share the
+ * proposal stream and use FP64 fields without statistics or traces; the batch captures their
+ * parameters at creation, and an engine must not be run on its own while the batch runs. */
+typedef struct wlx_batch wlx_batch;
+int wlx_batch_create(wlx_engine *const *engines, uint64_t count, wlx_batch **out);
+int wlx_batch_destroy(wlx_batch *batch);
+/* Captures `epochs_per_graph` epochs in a HIP graph that run replays (0 = plain launches). */
+int wlx_batch_set_graph(wlx_batch *batch, uint64_t epochs_per_graph);
+/* Up to max_epochs epochs (engines that finish stop on the device); returns the epochs
+ * enqueued and how many engines are done. */
+int wlx_batch_run(wlx_batch *batch, uint64_t max_epochs, uint64_t *epochs,
+                  uint64_t *done_engines);
 /* WangLandauWalker::histogram_statistics (wang_landau.cpp:344-367), computed on the device. */
 int wlx_engine_histogram_statistics(wlx_engine *engine, uint64_t walker,
                                     wlx_histogram_stats *out);

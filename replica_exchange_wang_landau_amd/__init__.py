@@ -7,6 +7,7 @@ include/rewl.h; this package is its Python face (ctypes) and the host-side helpe
 from .abi import load_library  # noqa: F401
 from .engine import (  # noqa: F401
     Backend,
+    Batch,
     DeviceError,
     Engine,
     GpuBackend,

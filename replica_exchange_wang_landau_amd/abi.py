@@ -31,7 +31,7 @@ FIELDS_FP32_KAHAN = 1
 KERNEL_AUTO = 0
 KERNEL_LOOKAHEAD = 1
 KERNEL_SEQUENTIAL = 2
-KERNEL_CSR = 3
+KERNEL_LDS = 3
 
 TRACE_ACCEPTED = 1
 TRACE_FORCED = 2
@@ -161,6 +161,10 @@ EXPORTS = [
     "wlx_engine_set_kernel",
     "wlx_engine_kernel",
     "wlx_engine_autotune",
+    "wlx_batch_create",
+    "wlx_batch_destroy",
+    "wlx_batch_set_graph",
+    "wlx_batch_run",
     "wlx_engine_histogram_statistics",
     "wlx_engine_ready",
     "wlx_engine_next_iteration",

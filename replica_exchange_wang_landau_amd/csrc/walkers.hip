@@ -657,21 +657,21 @@ __global__ __launch_bounds__(kWave) void sample_kernel(Args a, uint64_t request,
 }
 
 // ---------------------------------------------------------------------------------------------
-// CSR walker (WLX_KERNEL_CSR_LDS): the same attempt sequence as sample_kernel for a CSR system,
-// with the configuration's fields and spins staged in LDS instead of registers, so one kernel
-// serves every N (no per-size register layout) and a batch of uneven systems shares a launch.
-// An accepted flip scatters its CSR row, h[col] += m val (physics.cpp:231-236), one column per
-// lane: the columns of a row are distinct, so the lanes never collide, and fma(m, val, h) is
-// the reference's rounding because m val is exact. LDS: bins | fields (np doubles) | spins
-// (np bytes).
+// LDS walker (WLX_KERNEL_LDS): the same attempt sequence as sample_kernel with the
+// configuration's fields and spins staged in LDS instead of registers, so one kernel serves
+// every N (no per-size register layout) and a batch of uneven systems shares a launch. An
+// accepted flip on a CSR system scatters its CSR row, h[col] += m val (physics.cpp:231-236),
+// one column per lane: the columns of a row are distinct, so the lanes never collide. On a
+// dense system it streams the row, h[j] += m J[s][j] (physics.cpp:190-195). fma(m, J, h) is the
+// reference's rounding because m J is exact. LDS: bins | fields (np doubles) | spins (np bytes).
 // ---------------------------------------------------------------------------------------------
-__host__ __device__ inline size_t csr_lds_bytes(uint32_t ws_stride, uint32_t np)
+__host__ __device__ inline size_t lds_walker_bytes(uint32_t ws_stride, uint32_t np)
 {
     return bins_lds_bytes(ws_stride) + size_t(np) * 9;
 }
 
 template <bool PHILOX>
-__device__ inline void csr_walk(const Args &a, uint32_t w, uint64_t request, uint32_t flags,
+__device__ inline void lds_walk(const Args &a, uint32_t w, uint64_t request, uint32_t flags,
                                 unsigned char *smem)
 {
     const uint32_t lane = lane_id();
@@ -740,10 +740,15 @@ __device__ inline void csr_walk(const Args &a, uint32_t w, uint64_t request, uin
         ++st.attempted;
         if (accepted) {
             const double m = -2.0 * sigma;
-            const uint64_t row_end = a.csr_offsets[s + 1];
-            for (uint64_t k = a.csr_offsets[s] + lane; k < row_end; k += kWave) {
-                const uint32_t c = a.csr_columns[k];
-                hf[c] = fma(m, a.csr_values[k], hf[c]);
+            if (a.csr_offsets) {
+                const uint64_t row_end = a.csr_offsets[s + 1];
+                for (uint64_t k = a.csr_offsets[s] + lane; k < row_end; k += kWave) {
+                    const uint32_t c = a.csr_columns[k];
+                    hf[c] = fma(m, a.csr_values[k], hf[c]);
+                }
+            } else {
+                const double *row = a.J + size_t(s) * a.np;
+                for (uint32_t j = lane; j < a.n; j += kWave) hf[j] = fma(m, row[j], hf[j]);
             }
             if (lane == 0) sp[s] = sigma < 0.0 ? int8_t(1) : int8_t(-1);
             __syncthreads(); // the scattered fields before the next attempt reads one
@@ -785,11 +790,11 @@ __device__ inline void csr_walk(const Args &a, uint32_t w, uint64_t request, uin
 }
 
 template <bool PHILOX>
-__global__ __launch_bounds__(kWave) void csr_sample_kernel(Args a, uint64_t request,
+__global__ __launch_bounds__(kWave) void lds_sample_kernel(Args a, uint64_t request,
                                                            uint32_t flags)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    csr_walk<PHILOX>(a, blockIdx.x, request, flags, smem);
+    lds_walk<PHILOX>(a, blockIdx.x, request, flags, smem);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2064,8 +2069,7 @@ __global__ __launch_bounds__(kWave) void pilot_kernel(Args a, uint64_t seed, uin
 // ---------------------------------------------------------------------------------------------
 constexpr int kExchangeBlock = 256;
 
-__global__ __launch_bounds__(kExchangeBlock) void exchange_kernel(Args a, uint32_t gated,
-                                                                  uint64_t epoch_arg)
+__device__ inline void exchange_epoch(const Args &a, uint32_t gated, uint64_t epoch_arg)
 {
     if (gated && a.glob->done) return;
     const uint64_t epoch = gated ? a.glob->epoch : epoch_arg;
@@ -2147,6 +2151,38 @@ __global__ __launch_bounds__(kExchangeBlock) void exchange_kernel(Args a, uint32
     }
 }
 
+__global__ __launch_bounds__(kExchangeBlock) void exchange_kernel(Args a, uint32_t gated,
+                                                                  uint64_t epoch_arg)
+{
+    exchange_epoch(a, gated, epoch_arg);
+}
+
+// ---------------------------------------------------------------------------------------------
+// batches (wlx_batch): the walkers of several engines in one sampling launch (one wave per
+// walker, the LDS walker, engine parameters from a device table) and their replica exchanges
+// in one launch (one workgroup per engine). Each engine keeps its own gate, epoch and counters.
+// ---------------------------------------------------------------------------------------------
+struct BatchSlot {
+    uint32_t engine, walker;
+};
+
+template <bool PHILOX>
+__global__ __launch_bounds__(kWave) void batch_sample_kernel(const Args *table,
+                                                             const BatchSlot *slots,
+                                                             uint32_t flags)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const BatchSlot b = slots[blockIdx.x];
+    const Args a = table[b.engine];
+    lds_walk<PHILOX>(a, b.walker, 0, flags, smem);
+}
+
+__global__ __launch_bounds__(kExchangeBlock) void batch_exchange_kernel(const Args *table)
+{
+    const Args a = table[blockIdx.x];
+    exchange_epoch(a, 1, 0);
+}
+
 // per-walker operations for the harness: 0 statistics, 1 ready, 2 begin_next_iteration
 __global__ __launch_bounds__(kWave) void walker_op_kernel(Args a, uint32_t w, uint32_t op,
                                                           uint64_t *out)
@@ -2182,7 +2218,7 @@ using PilotFn = void (*)(Args, uint64_t, uint64_t, double *);
 
 struct KernelSet {
     SampleFn sample_xo, sample_xo_fast, sample_ph, sample_ph_fast, sample_f32, sample_ph_seq;
-    SampleFn csr_xo, csr_ph; // any N
+    SampleFn lds_xo, lds_ph; // any N
     InitFn init;
     FieldsFn fields;
     PilotFn pilot;
@@ -2193,7 +2229,7 @@ template <int NPL> KernelSet kernel_set()
     return {sample_kernel<NPL, false>, lookahead_xo_kernel<NPL>, lookahead_kernel<NPL>,
             lookahead_fast_kernel<NPL>,
             lookahead_f32_kernel<NPL>, sample_kernel<NPL, true>,
-            csr_sample_kernel<false>, csr_sample_kernel<true>,
+            lds_sample_kernel<false>, lds_sample_kernel<true>,
             init_kernel<NPL>, fields_kernel<NPL>, pilot_kernel<NPL>};
 }
 
@@ -2292,7 +2328,7 @@ struct wlx_engine {
     uint64_t *d_op = nullptr;
     hipStream_t stream = nullptr;
     bool initialized = false;
-    size_t lds_sample = 0, lds_init = 0, lds_csr = 0;
+    size_t lds_sample = 0, lds_init = 0, lds_walker = 0;
     uint64_t kernel_choice = WLX_KERNEL_AUTO;
     bool timing = false;
     std::vector<hipEvent_t> events;
@@ -2354,7 +2390,7 @@ struct wlx_engine {
         a.nalivaiko = cfg.wl.nalivaiko_mod ? 1 : 0;
         a.philox = cfg.rng == WLX_RNG_PHILOX ? 1 : 0;
         a.unlimited = cfg.max_attempts == 0 ? 1 : 0;
-        a.csr_offsets = sys->d_offsets;
+        a.csr_offsets = sys->csr ? sys->d_offsets : nullptr; // null: dense rows
         a.csr_columns = sys->d_columns;
         a.csr_values = sys->d_values;
         a.sq_disp = d_sq;
@@ -2374,7 +2410,7 @@ struct wlx_engine {
         const bool philox = cfg.rng == WLX_RNG_PHILOX;
         // the statistics are recorded by the one-flip-at-a-time kernels
         if (collect) return philox ? kernels.sample_ph_seq : kernels.sample_xo;
-        if (kernel_choice == WLX_KERNEL_CSR) return philox ? kernels.csr_ph : kernels.csr_xo;
+        if (kernel_choice == WLX_KERNEL_LDS) return philox ? kernels.lds_ph : kernels.lds_xo;
         if (kernel_choice == WLX_KERNEL_SEQUENTIAL)
             return philox ? kernels.sample_ph_seq : kernels.sample_xo;
         const bool common =
@@ -2389,8 +2425,8 @@ struct wlx_engine {
     {
         if (trace_cap) flags |= kTrace;
         const SampleFn fn = sample_fn();
-        const bool csr_fn = fn == kernels.csr_xo || fn == kernels.csr_ph;
-        hipLaunchKernelGGL(fn, dim3(walkers), dim3(kWave), csr_fn ? lds_csr : lds_sample, stream, args(),
+        const bool lds_fn = fn == kernels.lds_xo || fn == kernels.lds_ph;
+        hipLaunchKernelGGL(fn, dim3(walkers), dim3(kWave), lds_fn ? lds_walker : lds_sample, stream, args(),
                            request, flags);
         REWL_HIP(hipGetLastError());
         return WLX_OK;
@@ -2608,11 +2644,11 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
             return status;
     if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.init), e->lds_init))
         return status;
-    if (sys->csr) { // the CSR walker stages the configuration in LDS beside the bins
-        e->lds_csr = csr_lds_bytes(e->ws_stride, uint32_t(sys->np));
-        if (e->lds_csr <= 160 * 1024)
-            for (SampleFn fn : {e->kernels.csr_xo, e->kernels.csr_ph})
-                if (int status = set_lds_limit(reinterpret_cast<const void *>(fn), e->lds_csr))
+    { // the LDS walker stages the configuration in LDS beside the bins
+        e->lds_walker = lds_walker_bytes(e->ws_stride, uint32_t(sys->np));
+        if (e->lds_walker <= 160 * 1024)
+            for (SampleFn fn : {e->kernels.lds_xo, e->kernels.lds_ph})
+                if (int status = set_lds_limit(reinterpret_cast<const void *>(fn), e->lds_walker))
                     return status;
     }
     if (int status = set_lds_limit(reinterpret_cast<const void *>(walker_op_kernel),
@@ -2845,10 +2881,8 @@ int wlx_engine_resync(wlx_engine *e)
 int wlx_engine_set_kernel(wlx_engine *e, uint64_t kernel)
 {
     if (int status = require_engine(e, false)) return status;
-    if (kernel > WLX_KERNEL_CSR) return set_error(WLX_INVALID_ARGUMENT, "Unknown walker kernel");
-    if (kernel == WLX_KERNEL_CSR && !e->sys->csr)
-        return set_error(WLX_INVALID_ARGUMENT, "The CSR walker kernel needs a CSR system");
-    if (kernel == WLX_KERNEL_CSR && e->lds_csr > 160 * 1024)
+    if (kernel > WLX_KERNEL_LDS) return set_error(WLX_INVALID_ARGUMENT, "Unknown walker kernel");
+    if (kernel == WLX_KERNEL_LDS && e->lds_walker > 160 * 1024)
         return set_error(WLX_INVALID_ARGUMENT, "Fields and window bins exceed the LDS of a CU");
     e->kernel_choice = kernel;
     return WLX_OK;
@@ -2864,9 +2898,9 @@ int wlx_engine_kernel(const wlx_engine *e, uint64_t *kernel, char *name, size_t 
     if (fn == k.sample_xo || fn == k.sample_ph_seq) {
         family = WLX_KERNEL_SEQUENTIAL;
         base = "sample_kernel";
-    } else if (fn == k.csr_xo || fn == k.csr_ph) {
-        family = WLX_KERNEL_CSR;
-        base = "csr_sample_kernel";
+    } else if (fn == k.lds_xo || fn == k.lds_ph) {
+        family = WLX_KERNEL_LDS;
+        base = "lds_sample_kernel";
     } else if (fn == k.sample_xo_fast) {
         base = "lookahead_xo_kernel";
     } else if (fn == k.sample_ph_fast) {
@@ -2876,7 +2910,7 @@ int wlx_engine_kernel(const wlx_engine *e, uint64_t *kernel, char *name, size_t 
     }
     if (kernel) *kernel = family;
     if (name && capacity) {
-        if (family == WLX_KERNEL_CSR)
+        if (family == WLX_KERNEL_LDS)
             std::snprintf(name, capacity, "%s<%s>", base,
                           e->cfg.rng == WLX_RNG_PHILOX ? "true" : "false");
         else if (family == WLX_KERNEL_SEQUENTIAL)
@@ -2893,9 +2927,9 @@ int wlx_engine_autotune(wlx_engine *e, uint64_t epochs, uint64_t *chosen, double
     if (int status = require_engine(e)) return status;
     if (epochs == 0) return set_error(WLX_INVALID_ARGUMENT, "Autotuning needs at least one epoch");
     if (seconds)
-        for (int k = 0; k <= int(WLX_KERNEL_CSR); ++k) seconds[k] = -1.0;
+        for (int k = 0; k <= int(WLX_KERNEL_LDS); ++k) seconds[k] = -1.0;
     std::vector<uint64_t> candidates{WLX_KERNEL_LOOKAHEAD, WLX_KERNEL_SEQUENTIAL};
-    if (e->sys->csr && e->lds_csr <= 160 * 1024) candidates.push_back(WLX_KERNEL_CSR);
+    if (e->lds_walker <= 160 * 1024) candidates.push_back(WLX_KERNEL_LDS);
     uint64_t best = e->kernel_choice == WLX_KERNEL_AUTO ? WLX_KERNEL_LOOKAHEAD : e->kernel_choice;
     double best_s = -1.0;
     // every variant yields the same trajectory, so the trials are part of the run itself
@@ -3393,6 +3427,174 @@ int wlx_engine_kernel_time(wlx_engine *e, double *sample_ms, uint64_t *launches)
     if (!e) return set_error(WLX_INVALID_ARGUMENT, "Null engine");
     if (sample_ms) *sample_ms = e->launches ? e->sample_ms / double(e->launches) : 0.0;
     if (launches) *launches = e->launches;
+    return WLX_OK;
+}
+
+} // extern "C"
+
+// ---- batches --------------------------------------------------------------------------------
+struct wlx_batch {
+    std::vector<wlx_engine *> engines;
+    Args *d_table = nullptr;
+    BatchSlot *d_slots = nullptr;
+    uint32_t slots = 0;
+    size_t lds = 0;
+    bool philox = false;
+    hipStream_t stream = nullptr;
+    uint64_t graph_epochs = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<Globals> globs;
+    double sample_seconds = 0.0;
+
+    int enqueue_epochs(uint64_t count)
+    {
+        const auto sample = philox ? batch_sample_kernel<true> : batch_sample_kernel<false>;
+        for (uint64_t k = 0; k < count; ++k) {
+            hipLaunchKernelGGL(sample, dim3(slots), dim3(kWave), lds, stream, d_table, d_slots,
+                               uint32_t(kGate | kCheck));
+            REWL_HIP(hipGetLastError());
+            hipLaunchKernelGGL(batch_exchange_kernel, dim3(uint32_t(engines.size())),
+                               dim3(kExchangeBlock), 0, stream, d_table);
+            REWL_HIP(hipGetLastError());
+        }
+        return WLX_OK;
+    }
+
+    void release_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+};
+
+namespace {
+int batch_poll(wlx_batch *b, uint64_t &done)
+{
+    for (size_t k = 0; k < b->engines.size(); ++k)
+        REWL_HIP(hipMemcpyAsync(&b->globs[k], b->engines[k]->d_glob, sizeof(Globals),
+                                hipMemcpyDeviceToHost, b->stream));
+    REWL_HIP(hipStreamSynchronize(b->stream));
+    done = 0;
+    for (const Globals &g : b->globs) {
+        if (g.escaped) return set_error(WLX_RUNTIME_ERROR, "Walker escaped its energy window");
+        done += g.done ? 1 : 0;
+    }
+    return WLX_OK;
+}
+} // namespace
+
+extern "C" {
+
+int wlx_batch_create(wlx_engine *const *engines, uint64_t count, wlx_batch **out)
+{
+    if (!engines || !out || count == 0) return set_error(WLX_INVALID_ARGUMENT, "Null argument");
+    *out = nullptr;
+    if (count > (uint64_t(1) << 20)) return set_error(WLX_INVALID_ARGUMENT, "Too many engines");
+    auto b = std::make_unique<wlx_batch>();
+    std::vector<Args> table;
+    std::vector<std::pair<uint64_t, BatchSlot>> order; // heaviest systems first
+    for (uint64_t k = 0; k < count; ++k) {
+        wlx_engine *e = engines[k];
+        if (int status = require_engine(e)) return status;
+        for (uint64_t j = 0; j < k; ++j)
+            if (engines[j] == e) return set_error(WLX_INVALID_ARGUMENT, "Engine listed twice");
+        const bool philox = e->cfg.rng == WLX_RNG_PHILOX;
+        if (k == 0) b->philox = philox;
+        if (philox != b->philox)
+            return set_error(WLX_INVALID_ARGUMENT, "Batched engines share one proposal stream");
+        if (e->f32 || e->collect || e->trace_cap)
+            return set_error(WLX_INVALID_ARGUMENT,
+                             "Batched engines run FP64 fields without statistics or traces");
+        if (e->lds_walker > 160 * 1024)
+            return set_error(WLX_INVALID_ARGUMENT, "Fields and window bins exceed the LDS of a CU");
+        REWL_HIP(hipStreamSynchronize(e->stream));
+        table.push_back(e->args());
+        b->lds = std::max(b->lds, e->lds_walker);
+        for (uint32_t w = 0; w < e->walkers; ++w)
+            order.push_back({e->sys->n, BatchSlot{uint32_t(k), w}});
+        b->engines.push_back(e);
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [](const auto &x, const auto &y) { return x.first > y.first; });
+    std::vector<BatchSlot> slots;
+    for (const auto &o : order) slots.push_back(o.second);
+    if (slots.size() >= (size_t(1) << 31)) return set_error(WLX_INVALID_ARGUMENT, "Too many walkers");
+    b->slots = uint32_t(slots.size());
+    b->globs.resize(count);
+    REWL_HIP(hipStreamCreate(&b->stream));
+    if (int st = device_alloc(&b->d_table, count)) return st;
+    if (int st = device_alloc(&b->d_slots, slots.size())) return st;
+    REWL_HIP(hipMemcpyAsync(b->d_table, table.data(), count * sizeof(Args), hipMemcpyHostToDevice,
+                            b->stream));
+    REWL_HIP(hipMemcpyAsync(b->d_slots, slots.data(), slots.size() * sizeof(BatchSlot),
+                            hipMemcpyHostToDevice, b->stream));
+    REWL_HIP(hipStreamSynchronize(b->stream));
+    for (const void *fn : {reinterpret_cast<const void *>(batch_sample_kernel<true>),
+                           reinterpret_cast<const void *>(batch_sample_kernel<false>)})
+        if (int status = set_lds_limit(fn, b->lds)) return status;
+    *out = b.release();
+    return WLX_OK;
+}
+
+int wlx_batch_destroy(wlx_batch *b)
+{
+    if (!b) return WLX_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    b->release_graph();
+    if (b->d_table) (void)hipFree(b->d_table);
+    if (b->d_slots) (void)hipFree(b->d_slots);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return WLX_OK;
+}
+
+int wlx_batch_set_graph(wlx_batch *b, uint64_t epochs_per_graph)
+{
+    if (!b) return set_error(WLX_INVALID_ARGUMENT, "Null batch");
+    if (epochs_per_graph > 4096)
+        return set_error(WLX_INVALID_ARGUMENT, "At most 4096 epochs per graph");
+    b->release_graph();
+    b->graph_epochs = epochs_per_graph;
+    if (epochs_per_graph == 0) return WLX_OK;
+    REWL_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int status = b->enqueue_epochs(epochs_per_graph);
+    hipGraph_t graph = nullptr;
+    const hipError_t end = hipStreamEndCapture(b->stream, &graph);
+    if (status) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return status;
+    }
+    REWL_HIP(end);
+    b->graph = graph;
+    REWL_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return WLX_OK;
+}
+
+int wlx_batch_run(wlx_batch *b, uint64_t max_epochs, uint64_t *epochs, uint64_t *done_engines)
+{
+    if (!b) return set_error(WLX_INVALID_ARGUMENT, "Null batch");
+    uint64_t ran = 0, done = 0;
+    if (int status = batch_poll(b, done)) return status;
+    uint64_t chunk = 1;
+    const auto t0 = std::chrono::steady_clock::now();
+    while (ran < max_epochs && done < b->engines.size()) {
+        if (b->exec && max_epochs - ran >= b->graph_epochs) {
+            REWL_HIP(hipGraphLaunch(b->exec, b->stream));
+            ran += b->graph_epochs;
+        } else {
+            const uint64_t n = std::min(max_epochs - ran, chunk);
+            if (int status = b->enqueue_epochs(n)) return status;
+            ran += n;
+            chunk = std::min<uint64_t>(chunk * 2, 64);
+        }
+        if (int status = batch_poll(b, done)) return status;
+    }
+    b->sample_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (epochs) *epochs = ran;
+    if (done_engines) *done_engines = done;
     return WLX_OK;
 }
 

@@ -525,3 +525,42 @@ class Engine:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Several engines whose epochs run together (wlx_batch): one sampling launch over all
+    their walkers and one exchange launch per epoch, optionally replayed from a HIP graph.
+    The engines stay usable for reads; each one's results equal running it alone."""
+
+    def __init__(self, engines):
+        if not engines:
+            raise InvalidArgument("A batch needs at least one engine")
+        self.engines = list(engines)
+        self.backend = self.engines[0].backend
+        handles = (ctypes.c_void_p * len(self.engines))(*[e.handle.value for e in self.engines])
+        out = ctypes.c_void_p()
+        self.backend.call("batch_create", handles, ctypes.c_uint64(len(self.engines)),
+                          ctypes.byref(out))
+        self.handle = out
+
+    def set_graph(self, epochs_per_graph: int):
+        """Replays `epochs_per_graph` captured epochs per launch (0: plain launches)."""
+        self.backend.call("batch_set_graph", self.handle, ctypes.c_uint64(epochs_per_graph))
+
+    def run(self, max_epochs: int = 2**63) -> tuple[int, int]:
+        """(epochs enqueued, engines done)."""
+        epochs, done = ctypes.c_uint64(), ctypes.c_uint64()
+        self.backend.call("batch_run", self.handle, ctypes.c_uint64(max_epochs),
+                          ctypes.byref(epochs), ctypes.byref(done))
+        return epochs.value, done.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.backend.call("batch_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,116 @@
+"""Batches of uneven systems in one launch (wlx_batch): every engine of a batch ends up
+bit-identical to the same engine run alone, and to the oracle; HIP-graph replays equal plain
+launches on every replay; engines that finish early stop on the device while the others go on.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from harness import assert_walkers_equal
+from replica_exchange_wang_landau_amd import Batch, abi
+from replica_exchange_wang_landau_amd.engine import (Engine, InvalidArgument, RunConfig, System,
+                                                     WlParameters, simple_cubic)
+from test_gpu_lds_walker import dilute, grid_for
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(64, 1), (300, 2), (1000, 3), (2500, 4), (4096, 5)]
+
+
+def c3_engines(backend, rng_mode=abi.RNG_PHILOX, max_epochs=0, sizes=SIZES, grids=None,
+               walkers=2):
+    """C3-shaped engines: dilute CSR systems (cutoff 3), windows in proportion to N, bin 1.0,
+    exchange every MCS. `grids` reuses energy ranges (they come from a GPU pilot)."""
+    engines, out_grids = [], []
+    for i, (n, seed) in enumerate(sizes):
+        system = System(*dilute(n, seed), backend=backend, cutoff=3.0)
+        emin, emax = grids[i] if grids else grid_for(system, n)
+        out_grids.append((emin, emax))
+        windows = min(16, max(4, round(4 + 12 * (n - 64) / 4032)))
+        cfg = RunConfig(emin=float(emin), emax=float(emax), bin_width=1.0, windows=windows,
+                        walkers_per_window=walkers, seed=1234 + seed,
+                        exchange_interval_attempts=n,
+                        max_attempts=n * max_epochs if max_epochs else 10**12, rng=rng_mode,
+                        wl=WlParameters(check_interval_attempts=5 * n))
+        e = Engine(system, cfg)
+        e.initialize()
+        if backend.__class__.__name__ == "GpuBackend":
+            e.set_kernel(abi.KERNEL_LDS)
+        engines.append(e)
+    return engines, out_grids
+
+
+@pytest.mark.parametrize("rng_mode", [abi.RNG_XOSHIRO, abi.RNG_PHILOX])
+def test_batch_equals_engines_alone_and_oracle(gpu, orc, rng_mode):
+    batched, grids = c3_engines(gpu, rng_mode)
+    alone, _ = c3_engines(gpu, rng_mode, grids=grids)
+    batch = Batch(batched)
+    epochs, done = batch.run(12)
+    assert (epochs, done) == (12, 0)
+    for e in alone:
+        e.run(12)
+    for a, b in zip(batched, alone):
+        sa, sb = a.status(), b.status()
+        assert (sa.epoch, sa.attempted, sa.accepted, sa.exchange_accepted) == (
+            sb.epoch, sb.attempted, sb.accepted, sb.exchange_accepted)
+        for k in range(a.walkers):
+            assert_walkers_equal(a, b, k)
+    oracle, _ = c3_engines(orc, rng_mode, sizes=SIZES[:2], grids=grids[:2])
+    for a, o in zip(batched, oracle):
+        o.run(12)
+        for k in range(a.walkers):
+            assert_walkers_equal(a, o, k)
+
+
+def test_graph_replays_equal_plain_launches(gpu):
+    graphed, grids = c3_engines(gpu)
+    plain, _ = c3_engines(gpu, grids=grids)
+    bg, bp = Batch(graphed), Batch(plain)
+    bg.set_graph(4)
+    for _ in range(3):  # three replays, host reads in between
+        assert bg.run(4) == (4, 0)
+        assert bp.run(4) == (4, 0)
+        for a, b in zip(graphed, plain):
+            for k in range(a.walkers):
+                assert_walkers_equal(a, b, k)
+    assert graphed[0].status().epoch == 12
+
+
+def test_finished_engines_stop_on_the_device(gpu):
+    """Engines with attempt limits of 3, 6 and 9 MCS: the batch runs until all are done and
+    each stops at its own limit, like run() of the engine alone."""
+    sizes = SIZES[:3]
+    batched, grids = [], []
+    for limit, size in zip((3, 6, 9), sizes):
+        e, g = c3_engines(gpu, max_epochs=limit, sizes=[size])
+        batched += e
+        grids += g
+    alone = []
+    for limit, size, g in zip((3, 6, 9), sizes, grids):
+        alone += c3_engines(gpu, max_epochs=limit, sizes=[size], grids=[g])[0]
+    batch = Batch(batched)
+    batch.set_graph(2)
+    epochs, done = batch.run(100)
+    assert done == 3 and epochs <= 12
+    for a, b in zip(batched, alone):
+        sb = b.run()
+        sa = a.status()
+        assert sa.done and sb.done and sa.epoch == sb.epoch and sa.attempted == sb.attempted
+        for k in range(a.walkers):
+            assert_walkers_equal(a, b, k)
+
+
+def test_batch_errors(gpu):
+    a, grids = c3_engines(gpu, abi.RNG_PHILOX, sizes=SIZES[:1])
+    b, _ = c3_engines(gpu, abi.RNG_XOSHIRO, sizes=SIZES[:1], grids=grids)
+    with pytest.raises(InvalidArgument, match="share one proposal stream"):
+        Batch(a + b)
+    with pytest.raises(InvalidArgument, match="listed twice"):
+        Batch(a + a)
+    f32 = Engine(System(*simple_cubic(3, 3, 3, 1.0, (0, 0, 1), True), backend=gpu),
+                 RunConfig(emin=-60.0, emax=80.0, bin_width=0.5, rng=abi.RNG_PHILOX,
+                           fields=abi.FIELDS_FP32_KAHAN))
+    f32.initialize()
+    with pytest.raises(InvalidArgument, match="FP64 fields"):
+        Batch(a + [f32])

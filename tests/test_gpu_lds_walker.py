@@ -1,5 +1,5 @@
-"""The CSR walker kernel (WLX_KERNEL_CSR: fields and spins in LDS, CSR row scatter) and the
-kernel autotuner, against the oracle.
+"""The LDS walker kernel (WLX_KERNEL_LDS: fields and spins in LDS, CSR row scatter or dense row
+stream) and the kernel autotuner, against the oracle.
 
 The oracle applies a flip with the reference's CsrCouplings::add_flip_delta
 (physics.cpp:231-236). Bar: bit-exact walkers (spins, fields, energies, log_g, histograms,
@@ -46,19 +46,20 @@ def pair(gpu, orc, geometry, cfg_kw, cutoff=3.0):
     eg, eo = Engine(sg, cfg), Engine(so, cfg)
     eg.initialize()
     eo.initialize()
-    eg.set_kernel(abi.KERNEL_CSR)
+    eg.set_kernel(abi.KERNEL_LDS)
     return eg, eo
 
 
 @pytest.mark.parametrize("rng_mode", [abi.RNG_XOSHIRO, abi.RNG_PHILOX])
 @pytest.mark.parametrize("n,seed", [(64, 1), (333, 2), (1000, 3), (4096, 4)])
 def test_csr_kernel_bit_exact(gpu, orc, rng_mode, n, seed):
+    windows = min(16, max(4, round(4 + 12 * (n - 64) / 4032)))  # C3's windows per system
     eg, eo = pair(gpu, orc, dilute(n, seed),
-                  dict(windows=3, walkers_per_window=2, seed=17, exchange_interval_attempts=n,
-                       max_attempts=n * 10, rng=rng_mode,
+                  dict(windows=windows, walkers_per_window=2, seed=17,
+                       exchange_interval_attempts=n, max_attempts=n * 10, rng=rng_mode,
                        wl=WlParameters(check_interval_attempts=4 * n)))
     family, name = eg.kernel()
-    assert family == abi.KERNEL_CSR and name.startswith("csr_sample_kernel")
+    assert family == abi.KERNEL_LDS and name.startswith("lds_sample_kernel")
     sg, so = eg.run(), eo.run()
     assert (sg.epoch, sg.attempted, sg.accepted, sg.exchange_accepted) == (
         so.epoch, so.attempted, so.accepted, so.exchange_accepted)
@@ -95,7 +96,7 @@ def test_autotune_keeps_the_trajectory(gpu, orc):
                   dict(windows=4, walkers_per_window=4, seed=23, exchange_interval_attempts=n,
                        wl=WlParameters(check_interval_attempts=10 * n)))
     chosen, seconds = eg.autotune(3)
-    assert chosen in (abi.KERNEL_LOOKAHEAD, abi.KERNEL_SEQUENTIAL, abi.KERNEL_CSR)
+    assert chosen in (abi.KERNEL_LOOKAHEAD, abi.KERNEL_SEQUENTIAL, abi.KERNEL_LDS)
     assert all(seconds[k] > 0 for k in (1, 2, 3))
     done = eg.status().epoch
     assert done == 3 * 4  # three families, one warm-up epoch and three timed epochs each
@@ -109,10 +110,31 @@ def test_autotune_keeps_the_trajectory(gpu, orc):
 def test_kernel_selection_errors(gpu):
     e = Engine(System(*simple_cubic(2, 2, 2, 1.0, (0, 0, 1), False), backend=gpu),
                RunConfig(emin=-20.0, emax=20.0, bin_width=0.25, seed=1))
-    with pytest.raises(InvalidArgument, match="needs a CSR system"):
-        e.set_kernel(abi.KERNEL_CSR)
     with pytest.raises(InvalidArgument, match="Unknown walker kernel"):
         e.set_kernel(7)
     e.initialize()
     e.set_kernel(abi.KERNEL_SEQUENTIAL)
     assert e.kernel() == (abi.KERNEL_SEQUENTIAL, "sample_kernel<1,false>")
+
+
+@pytest.mark.parametrize("rng_mode", [abi.RNG_XOSHIRO, abi.RNG_PHILOX])
+@pytest.mark.parametrize("geometry", [(8, 8, 8, 1.0, (0, 0, 1), True),
+                                      (5, 7, 3, 1.0, (0.6, 0.0, 0.8), True)])
+def test_lds_kernel_dense_rows_bit_exact(gpu, orc, rng_mode, geometry):
+    """The LDS walker on dense systems streams whole rows (DenseCouplings::add_flip_delta)."""
+    from harness import c2_config
+    n = geometry[0] * geometry[1] * geometry[2]
+    cfg = c2_config(max_attempts=n * 20, rng=rng_mode, exchange_interval_attempts=n)
+    if n != 512:
+        cfg.emin, cfg.emax, cfg.bin_width = -120.0, 120.0, 0.3
+    eg = Engine(System(*simple_cubic(*geometry), backend=gpu), cfg)
+    eo = Engine(System(*simple_cubic(*geometry), backend=orc), cfg)
+    eg.initialize()
+    eo.initialize()
+    eg.set_kernel(abi.KERNEL_LDS)
+    assert eg.kernel()[1].startswith("lds_sample_kernel")
+    sg, so = eg.run(), eo.run()
+    assert (sg.attempted, sg.accepted, sg.exchange_accepted) == (
+        so.attempted, so.accepted, so.exchange_accepted)
+    for k in range(eg.walkers):
+        assert_walkers_equal(eg, eo, k)
