@@ -26,6 +26,9 @@ is within max|dlog g| < 0.01 of a converged C2 reference DOS, after mean-offset 
 
 `--c4` runs only the C4 precision study instead: a 16x16x16 dense system with FP32 Kahan
 fields, the FP64 matrix-core resync every 10 MCS, and the drift it measures.
+
+`--c3` runs only the batched study: 64 uneven dilute CSR systems (N from 64 to 4096) whose
+walkers all sample in one launch per epoch (the LDS walker kernel), replayed from HIP graphs.
 """
 from __future__ import annotations
 
@@ -244,6 +247,93 @@ def c4_leg(epochs: int, warmup: int, resyncs: int):
     }
 
 
+def c3_systems(count: int = 64, seed: int = 1234):
+    """C3: `count` dilute dipole systems. N_s uniform in {64..4096} from default_rng(seed);
+    positions uniform in a periodic cube of side (N/0.3)^(1/3) and easy axes uniform on the
+    unit sphere from default_rng(seed + s); D = 1; CSR couplings within cutoff 3."""
+    import numpy as np
+
+    from replica_exchange_wang_landau_amd import System
+
+    sizes = np.random.default_rng(seed).integers(64, 4097, size=count)
+    systems = []
+    for s, n in enumerate(sizes):
+        rng = np.random.default_rng(seed + s)
+        side = (n / 0.3) ** (1 / 3)
+        pos = rng.uniform(0.0, side, size=(n, 3))
+        axes = rng.normal(size=(n, 3))
+        axes /= np.sqrt((axes * axes).sum(axis=1))[:, None]
+        systems.append(System(pos, axes, np.array([side, side, side]), True, cutoff=3.0))
+    return systems
+
+
+def c3_leg(epochs: int, warmup: int, graph: int, pilot_mcs: int = 10_000):
+    """C3 batched study: per system the seed-1234 pilot range (pilot_mcs MCS, all systems in
+    one launch) padded by 5 % and snapped outward to bin 1.0; windows = clamp(round(4 + 12
+    (N - 64) / 4032), 4, 16) at overlap 0.75, 8 walkers per window, exchange every MCS, 1/t to
+    ln f = 1e-7, seed 1234 + s, Philox stream, FP64 fields. Every walker of every system runs
+    in one sampling launch per epoch (LDS walker kernel), epochs replayed from HIP graphs of
+    `graph` epochs."""
+    import math
+
+    import numpy as np
+
+    from replica_exchange_wang_landau_amd import Batch, Engine, RunConfig, System, WlParameters, abi
+
+    t_setup = time.perf_counter()
+    systems = c3_systems()
+    lo, hi = System.pilot_many(systems, 1234, [pilot_mcs * s.n for s in systems])
+    pilot_s = time.perf_counter() - t_setup
+    print(f"c3: pilots of {len(systems)} systems in {pilot_s:.1f} s", file=sys.stderr, flush=True)
+    engines = []
+    for s, (system, a, b) in enumerate(zip(systems, lo, hi)):
+        n = system.n
+        pad = 0.05 * (b - a)
+        windows = min(16, max(4, round(4 + 12 * (n - 64) / 4032)))
+        cfg = RunConfig(emin=float(math.floor(a - pad)), emax=float(math.ceil(b + pad)),
+                        bin_width=1.0, windows=windows, walkers_per_window=8, overlap=0.75,
+                        seed=1234 + s, exchange_interval_attempts=n, max_attempts=0,
+                        rng=abi.RNG_PHILOX,
+                        wl=WlParameters(final_factor=1e-7, check_interval_attempts=100 * n))
+        e = Engine(system, cfg)
+        e.initialize()
+        e.set_kernel(abi.KERNEL_LDS)
+        engines.append(e)
+        if s % 16 == 15:
+            print(f"c3: {s + 1} engines initialised", file=sys.stderr, flush=True)
+    setup_s = time.perf_counter() - t_setup
+    batch = Batch(engines)
+    batch.set_graph(graph)
+    batch.run(warmup)
+    before = [e.status() for e in engines]
+    t0 = time.perf_counter()
+    ran, done = batch.run(epochs)
+    seconds = time.perf_counter() - t0
+    after = [e.status() for e in engines]
+    attempted = sum(x.attempted - y.attempted for x, y in zip(after, before))
+    accepted = sum(x.accepted - y.accepted for x, y in zip(after, before))
+    # algorithmic bytes: an accepted flip reads its CSR row, z (value 8 B + column 4 B)
+    row_bytes = sum((x.accepted - y.accepted) * (e.system.nnz / e.system.n) * 12.0
+                    for x, y, e in zip(after, before, engines))
+    walkers = sum(e.walkers for e in engines)
+    return {
+        "workload": "C3: 64 dilute dipole systems, N uniform in {64..4096} (seed 1234), density "
+                    "0.3 periodic cube, axes uniform on the sphere, CSR cutoff 3 (FP64 values), "
+                    "4-16 windows x 8 walkers, bin 1.0, exchange every MCS, Philox stream",
+        "systems": len(engines), "walkers": walkers,
+        "spins": [int(e.system.n) for e in engines],
+        "attempted_flips_per_s": attempted / seconds, "epochs_timed": ran, "engines_done": done,
+        "acceptance": accepted / max(1, attempted),
+        "ms_per_epoch": seconds * 1e3 / max(1, ran),
+        "graph_epochs": graph,
+        "roofline": {"bound": "hbm", "achieved": row_bytes / seconds / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": row_bytes / seconds / 1e9 / HBM_PEAK_GBS,
+                     "note": "CSR row bytes of accepted flips over the whole epoch time"},
+        "setup_seconds": {"pilot": pilot_s, "total": setup_s},
+    }
+
+
 def calibrate_copy(mib: int):
     """A plain device copy of known size (read mib MiB, write mib MiB), five times: the
     reference point that scales FETCH_SIZE / WRITE_SIZE counter readings to bytes."""
@@ -271,9 +361,15 @@ def main():
     ap.add_argument("--tta-interval", type=int, default=4000, help="epochs between snapshots")
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
                     help="only run a plain copy of MIB MiB (counter calibration)")
+    ap.add_argument("--c3", action="store_true",
+                    help="only run the C3 batched study (64 uneven CSR systems in one launch)")
+    ap.add_argument("--graph", type=int, default=16, help="epochs per HIP graph (--c3)")
     ap.add_argument("--c4", action="store_true",
                     help="only run the C4 FP32-field precision study (resync kernel, drift)")
     args = ap.parse_args()
+    if args.c3:
+        print(json.dumps(c3_leg(args.steps if args.steps != 3000 else 200, 20, args.graph)))
+        return
     if args.c4:
         print(json.dumps(c4_leg(args.steps if args.steps != 3000 else 200, 20, 10)))
         return

@@ -184,6 +184,11 @@ int wlx_system_fields(const wlx_system *system, const int8_t *spins, uint64_t co
  * single-spin flips drawn from Xoshiro256**(seed, 0), tracking the sampled energy range. */
 int wlx_system_pilot(const wlx_system *system, uint64_t seed, uint64_t attempts,
                      double *minimum, double *maximum);
+/* The same sweep for `count` systems in one launch (one workgroup each; attempts[k] flips for
+ * system k), fields and spins in LDS: same results as wlx_system_pilot per system. No
+ * reference counterpart (the reference pilots one system per run). */
+int wlx_systems_pilot(wlx_system *const *systems, uint64_t count, uint64_t seed,
+                      const uint64_t *attempts, double *minimum, double *maximum);
 /* Name of the device the library runs on and its compute-unit count. No reference
  * counterpart. */
 int wlx_device_info(char *name, size_t capacity, uint64_t *compute_units);

@@ -140,6 +140,7 @@ EXPORTS = [
     "wlx_system_read_csr",
     "wlx_system_fields",
     "wlx_system_pilot",
+    "wlx_systems_pilot",
     "wlx_device_info",
     "wlx_engine_create",
     "wlx_engine_destroy",

@@ -2063,6 +2063,69 @@ __global__ __launch_bounds__(kWave) void pilot_kernel(Args a, uint64_t seed, uin
     }
 }
 
+// The same sweep for many systems at once, one workgroup per system, with the fields and spins
+// in LDS (any N): a flip scatters the CSR row of a CSR system or streams the dense row.
+struct PilotJob {
+    const double *J;
+    const uint64_t *offsets; // null for dense systems
+    const uint32_t *columns;
+    const double *values;
+    uint32_t n, np;
+    Bounded bounded;
+    uint64_t seed, attempts;
+    double *range;
+};
+
+__global__ __launch_bounds__(kWave) void pilot_lds_kernel(const PilotJob *jobs)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const PilotJob j = jobs[blockIdx.x];
+    const uint32_t lane = lane_id();
+    double *hf = reinterpret_cast<double *>(smem);
+    int8_t *sp = reinterpret_cast<int8_t *>(hf + j.np);
+    // local_fields of all spins +1 (physics.cpp:238-251): h_i = sum_j J_ij in column order; the
+    // CSR row holds the nonzero columns in that order, and the skipped zeros add nothing
+    for (uint32_t i = lane; i < j.n; i += kWave) {
+        double h = 0.0;
+        if (j.offsets) {
+            for (uint64_t k = j.offsets[i]; k < j.offsets[i + 1]; ++k) h += j.values[k];
+        } else {
+            for (uint32_t c = 0; c < j.n; ++c) h += j.J[size_t(i) * j.np + c];
+        }
+        hf[i] = h;
+        sp[i] = 1;
+    }
+    __syncthreads();
+    double energy = 0.0; // total_energy (physics.cpp:253-259), site order
+    for (uint32_t i = 0; i < j.n; ++i) energy += 0.5 * hf[i];
+    double lo = energy, hi = energy;
+    Xoshiro rng = Xoshiro::seeded(j.seed, 0);
+    for (uint64_t t = 0; t < j.attempts; ++t) {
+        const uint32_t s = uint32_t(draw_bounded(rng, j.bounded));
+        const double sigma = sp[s] < 0 ? -1.0 : 1.0;
+        energy += -2.0 * sigma * hf[s];
+        const double m = -2.0 * sigma;
+        if (j.offsets) {
+            const uint64_t row_end = j.offsets[s + 1];
+            for (uint64_t k = j.offsets[s] + lane; k < row_end; k += kWave) {
+                const uint32_t c = j.columns[k];
+                hf[c] = fma(m, j.values[k], hf[c]);
+            }
+        } else {
+            const double *row = j.J + size_t(s) * j.np;
+            for (uint32_t c = lane; c < j.n; c += kWave) hf[c] = fma(m, row[c], hf[c]);
+        }
+        if (lane == 0) sp[s] = sigma < 0.0 ? int8_t(1) : int8_t(-1);
+        __syncthreads();
+        lo = energy < lo ? energy : lo;
+        hi = hi < energy ? energy : hi;
+    }
+    if (lane == 0) {
+        j.range[0] = lo;
+        j.range[1] = hi;
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // exchange: the even/odd schedule of parallel.cpp:410-421. Each walker is in at most one pair
 // per epoch, so all pairs are independent; a swap exchanges two slot indices.
@@ -2549,6 +2612,58 @@ int wlx_system_pilot(const wlx_system *s, uint64_t seed, uint64_t attempts, doub
         return status;
     *minimum = range[0];
     *maximum = range[1];
+    return WLX_OK;
+}
+
+int wlx_systems_pilot(wlx_system *const *systems, uint64_t count, uint64_t seed,
+                      const uint64_t *attempts, double *minimum, double *maximum)
+{
+    if (!systems || !attempts || !minimum || !maximum || count == 0)
+        return set_error(WLX_INVALID_ARGUMENT, "Null argument");
+    if (count > (uint64_t(1) << 20)) return set_error(WLX_INVALID_ARGUMENT, "Too many systems");
+    std::vector<PilotJob> jobs(count);
+    size_t lds = 0;
+    for (uint64_t k = 0; k < count; ++k) {
+        const wlx_system *s = systems[k];
+        if (!s) return set_error(WLX_INVALID_ARGUMENT, "Null system");
+        jobs[k] = PilotJob{s->d_J64, s->csr ? s->d_offsets : nullptr, s->d_columns, s->d_values,
+                           uint32_t(s->n), uint32_t(s->np), make_bounded(s->n), seed,
+                           attempts[k], nullptr};
+        lds = std::max(lds, size_t(s->np) * 9);
+    }
+    hipStream_t stream = systems[0]->stream;
+    PilotJob *d_jobs = nullptr;
+    double *d_range = nullptr;
+    REWL_HIP(hipMalloc(&d_jobs, count * sizeof(PilotJob)));
+    if (hipMalloc(&d_range, 2 * count * sizeof(double)) != hipSuccess) {
+        (void)hipFree(d_jobs);
+        return set_error(WLX_DEVICE_ERROR, "HIP error: out of memory for pilot ranges");
+    }
+    for (uint64_t k = 0; k < count; ++k) jobs[k].range = d_range + 2 * k;
+    std::vector<double> range(2 * count);
+    hipError_t err = set_lds_limit(reinterpret_cast<const void *>(pilot_lds_kernel), lds) == WLX_OK
+                         ? hipSuccess
+                         : hipErrorInvalidValue;
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(d_jobs, jobs.data(), count * sizeof(PilotJob),
+                             hipMemcpyHostToDevice, stream);
+    if (err == hipSuccess) {
+        hipLaunchKernelGGL(pilot_lds_kernel, dim3(uint32_t(count)), dim3(kWave), lds, stream,
+                           d_jobs);
+        err = hipGetLastError();
+    }
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(range.data(), d_range, 2 * count * sizeof(double),
+                             hipMemcpyDeviceToHost, stream);
+    const hipError_t sync = hipStreamSynchronize(stream);
+    (void)hipFree(d_jobs);
+    (void)hipFree(d_range);
+    REWL_HIP(err);
+    REWL_HIP(sync);
+    for (uint64_t k = 0; k < count; ++k) {
+        minimum[k] = range[2 * k];
+        maximum[k] = range[2 * k + 1];
+    }
     return WLX_OK;
 }
 

@@ -247,6 +247,19 @@ class System:
                           ctypes.c_uint64(attempts), ctypes.byref(lo), ctypes.byref(hi))
         return lo.value, hi.value
 
+    @staticmethod
+    def pilot_many(systems, seed: int, attempts) -> tuple[np.ndarray, np.ndarray]:
+        """The pilot sweep of several systems in one launch (wlx_systems_pilot); attempts per
+        system. Returns (minima, maxima)."""
+        count = len(systems)
+        handles = (ctypes.c_void_p * count)(*[s.handle.value for s in systems])
+        att = np.ascontiguousarray(attempts, dtype=np.uint64)
+        lo, hi = np.empty(count), np.empty(count)
+        systems[0].backend.call("systems_pilot", handles, ctypes.c_uint64(count),
+                                ctypes.c_uint64(seed), _ptr(att, ctypes.c_uint64),
+                                _ptr(lo, ctypes.c_double), _ptr(hi, ctypes.c_double))
+        return lo, hi
+
     def fields(self, spins) -> tuple[np.ndarray, np.ndarray]:
         """local_fields + total_energy (physics.cpp:238-259) per configuration."""
         s = np.ascontiguousarray(spins, dtype=np.int8)

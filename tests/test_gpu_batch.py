@@ -114,3 +114,16 @@ def test_batch_errors(gpu):
     f32.initialize()
     with pytest.raises(InvalidArgument, match="FP64 fields"):
         Batch(a + [f32])
+
+
+def test_batched_pilot_equals_single_pilots(gpu):
+    """wlx_systems_pilot (one launch, LDS fields, CSR or dense rows) against the per-system
+    wl_run --pilot sweep, whose output is pinned to the reference's wl_run."""
+    systems = [System(*dilute(n, seed), backend=gpu, cutoff=3.0) for n, seed in SIZES]
+    systems.append(System(*simple_cubic(8, 8, 8, 1.0, (0, 0, 1), True), backend=gpu))
+    systems.append(System(*simple_cubic(5, 7, 3, 1.0, (0.6, 0.0, 0.8), True), backend=gpu))
+    attempts = [20 * s.n for s in systems]
+    lo, hi = System.pilot_many(systems, 1234, attempts)
+    for k, s in enumerate(systems):
+        a, b = s.pilot(1234, attempts[k])
+        assert (lo[k].hex(), hi[k].hex()) == (a.hex(), b.hex()), k
