@@ -247,10 +247,16 @@ def c4_leg(epochs: int, warmup: int, resyncs: int):
     }
 
 
+C3_MIN_SEPARATION = 0.8
+
+
 def c3_systems(count: int = 64, seed: int = 1234):
     """C3: `count` dilute dipole systems. N_s uniform in {64..4096} from default_rng(seed);
-    positions uniform in a periodic cube of side (N/0.3)^(1/3) and easy axes uniform on the
-    unit sphere from default_rng(seed + s); D = 1; CSR couplings within cutoff 3."""
+    positions uniform in a periodic cube of side (N/0.3)^(1/3) by random sequential addition
+    with a minimum (minimum-image) separation of 0.8, and easy axes uniform on the unit
+    sphere, from default_rng(seed + s); D = 1; CSR couplings within cutoff 3 (z = 33.5).
+    Without the hard core, uniform placement puts some pairs at r ~ 0.01, whose couplings
+    (~1/r^3, up to ~2e3 at N = 1000) spread the energy over ~10^4 bins of width 1."""
     import numpy as np
 
     from replica_exchange_wang_landau_amd import System
@@ -260,7 +266,16 @@ def c3_systems(count: int = 64, seed: int = 1234):
     for s, n in enumerate(sizes):
         rng = np.random.default_rng(seed + s)
         side = (n / 0.3) ** (1 / 3)
-        pos = rng.uniform(0.0, side, size=(n, 3))
+        pos = np.empty((n, 3))
+        placed = 0
+        while placed < n:
+            p = rng.uniform(0.0, side, size=3)
+            d = pos[:placed] - p
+            d -= side * np.round(d / side)
+            if placed and np.min((d * d).sum(axis=1)) < C3_MIN_SEPARATION ** 2:
+                continue
+            pos[placed] = p
+            placed += 1
         axes = rng.normal(size=(n, 3))
         axes /= np.sqrt((axes * axes).sum(axis=1))[:, None]
         systems.append(System(pos, axes, np.array([side, side, side]), True, cutoff=3.0))
@@ -318,8 +333,9 @@ def c3_leg(epochs: int, warmup: int, graph: int, pilot_mcs: int = 10_000):
     walkers = sum(e.walkers for e in engines)
     return {
         "workload": "C3: 64 dilute dipole systems, N uniform in {64..4096} (seed 1234), density "
-                    "0.3 periodic cube, axes uniform on the sphere, CSR cutoff 3 (FP64 values), "
-                    "4-16 windows x 8 walkers, bin 1.0, exchange every MCS, Philox stream",
+                    "0.3 periodic cube (random sequential addition, minimum separation 0.8), "
+                    "axes uniform on the sphere, CSR cutoff 3 (FP64 values), 4-16 windows x 8 "
+                    "walkers, bin 1.0, exchange every MCS, Philox stream",
         "systems": len(engines), "walkers": walkers,
         "spins": [int(e.system.n) for e in engines],
         "attempted_flips_per_s": attempted / seconds, "epochs_timed": ran, "engines_done": done,
