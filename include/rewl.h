@@ -288,6 +288,28 @@ int wlx_batch_set_graph(wlx_batch *batch, uint64_t epochs_per_graph);
  * enqueued and how many engines are done. */
 int wlx_batch_run(wlx_batch *batch, uint64_t max_epochs, uint64_t *epochs,
                   uint64_t *done_engines);
+/* ---- warm-start network (no reference counterpart) ---------------------------------------
+ * A 4-layer, 128-wide MLP, 6 features -> 128 -> 128 -> 128 -> 1 with SiLU hidden
+ * activations, that maps (E/N, N, density, dilution, slab thickness, axis mode), scaled by the
+ * caller, to log g / N. Matrix products take BF16 operands on the matrix cores with FP32
+ * accumulation; biases, the output layer, the mean-squared-error loss and Adam (0.9, 0.999,
+ * 1e-8) are FP32 on FP32 master weights. Parameters are one FP32 array of wlx_mlp_params()
+ * numbers: W0 [128 x 16] (columns 6..15 unused), b0, W1 [128 x 128], b1, W2, b2, w3 [128],
+ * b3. x is row-major [count x 6]. */
+typedef struct wlx_mlp wlx_mlp;
+int wlx_mlp_params(uint64_t *count);
+int wlx_mlp_create(uint64_t seed, wlx_mlp **out);
+int wlx_mlp_destroy(wlx_mlp *mlp);
+int wlx_mlp_get_params(wlx_mlp *mlp, float *params);
+int wlx_mlp_set_params(wlx_mlp *mlp, const float *params);
+int wlx_mlp_predict(wlx_mlp *mlp, const float *x, uint64_t count, float *y);
+/* Loss and gradient of the mean squared error over the samples, without an update. */
+int wlx_mlp_gradient(wlx_mlp *mlp, const float *x, const float *y, uint64_t count, float *loss,
+                     float *gradient);
+/* `steps` Adam steps on minibatches of `batch` samples (epoch-wise shuffles from `seed`);
+ * losses[k] = the minibatch loss before step k (NULL to skip). */
+int wlx_mlp_train(wlx_mlp *mlp, const float *x, const float *y, uint64_t count, uint64_t steps,
+                  uint64_t batch, float learning_rate, uint64_t seed, float *losses);
 /* WangLandauWalker::histogram_statistics (wang_landau.cpp:344-367), computed on the device. */
 int wlx_engine_histogram_statistics(wlx_engine *engine, uint64_t walker,
                                     wlx_histogram_stats *out);

@@ -166,6 +166,14 @@ EXPORTS = [
     "wlx_batch_destroy",
     "wlx_batch_set_graph",
     "wlx_batch_run",
+    "wlx_mlp_params",
+    "wlx_mlp_create",
+    "wlx_mlp_destroy",
+    "wlx_mlp_get_params",
+    "wlx_mlp_set_params",
+    "wlx_mlp_predict",
+    "wlx_mlp_gradient",
+    "wlx_mlp_train",
     "wlx_engine_histogram_statistics",
     "wlx_engine_ready",
     "wlx_engine_next_iteration",
