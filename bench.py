@@ -27,6 +27,10 @@ is within max|dlog g| < 0.01 of a converged C2 reference DOS, after mean-offset 
 `--c4` runs only the C4 precision study instead: a 16x16x16 dense system with FP32 Kahan
 fields, the FP64 matrix-core resync every 10 MCS, and the drift it measures.
 
+`--c5` runs only the learned warm start: converged DOS curves of a slab family (one batch),
+the BF16 network trained on them by the project's kernels, and time to accuracy from flat and
+warm starts on a held-out slab and on C2.
+
 `--c3` runs only the batched study: 64 uneven dilute CSR systems (N from 64 to 4096) whose
 walkers all sample in one launch per epoch (the LDS walker kernel), replayed from HIP graphs.
 """
@@ -350,6 +354,118 @@ def c3_leg(epochs: int, warmup: int, graph: int, pilot_mcs: int = 10_000):
     }
 
 
+def accuracy_time(engine, ref_log_g, ref_valid, budget_s: float, interval: int,
+                  threshold: float = 0.01):
+    """Device seconds (Engine.run only) until the stitched DOS holds every reference bin and
+    is within `threshold` of it after mean-offset alignment; (seconds or None, attempts,
+    best deviation)."""
+    import numpy as np
+
+    from replica_exchange_wang_landau_amd import analysis
+
+    device, best, attempted = 0.0, float("inf"), 0
+    while device < budget_s:
+        t0 = time.perf_counter()
+        st = engine.run(interval)
+        device += time.perf_counter() - t0
+        attempted = st.attempted
+        try:
+            lg, valid = analysis.engine_dos(engine)
+        except (RuntimeError, ValueError):
+            continue
+        if not np.any((ref_valid != 0) & (valid == 0)):
+            dev = analysis.max_deviation(lg, valid, ref_log_g, ref_valid)
+            best = min(best, dev)
+            if dev < threshold:
+                return device, attempted, best
+        if st.done:
+            break
+    return None, attempted, best
+
+
+def c5_leg(curves: int, steps: int, budget_s: float):
+    """Learned warm start. Training: `curves` slabs (Lx = Ly in 6..12, Lz in 2..6, dilution
+    in [0, 0.4], +z or random in-plane axes; seed 1234) run to ln f = 1e-3 together in one
+    batch; the 4 x 128 BF16 network is trained on (E/N, N, density, dilution, Lz, axis mode)
+    -> log g / N by the project's kernels (Adam, lr 1e-3, batch 1024). Evaluation: time to
+    max|dlog g| < 0.01 from a flat start and from a warm start (network log g, ln f = 1e-3) on
+    the held-out 12x12x4 slab at dilution 0.2 (reference: a run of this engine to ln f = 1e-6)
+    and on C2 (reference: tests/golden/c2_reference_dos.json)."""
+    import numpy as np
+
+    from replica_exchange_wang_landau_amd import Engine, System, abi, analysis
+    from replica_exchange_wang_landau_amd import warmstart as ws
+
+    t0 = time.perf_counter()
+    slabs = ws.slab_family(curves, 1234)
+    data = ws.converged_curves(
+        slabs, final_factor=1e-3,
+        report=lambda d, n: print(f"c5: {d}/{n} training curves converged", file=sys.stderr,
+                                  flush=True))
+    keep = [(s, c) for s, c in zip(slabs, data) if (c[2] != 0).sum() > 4]
+    x, y = ws.training_set([s for s, _ in keep], [c for _, c in keep])
+    curve_s = time.perf_counter() - t0
+    mlp = ws.Mlp(1234)
+    t1 = time.perf_counter()
+    losses = mlp.train(x, y, steps=steps, batch=1024, learning_rate=1e-3, seed=5)
+    train_s = time.perf_counter() - t1
+    print(f"c5: {len(keep)} curves, {len(x)} samples in {curve_s:.0f} s; trained {steps} steps "
+          f"in {train_s:.1f} s, loss {losses[0]:.3g} -> {losses[-1]:.3g}", file=sys.stderr,
+          flush=True)
+
+    def evaluate(make_engine, ref_lg, ref_valid, features, interval):
+        out = {}
+        for mode in ("flat", "warm"):
+            engine = make_engine()
+            if mode == "warm":
+                engine.warm_start(ws.predicted_log_g(mlp, engine, features), 1e-3)
+            seconds, attempted, best = accuracy_time(engine, ref_lg, ref_valid, budget_s, interval)
+            out[mode] = {"seconds": seconds, "attempted": int(attempted), "best_max_dev": best}
+            print(f"c5: {mode} {out[mode]}", file=sys.stderr, flush=True)
+        if out["flat"]["seconds"] and out["warm"]["seconds"]:
+            out["speedup"] = out["flat"]["seconds"] / out["warm"]["seconds"]
+        return out
+
+    # held-out slab: reference DOS from a long run of this engine
+    held = ws.Slab(12, 4, 0.2, 0, 999_001)
+    hs = held.system()
+    lo, hi = System.pilot_many([hs], 1234, [2000 * hs.n])
+    def held_engine(seed=77, final=1e-6):
+        cfg = ws.slab_config(hs, lo[0], hi[0], seed, final)
+        e = Engine(hs, cfg)
+        e.initialize()
+        return e
+    ref = held_engine(4242)
+    ref.autotune(2)
+    t2 = time.perf_counter()
+    ref.run()
+    ref_s = time.perf_counter() - t2
+    ref_lg, ref_valid = analysis.engine_dos(ref)
+    print(f"c5: held-out reference ({hs.n} spins) in {ref_s:.0f} s", file=sys.stderr, flush=True)
+    held_result = evaluate(held_engine, ref_lg, ref_valid, held.features(hs.n), 500)
+
+    c2_result = None
+    if REFERENCE_DOS.exists():
+        ref = json.loads(REFERENCE_DOS.read_text())
+        c2_lg = np.array([float.fromhex(v) for v in ref["log_g"]])
+        c2_valid = np.array(ref["valid"], dtype=np.uint8)
+        c2_sys = System.simple_cubic(8, 8, 8, 1.0, (0.0, 0.0, 1.0), True)
+        def c2_engine():
+            e = Engine(c2_sys, c2_config(rng=abi.RNG_PHILOX))
+            e.initialize()
+            return e
+        c2_result = evaluate(c2_engine, c2_lg, c2_valid, (512.0, 1.0, 0.0, 8.0, 0.0), 4000)
+    return {
+        "workload": "C5: learned warm start (4 x 128 BF16 MLP, FP32 accumulate)",
+        "training": {"curves": len(keep), "samples": int(len(x)), "steps": steps,
+                     "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+                     "curve_seconds": curve_s, "train_seconds": train_s},
+        "held_out": {"slab": "12x12x4, dilution 0.2, +z axes", "spins": int(hs.n),
+                     "reference_seconds": ref_s, **held_result},
+        "c2": c2_result,
+    }
+
+
 def calibrate_copy(mib: int):
     """A plain device copy of known size (read mib MiB, write mib MiB), five times: the
     reference point that scales FETCH_SIZE / WRITE_SIZE counter readings to bytes."""
@@ -377,12 +493,18 @@ def main():
     ap.add_argument("--tta-interval", type=int, default=4000, help="epochs between snapshots")
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
                     help="only run a plain copy of MIB MiB (counter calibration)")
+    ap.add_argument("--c5", action="store_true",
+                    help="only run the learned warm-start study")
+    ap.add_argument("--c5-curves", type=int, default=256, help="training curves (--c5)")
     ap.add_argument("--c3", action="store_true",
                     help="only run the C3 batched study (64 uneven CSR systems in one launch)")
     ap.add_argument("--graph", type=int, default=16, help="epochs per HIP graph (--c3)")
     ap.add_argument("--c4", action="store_true",
                     help="only run the C4 FP32-field precision study (resync kernel, drift)")
     args = ap.parse_args()
+    if args.c5:
+        print(json.dumps(c5_leg(args.c5_curves, 3000, 120.0)))
+        return
     if args.c3:
         print(json.dumps(c3_leg(args.steps if args.steps != 3000 else 200, 20, args.graph)))
         return

@@ -263,6 +263,13 @@ int wlx_engine_read_sampling(wlx_engine *engine, uint64_t walker, double *square
                              uint64_t *displacement_samples, uint64_t *round_trips,
                              double *representative_energies, uint8_t *representative_present,
                              int8_t *representative_spins);
+/* Warm start (no reference counterpart: the reference starts WL from log g = 0, ln f = 1,
+ * wang_landau.cpp:91-118). Presets every walker of an initialized engine: its window of
+ * log_g (full grid, any offset, finite everywhere), ln f = factor (0 < factor <= 1) in the WL
+ * stage, empty histograms and visited-bin masks. The 1/t clock starts at window_bins / factor,
+ * so that after the switch to 1/t the factor continues near `factor` (active / (attempted +
+ * window_bins / factor)). Configurations, counters and streams are kept. */
+int wlx_engine_warm_start(wlx_engine *engine, const double *log_g, double factor);
 /* Kernel selection. No reference counterpart: the reference has one CPU walker loop
  * (wang_landau.cpp:244-288). set_kernel forces a family (enum wlx_walker_kernel); kernel
  * reports the family and kernel name the next sampling launch uses; autotune runs `epochs`

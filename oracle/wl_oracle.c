@@ -426,6 +426,8 @@ typedef struct {
     uint32_t *trace_sites, *trace_bins;
     uint8_t *trace_flags;
     int escaped;
+    /* 1/t clock offset of a warm start: factor = active / (attempted + clock_offset) */
+    uint64_t clock_offset;
     /* pilot statistics (collect_window_statistics), wang_landau.cpp:104-118 */
     double *sq_disp;         /* full grid */
     uint64_t *disp_samples;  /* full grid */
@@ -479,7 +481,7 @@ static void update_inverse_time_factor(orc_engine *e, walker_t *w)
 {
     if (w->stage != WLX_STAGE_INVERSE_TIME) return;
     if (w->active_count != 0 && w->attempted != 0)
-        w->factor = (double)w->active_count / (double)w->attempted;
+        w->factor = (double)w->active_count / (double)(w->attempted + w->clock_offset);
     if (w->factor <= e->cfg.wl.final_factor) w->stage = WLX_STAGE_FROZEN;
 }
 
@@ -651,8 +653,8 @@ static int begin_next_iteration(orc_engine *e, walker_t *w)
         w->refine_count = 0;
     }
     if (e->cfg.wl.inverse_time_enabled) {
-        double inverse_time = w->attempted == 0 ? 1.0
-                                                : (double)w->active_count / (double)w->attempted;
+        const uint64_t clock = w->attempted + w->clock_offset;
+        double inverse_time = clock == 0 ? 1.0 : (double)w->active_count / (double)clock;
         if (w->factor <= inverse_time) {
             w->stage = WLX_STAGE_INVERSE_TIME;
             w->factor = inverse_time;
@@ -686,6 +688,7 @@ static int initialize_walker(orc_engine *e, walker_t *w, const int8_t *initial)
     w->attempted = w->accepted = w->forced = w->last_accepted = 0;
     w->stage = WLX_STAGE_WANG_LANDAU;
     w->last_check = 0;
+    w->clock_offset = 0;
     w->last_flatness = 0.0;
     orc_xoshiro_seed(e->cfg.seed, w->id, w->rng);
     if (p->collect_window_statistics) { /* wang_landau.cpp:104-118 */
@@ -1088,6 +1091,36 @@ int orc_engine_read_sampling(orc_engine *e, uint64_t k, double *squared, uint64_
     return WLX_OK;
 }
 
+/* wlx_engine_warm_start: window log g from `log_g`, ln f = factor in the WL stage, empty
+ * histograms and masks, 1/t clock offset ceil(window_bins / factor) (no reference counterpart) */
+int orc_engine_warm_start(orc_engine *e, const double *log_g, double factor)
+{
+    if (!log_g) return fail(WLX_INVALID_ARGUMENT, "Null argument");
+    if (!(factor > 0.0) || !(factor <= 1.0))
+        return fail(WLX_INVALID_ARGUMENT, "The initial ln f must lie in (0, 1]");
+    for (uint64_t b = 0; b < e->grid.count; ++b)
+        if (!isfinite(log_g[b]))
+            return fail(WLX_INVALID_ARGUMENT, "The warm-start log g must be finite on every bin");
+    for (uint64_t k = 0; k < e->walkers; ++k) {
+        walker_t *w = &e->w[k];
+        double lowest = log_g[w->begin];
+        for (uint64_t b = w->begin + 1; b < w->end; ++b)
+            if (log_g[b] < lowest) lowest = log_g[b];
+        for (uint64_t b = 0; b < e->grid.count; ++b) {
+            w->log_g[b] = (b >= w->begin && b < w->end) ? log_g[b] - lowest : 0.0;
+            w->histogram[b] = 0;
+            w->active[b] = 0;
+            w->refine[b] = 0;
+        }
+        w->factor = factor;
+        w->stage = WLX_STAGE_WANG_LANDAU;
+        w->active_count = w->refine_count = 0;
+        w->last_check = w->attempted;
+        w->clock_offset = (uint64_t)ceil((double)(w->end - w->begin) / factor);
+    }
+    return WLX_OK;
+}
+
 int orc_engine_set_walker_cadence(orc_engine *e, uint64_t k, uint64_t last_flatness_check,
                                   double last_flatness)
 {
@@ -1225,6 +1258,7 @@ int orc_engine_write_walker(orc_engine *e, uint64_t k, const wlx_walker_info *s,
     w->forced = s->forced_accepted;
     w->last_accepted = s->last_accepted_attempt;
     w->stage = (int)s->stage;
+    w->clock_offset = 0;
     memcpy(w->rng, s->rng_state, sizeof w->rng);
     double *exact = malloc(n * sizeof(double));
     local_fields(e->sys, w->spins, exact);

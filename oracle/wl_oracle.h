@@ -73,6 +73,7 @@ int orc_engine_status(orc_engine *engine, wlx_run_status *status);
 int orc_engine_read_sampling(orc_engine *engine, uint64_t walker, double *squared,
                              uint64_t *samples, uint64_t *round_trips, double *rep_energy,
                              uint8_t *rep_present, int8_t *rep_spins);
+int orc_engine_warm_start(orc_engine *engine, const double *log_g, double factor);
 int orc_engine_set_walker_cadence(orc_engine *engine, uint64_t walker,
                                   uint64_t last_flatness_check, double last_flatness);
 int orc_engine_set_run_state(orc_engine *engine, uint64_t epoch, uint64_t exchange_attempted,

@@ -159,6 +159,7 @@ EXPORTS = [
     "wlx_engine_resync",
     "wlx_engine_drift",
     "wlx_engine_read_sampling",
+    "wlx_engine_warm_start",
     "wlx_engine_set_kernel",
     "wlx_engine_kernel",
     "wlx_engine_autotune",

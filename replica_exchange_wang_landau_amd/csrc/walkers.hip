@@ -37,8 +37,11 @@ struct WalkerState {
     uint32_t active_count, refine_count;
     uint32_t begin, end;
     uint32_t trace_count, escaped;
+    // 1/t clock offset of a warm start (wlx_engine_warm_start): the 1/t factor is
+    // active / (attempted + clock_offset); 0 gives the reference's active / attempted
+    uint64_t clock_offset;
 };
-static_assert(sizeof(WalkerState) == 128, "walker state layout");
+static_assert(sizeof(WalkerState) == 136, "walker state layout");
 
 struct Globals {
     uint64_t epoch;
@@ -394,8 +397,8 @@ __device__ inline void next_iteration(const Args &a, WalkerState &st, Bins v, ui
         st.refine_count = 0;
     }
     if (a.inverse_time) {
-        const double inverse_time =
-            st.attempted == 0 ? 1.0 : double(st.active_count) / double(st.attempted);
+        const uint64_t clock = st.attempted + st.clock_offset;
+        const double inverse_time = clock == 0 ? 1.0 : double(st.active_count) / double(clock);
         if (st.factor <= inverse_time) {
             st.stage = WLX_STAGE_INVERSE_TIME;
             st.factor = inverse_time;
@@ -626,7 +629,7 @@ __global__ __launch_bounds__(kWave) void sample_kernel(Args a, uint64_t request,
         if (a.collect) record_displacement(col, old_rel, accepted ? delta : 0.0);
         if (st.stage == WLX_STAGE_INVERSE_TIME) { // update_inverse_time_factor, 309-315
             if (st.active_count != 0 && st.attempted != 0)
-                st.factor = double(st.active_count) / double(st.attempted);
+                st.factor = double(st.active_count) / double(st.attempted + st.clock_offset);
             if (st.factor <= a.final_factor) st.stage = WLX_STAGE_FROZEN;
         }
         update_current_bin(a, st, v, uint32_t(cur - st.begin));
@@ -760,7 +763,7 @@ __device__ inline void lds_walk(const Args &a, uint32_t w, uint64_t request, uin
         }
         if (st.stage == WLX_STAGE_INVERSE_TIME) { // update_inverse_time_factor, 309-315
             if (st.active_count != 0 && st.attempted != 0)
-                st.factor = double(st.active_count) / double(st.attempted);
+                st.factor = double(st.active_count) / double(st.attempted + st.clock_offset);
             if (st.factor <= a.final_factor) st.stage = WLX_STAGE_FROZEN;
         }
         update_current_bin(a, st, v, uint32_t(cur - st.begin));
@@ -940,7 +943,8 @@ __global__ __launch_bounds__(kWave) void lookahead_kernel(Args a, uint64_t reque
             double fac_lane = 0.0;
             if (st.stage == WLX_STAGE_INVERSE_TIME)
                 fac_lane = double(st.active_count) /
-                           double(st.attempted + 1 + uint64_t(lane) - uint64_t(used));
+                           double(st.attempted + st.clock_offset + 1 + uint64_t(lane) -
+                                  uint64_t(used));
             uint32_t m_hi = 0;
             double lr_lane = 0.0;
             uint64_t drew_mask = 0;
@@ -996,7 +1000,8 @@ __global__ __launch_bounds__(kWave) void lookahead_kernel(Args a, uint64_t reque
                         if (lane == 0) v.act[crel] = 1;
                         if (st.stage == WLX_STAGE_INVERSE_TIME)
                             fac_lane = double(st.active_count) /
-                                       double(st.attempted + uint64_t(lane) - uint64_t(L));
+                                       double(st.attempted + st.clock_offset + uint64_t(lane) -
+                                              uint64_t(L));
                     }
                     if (!cur_ref) {
                         ++st.refine_count;
@@ -1129,7 +1134,7 @@ __global__ __launch_bounds__(kWave) void lookahead_fast_kernel(Args a, uint64_t 
         st.escaped = 1; // wang_landau.cpp:247-248, decided up front as in the general kernel
     } else if (count > 0) {
         const uint32_t begin = st.begin;
-        const uint64_t att0 = st.attempted;
+        const uint64_t att0 = st.attempted, clk0 = att0 + st.clock_offset;
         uint32_t cur = uint32_t(start_bin);
         double lg_cur = v.lg[cur - begin];
         bool cur_seen = v.act[cur - begin] != 0;
@@ -1164,7 +1169,7 @@ __global__ __launch_bounds__(kWave) void lookahead_fast_kernel(Args a, uint64_t 
                     p_site, min(used + uint32_t(q), uint32_t(kWave - 1)));
                 jc[q] = a.J[size_t(sq) * a.np + p_site];
             }
-            double inc = batch_factors(stage, factor, active, att0 + done_total, used);
+            double inc = batch_factors(stage, factor, active, clk0 + done_total, used);
             uint32_t m_hi = 0, cur_lane = 0;
             double lr_lane = 0.0;
 #pragma unroll
@@ -1214,7 +1219,7 @@ __global__ __launch_bounds__(kWave) void lookahead_fast_kernel(Args a, uint64_t 
                     if (lane == 0) v.act[crel] = 1;
                     if (stage == WLX_STAGE_INVERSE_TIME) // later attempts see the new count
                         inc = lane > L ? batch_factors(stage, factor, active,
-                                                       att0 + done_total + b + 1, L + 1)
+                                                       clk0 + done_total + b + 1, L + 1)
                                        : inc;
                 }
                 lg_cur = lg_cur + (stage != WLX_STAGE_FROZEN ? f : 0.0);
@@ -1303,7 +1308,7 @@ __global__ __launch_bounds__(kWave) void lookahead_xo_kernel(Args a, uint64_t re
         st.escaped = 1; // wang_landau.cpp:247-248
     } else if (count > 0) {
         const uint32_t begin = st.begin;
-        const uint64_t att0 = st.attempted;
+        const uint64_t att0 = st.attempted, clk0 = att0 + st.clock_offset;
         uint32_t cur = uint32_t(start_bin);
         double lg_cur = v.lg[cur - begin];
         bool cur_seen = v.act[cur - begin] != 0;
@@ -1352,7 +1357,7 @@ __global__ __launch_bounds__(kWave) void lookahead_xo_kernel(Args a, uint64_t re
                 jc[q] = a.J[size_t(sq) * a.np + p_site];
             }
             // lane j: the log_g increment of the batch's j-th attempt
-            double inc = batch_factors(stage, factor, active, att0 + done_total, 0);
+            double inc = batch_factors(stage, factor, active, clk0 + done_total, 0);
             uint32_t m_hi = 0, cur_lane = 0, attempts = 0;
             double lr_lane = 0.0;
             uint64_t attempt_lanes = 0;
@@ -1408,7 +1413,7 @@ __global__ __launch_bounds__(kWave) void lookahead_xo_kernel(Args a, uint64_t re
                     if (stage == WLX_STAGE_INVERSE_TIME)
                         inc = lane > attempts
                                   ? batch_factors(stage, factor, active,
-                                                  att0 + done_total + attempts + 1, attempts + 1)
+                                                  clk0 + done_total + attempts + 1, attempts + 1)
                                   : inc;
                 }
                 lg_cur = lg_cur + (stage != WLX_STAGE_FROZEN ? f : 0.0);
@@ -1591,7 +1596,7 @@ __global__ __launch_bounds__(kWave) void lookahead_f32_kernel(Args a, uint64_t r
         st.escaped = 1; // wang_landau.cpp:247-248
     } else if (count > 0) {
         const uint32_t begin = st.begin;
-        const uint64_t att0 = st.attempted;
+        const uint64_t att0 = st.attempted, clk0 = att0 + st.clock_offset;
         uint32_t cur = uint32_t(start_bin);
         double lg_cur = v.lg[cur - begin];
         bool cur_seen = v.act[cur - begin] != 0;
@@ -1626,7 +1631,7 @@ __global__ __launch_bounds__(kWave) void lookahead_f32_kernel(Args a, uint64_t r
                     p_site, min(used + uint32_t(q), uint32_t(kWave - 1)));
                 jc[q] = a.J32[size_t(sq) * a.np + p_site];
             }
-            double inc = batch_factors(stage, factor, active, att0 + done_total, used);
+            double inc = batch_factors(stage, factor, active, clk0 + done_total, used);
             uint32_t m_hi = 0, cur_lane = 0;
             double lr_lane = 0.0;
 #pragma unroll
@@ -1679,7 +1684,7 @@ __global__ __launch_bounds__(kWave) void lookahead_f32_kernel(Args a, uint64_t r
                     if (lane == 0) v.act[crel] = 1;
                     if (stage == WLX_STAGE_INVERSE_TIME)
                         inc = lane > L ? batch_factors(stage, factor, active,
-                                                       att0 + done_total + b + 1, L + 1)
+                                                       clk0 + done_total + b + 1, L + 1)
                                        : inc;
                 }
                 lg_cur = lg_cur + (stage != WLX_STAGE_FROZEN ? f : 0.0);
@@ -1895,6 +1900,7 @@ __global__ __launch_bounds__(kWave) void init_kernel(Args a, const int8_t *initi
     st.active_count = st.refine_count = 0;
     st.trace_count = 0;
     st.escaped = 0;
+    st.clock_offset = 0;
     double h[NPL];
     compute_fields<NPL>(a, neg, h);
     double energy = total_energy<NPL>(a, neg, h, scratch);
@@ -2993,6 +2999,46 @@ int wlx_engine_resync(wlx_engine *e)
     return WLX_OK;
 }
 
+int wlx_engine_warm_start(wlx_engine *e, const double *log_g, double factor)
+{
+    if (int status = require_engine(e)) return status;
+    if (!log_g) return set_error(WLX_INVALID_ARGUMENT, "Null argument");
+    if (!(factor > 0.0) || !(factor <= 1.0))
+        return set_error(WLX_INVALID_ARGUMENT, "The initial ln f must lie in (0, 1]");
+    for (uint64_t b = 0; b < e->bins; ++b)
+        if (!std::isfinite(log_g[b]))
+            return set_error(WLX_INVALID_ARGUMENT, "The warm-start log g must be finite on every bin");
+    const size_t W = e->walkers, S = e->ws_stride;
+    std::vector<WalkerState> states(W);
+    REWL_HIP(hipMemcpyAsync(states.data(), e->d_state, W * sizeof(WalkerState),
+                            hipMemcpyDeviceToHost, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    std::vector<double> lg(W * S, 0.0);
+    for (size_t w = 0; w < W; ++w) {
+        WalkerState &st = states[w];
+        const uint32_t ws = st.end - st.begin;
+        double lowest = log_g[st.begin];
+        for (uint32_t b = 1; b < ws; ++b) lowest = std::min(lowest, log_g[st.begin + b]);
+        for (uint32_t b = 0; b < ws; ++b) lg[w * S + b] = log_g[st.begin + b] - lowest;
+        st.factor = factor;
+        st.stage = WLX_STAGE_WANG_LANDAU;
+        st.active_count = st.refine_count = 0;
+        st.last_check = st.attempted;
+        // as if 1/t had run for window_bins / factor attempts: once the stage switches, the
+        // factor continues from about `factor` instead of restarting at active / attempted
+        st.clock_offset = uint64_t(std::ceil(double(ws) / factor));
+    }
+    REWL_HIP(hipMemcpyAsync(e->d_logg, lg.data(), W * S * sizeof(double), hipMemcpyHostToDevice,
+                            e->stream));
+    REWL_HIP(hipMemsetAsync(e->d_hist, 0, W * S * sizeof(uint64_t), e->stream));
+    REWL_HIP(hipMemsetAsync(e->d_active, 0, W * S, e->stream));
+    REWL_HIP(hipMemsetAsync(e->d_refine, 0, W * S, e->stream));
+    REWL_HIP(hipMemcpyAsync(e->d_state, states.data(), W * sizeof(WalkerState),
+                            hipMemcpyHostToDevice, e->stream));
+    REWL_HIP(hipStreamSynchronize(e->stream));
+    return WLX_OK;
+}
+
 int wlx_engine_set_kernel(wlx_engine *e, uint64_t kernel)
 {
     if (int status = require_engine(e, false)) return status;
@@ -3385,6 +3431,7 @@ int wlx_engine_write_walker(wlx_engine *e, uint64_t w, const wlx_walker_info *in
     st.refine_count = refine_count;
     for (int k = 0; k < 4; ++k) st.rng[k] = info->rng_state[k];
     st.escaped = 0;
+    st.clock_offset = 0;
     std::vector<int8_t> sp(s->np, 1);
     std::vector<double> fl(s->np, 0.0);
     std::memcpy(sp.data(), spins, s->n);

@@ -351,6 +351,14 @@ class Engine:
         """FP64 recompute of every configuration's fields and energy (FP32 fields mode)."""
         self.backend.call("engine_resync", self.handle)
 
+    def warm_start(self, log_g, factor: float):
+        """Presets every walker's window log g and ln f = factor (wlx_engine_warm_start)."""
+        lg = np.ascontiguousarray(log_g, dtype=np.float64)
+        if lg.shape != (self.bins,):
+            raise InvalidArgument("The warm-start log g must span the grid")
+        self.backend.call("engine_warm_start", self.handle, _ptr(lg, ctypes.c_double),
+                          ctypes.c_double(factor))
+
     def set_kernel(self, kernel: int):
         """Forces a walker kernel family (abi.KERNEL_*); all give the same trajectory."""
         self.backend.call("engine_set_kernel", self.handle, ctypes.c_uint64(kernel))

@@ -207,23 +207,6 @@ def predicted_log_g(mlp: Mlp, engine: Engine, features) -> np.ndarray:
 
 
 def warm_start(engine: Engine, log_g, factor: float) -> None:
-    """Presets every walker of an initialized engine: log g from `log_g` (full grid, any
-    offset) and ln f = factor in the WL stage, with empty histograms. The walkers keep their
-    configurations, counters and streams (WangLandauWalker::restore's checks apply)."""
-    if not (0.0 < factor <= 1.0):
-        raise InvalidArgument("The initial ln f must lie in (0, 1]")
-    log_g = np.asarray(log_g, dtype=np.float64)
-    if log_g.shape != (engine.bins,) or not np.all(np.isfinite(log_g)):
-        raise InvalidArgument("The warm-start log g must be finite on every bin")
-    for k in range(engine.walkers):
-        snap = engine.read_walker(k)
-        b, e = int(snap.window_begin), int(snap.window_end)
-        window = log_g[b:e] - log_g[b:e].min()
-        snap.log_g = np.zeros(engine.bins)
-        snap.log_g[b:e] = window
-        snap.histogram = np.zeros(engine.bins, dtype=np.uint64)
-        active = np.zeros(engine.bins, dtype=np.uint8)
-        snap.active = active
-        snap.info.factor = factor
-        snap.info.stage = abi.STAGE_WANG_LANDAU
-        engine.write_walker(k, snap)
+    """Presets every walker of an initialized engine from `log_g` (full grid, any offset) with
+    ln f = factor (Engine.warm_start / wlx_engine_warm_start)."""
+    engine.warm_start(log_g, factor)

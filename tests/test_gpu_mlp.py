@@ -8,16 +8,16 @@ norms (1e-2) rather than bit for bit.
 """
 from __future__ import annotations
 
-import json
 import sys
 from pathlib import Path
 
 import numpy as np
 import pytest
 
-from harness import GOLDEN, c1_config, c1_geometry
+from harness import assert_walkers_equal, c1_geometry
 from replica_exchange_wang_landau_amd import abi
-from replica_exchange_wang_landau_amd.engine import Engine, InvalidArgument, System
+from replica_exchange_wang_landau_amd.engine import (Engine, InvalidArgument, RunConfig, System,
+                                                     WlParameters)
 from replica_exchange_wang_landau_amd.warmstart import Mlp, warm_start
 
 sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
@@ -45,7 +45,7 @@ def test_parameters_and_initialisation(gpu):
     w = mo.unpack(pa)
     assert not w["W0"][:, mo.FEATURES:].any() and not w["b1"].any()
     bound = np.sqrt(6.0 / mo.WIDTH)
-    assert np.abs(w["W1"]).max() <= bound and np.abs(w["W1"]).std() > 0.3 * bound
+    assert np.abs(w["W1"]).max() <= bound and w["W1"].std() > 0.5 * bound
     q = pa * 0.5
     a.set_params(q)
     assert np.array_equal(a.params(), q)
@@ -85,28 +85,64 @@ def test_training_reduces_the_loss(gpu):
     assert np.array_equal(again.params(), mlp.params())
 
 
-def test_warm_start_from_exact_dos_meets_the_same_bound(gpu):
-    """C1 plain WL warm-started from the exact log g at ln f = 1e-4 converges to ln f = 1e-6
-    within the flat start's bound (0.03 on max |dlog g|), in fewer attempts."""
+def c1_shifted(backend):
+    """C1 on the grid [-20.125, 19.875) (no level within 0.15 bin of an edge) with its exact
+    counts from all 2^8 configurations."""
+    system = System(*c1_geometry(), backend=backend)
+    every = np.array([[1 if (c >> i) & 1 else -1 for i in range(8)] for c in range(256)],
+                     dtype=np.int8)
+    _, energies = system.fields(every)
+    exact = np.bincount(np.floor((energies + 20.125) / 0.25).astype(int), minlength=160)
+    cfg = RunConfig(emin=-20.125, emax=19.875, bin_width=0.25, windows=1, walkers_per_window=2,
+                    seed=1234, exchange_interval_attempts=80, max_attempts=0, rng=abi.RNG_PHILOX,
+                    wl=WlParameters(final_factor=1e-6, check_interval_attempts=800))
+    e = Engine(system, cfg)
+    e.initialize()
+    return e, exact
+
+
+def max_dev(e, exact):
     from replica_exchange_wang_landau_amd import analysis
-    exact = json.loads((GOLDEN / "exact_c1.json").read_text())["counts"]
-    occupied = sorted(int(b) for b in exact)
-    results = {}
-    for mode in ("flat", "warm"):
-        e = Engine(System(*c1_geometry(), backend=gpu), c1_config(rng=abi.RNG_PHILOX))
-        e.initialize()
-        if mode == "warm":
-            log_g = np.zeros(e.bins)
-            for b in occupied:
-                log_g[b] = np.log(exact[str(b)])
-            warm_start(e, log_g, 1e-4)
-            assert e.read_walker(0).factor == 1e-4
-        st = e.run()
-        assert st.converged
-        f = analysis.summarize_window([e.read_walker(0), e.read_walker(1)], e.bins)
-        d = np.array([f.log_g[b] - np.log(exact[str(b)]) for b in occupied])
-        results[mode] = (st.attempted, float(np.max(np.abs(d - d.mean()))))
-    assert results["warm"][1] < 0.03 and results["flat"][1] < 0.03
-    assert results["warm"][0] < results["flat"][0], results
+    occupied = np.nonzero(exact)[0]
+    f = analysis.summarize_window([e.read_walker(k) for k in range(e.walkers)], e.bins,
+                                  allow_empty=True)
+    if not all(f.valid[b] for b in occupied):
+        return np.inf
+    d = np.array([f.log_g[b] - np.log(exact[b]) for b in occupied])
+    return float(np.max(np.abs(d - d.mean())))
+
+
+def test_warm_start_bit_exact_with_oracle(gpu, orc):
+    eg, exact = c1_shifted(gpu)
+    eo, _ = c1_shifted(orc)
+    log_g = np.where(exact > 0, np.log(np.maximum(exact, 1)), 0.0) * 0.9  # an imperfect start
+    for e in (eg, eo):
+        e.warm_start(log_g, 1e-3)
+        e.run(300)
+    for k in range(2):
+        assert_walkers_equal(eg, eo, k)
     with pytest.raises(InvalidArgument):
-        warm_start(e, np.zeros(e.bins), 0.0)
+        eg.warm_start(log_g, 0.0)
+    with pytest.raises(InvalidArgument):
+        eg.warm_start(np.full(eg.bins, np.nan), 0.5)
+
+
+def test_warm_start_reaches_accuracy_sooner_and_meets_the_bound(gpu):
+    """From the exact log g at ln f = 1e-4, C1 reaches max|dlog g| < 0.01 in fewer attempts
+    than from a flat start, and both converged runs (ln f = 1e-6) meet the 0.03 bound."""
+    reached = {}
+    for mode in ("flat", "warm"):
+        e, exact = c1_shifted(gpu)
+        if mode == "warm":
+            warm_start(e, np.where(exact > 0, np.log(np.maximum(exact, 1)), 0.0), 1e-4)
+        first = None
+        while True:
+            st = e.run(200)
+            if first is None and max_dev(e, exact) < 0.01:
+                first = st.attempted
+            if st.done:
+                break
+        assert st.converged and max_dev(e, exact) < 0.03, mode
+        reached[mode] = first
+    assert reached["warm"] is not None and reached["flat"] is not None
+    assert reached["warm"] < reached["flat"], reached
