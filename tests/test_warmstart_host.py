@@ -12,7 +12,7 @@ def test_slab_geometry():
     pos, axes, box, periodic = s.geometry()
     assert periodic and list(box) == [8.0, 8.0, 0.0]  # z open: box length 0
     assert 0.6 * 192 < len(pos) < 0.9 * 192
-    assert np.all((pos >= 0) & (pos[:, :2] < 8) & (pos[:, 2] < 3))
+    assert np.all(pos >= 0) and np.all(pos[:, :2] < 8) and np.all(pos[:, 2] < 3)
     assert np.allclose(np.linalg.norm(axes, axis=1), 1.0) and not axes[:, 2].any()
     pos2, axes2, _, _ = s.geometry()
     assert np.array_equal(pos, pos2) and np.array_equal(axes, axes2)
