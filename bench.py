@@ -372,8 +372,12 @@ def accuracy_time(engine, ref_log_g, ref_valid, budget_s: float, interval: int,
         try:
             lg, valid = analysis.engine_dos(engine)
         except (RuntimeError, ValueError):
+            print(f"accuracy: {device:.1f} s, windows not yet joined", file=sys.stderr, flush=True)
             continue
-        if not np.any((ref_valid != 0) & (valid == 0)):
+        if np.any((ref_valid != 0) & (valid == 0)):
+            print(f"accuracy: {device:.1f} s, reference bins not all found", file=sys.stderr,
+                  flush=True)
+        else:
             dev = analysis.max_deviation(lg, valid, ref_log_g, ref_valid)
             best = min(best, dev)
             print(f"accuracy: {device:.1f} s, max|dlog g| {dev:.4f}", file=sys.stderr, flush=True)
