@@ -393,7 +393,8 @@ def c5_leg(curves: int, steps: int, budget_s: float):
     in [0, 0.4], +z or random in-plane axes; seed 1234) run to ln f = 1e-3 together in one
     batch; the 4 x 128 BF16 network is trained on (E/N, N, density, dilution, Lz, axis mode)
     -> log g / N by the project's kernels (Adam, lr 1e-3, batch 1024). Evaluation: time to
-    max|dlog g| < 0.01 from a flat start and from a warm start (network log g, ln f = 1e-3) on
+    max|dlog g| < 0.01 from a flat start and from a warm start (network log g; ln f from the
+    network's validation error) on
     the held-out 12x12x4 slab at dilution 0.2 (reference: a run of this engine to ln f = 1e-5)
     and on C2 (reference: tests/golden/c2_reference_dos.json)."""
     import numpy as np
@@ -408,7 +409,9 @@ def c5_leg(curves: int, steps: int, budget_s: float):
         report=lambda d, n: print(f"c5: {d}/{n} training curves converged", file=sys.stderr,
                                   flush=True))
     keep = [(s, c) for s, c in zip(slabs, data) if (c[2] != 0).sum() > 4]
-    x, y = ws.training_set([s for s, _ in keep], [c for _, c in keep])
+    split = len(keep) * 9 // 10  # the last tenth of the curves validates the network
+    x, y = ws.training_set([s for s, _ in keep[:split]], [c for _, c in keep[:split]])
+    vx, vy = ws.training_set([s for s, _ in keep[split:]], [c for _, c in keep[split:]])
     curve_s = time.perf_counter() - t0
     mlp = ws.Mlp(1234)
     t1 = time.perf_counter()
@@ -418,6 +421,20 @@ def c5_leg(curves: int, steps: int, budget_s: float):
         mlp.train(x, y, steps=steps - steps // 2 - steps // 4, batch=1024, learning_rate=1e-4,
                   seed=7)])
     train_s = time.perf_counter() - t1
+    # per-spin RMS error on unseen curves, after removing each curve's mean offset
+    pred = mlp.predict(vx)
+    offsets, at = [], 0
+    for _, c in keep[split:]:
+        m = int((c[2] != 0).sum())
+        d = pred[at:at + m] - vy[at:at + m]
+        offsets.append(d - d.mean())
+        at += m
+    val_rms = float(np.sqrt(np.mean(np.concatenate(offsets) ** 2)))
+
+    def initial_factor(n):
+        """ln f of a warm start: the square of the expected worst log g error of the network
+        (3 x the validation RMS per spin x N), within [1e-4, 1]."""
+        return float(min(1.0, max(1e-4, (3.0 * val_rms * n) ** 2)))
     print(f"c5: {len(keep)} curves, {len(x)} samples in {curve_s:.0f} s; trained {steps} steps "
           f"in {train_s:.1f} s, loss {losses[0]:.3g} -> {losses[-1]:.3g}", file=sys.stderr,
           flush=True)
@@ -427,9 +444,12 @@ def c5_leg(curves: int, steps: int, budget_s: float):
         for mode in ("flat", "warm"):
             engine = make_engine()
             if mode == "warm":
-                engine.warm_start(ws.predicted_log_g(mlp, engine, features), 1e-3)
+                f0 = initial_factor(features[0])
+                engine.warm_start(ws.predicted_log_g(mlp, engine, features), f0)
             seconds, attempted, best = accuracy_time(engine, ref_lg, ref_valid, budget_s, interval)
             out[mode] = {"seconds": seconds, "attempted": int(attempted), "best_max_dev": best}
+            if mode == "warm":
+                out[mode]["initial_ln_f"] = f0
             print(f"c5: {mode} {out[mode]}", file=sys.stderr, flush=True)
         if out["flat"]["seconds"] and out["warm"]["seconds"]:
             out["speedup"] = out["flat"]["seconds"] / out["warm"]["seconds"]
@@ -473,7 +493,8 @@ def c5_leg(curves: int, steps: int, budget_s: float):
         c2_result = evaluate(c2_engine, c2_lg, c2_valid, (512.0, 1.0, 0.0, 8.0, 0.0), 4000)
     return {
         "workload": "C5: learned warm start (4 x 128 BF16 MLP, FP32 accumulate)",
-        "training": {"curves": len(keep), "samples": int(len(x)), "steps": steps,
+        "training": {"curves": split, "validation_curves": len(keep) - split,
+                     "validation_rms_per_spin": val_rms, "samples": int(len(x)), "steps": steps,
                      "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
                      "curve_seconds": curve_s, "train_seconds": train_s},
         "held_out": {"slab": "12x12x4, dilution 0.2, +z axes", "spins": int(hs.n),
@@ -519,7 +540,7 @@ def main():
                     help="only run the C4 FP32-field precision study (resync kernel, drift)")
     args = ap.parse_args()
     if args.c5:
-        print(json.dumps(c5_leg(args.c5_curves, 20_000, 120.0)))
+        print(json.dumps(c5_leg(args.c5_curves, 60_000, 120.0)))
         return
     if args.c3:
         print(json.dumps(c3_leg(args.steps if args.steps != 3000 else 200, 20, args.graph)))
