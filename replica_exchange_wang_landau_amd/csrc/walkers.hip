@@ -668,9 +668,30 @@ __global__ __launch_bounds__(kWave) void sample_kernel(Args a, uint64_t request,
 // dense system it streams the row, h[j] += m J[s][j] (physics.cpp:190-195). fma(m, J, h) is the
 // reference's rounding because m J is exact. LDS: bins | fields (np doubles) | spins (np bytes).
 // ---------------------------------------------------------------------------------------------
-__host__ __device__ inline size_t lds_walker_bytes(uint32_t ws_stride, uint32_t np)
+// fields and spins of n sites, rounded up to 16 sites
+__host__ __device__ inline uint32_t lds_sites(uint32_t n) { return (n + 15u) & ~15u; }
+
+__host__ __device__ inline size_t lds_walker_bytes(uint32_t ws_stride, uint32_t n)
 {
-    return bins_lds_bytes(ws_stride) + size_t(np) * 9;
+    return bins_lds_bytes(ws_stride) + size_t(lds_sites(n)) * 9;
+}
+
+// up to 64 (column, value) entries of a CSR row, one per lane, loaded ahead of their use
+struct RowAhead {
+    uint64_t begin, end;
+    uint32_t column;
+    double value;
+};
+
+__device__ inline RowAhead load_row_ahead(const Args &a, uint32_t site)
+{
+    RowAhead r;
+    r.begin = a.csr_offsets[site];
+    r.end = a.csr_offsets[site + 1];
+    const uint64_t k = r.begin + lane_id();
+    r.column = k < r.end ? a.csr_columns[k] : 0u;
+    r.value = k < r.end ? a.csr_values[k] : 0.0;
+    return r;
 }
 
 template <bool PHILOX>
@@ -684,7 +705,7 @@ __device__ inline void lds_walk(const Args &a, uint32_t w, uint64_t request, uin
     const uint32_t ws = st.end - st.begin;
     Bins v = bins_view(smem, a.ws_stride);
     double *hf = reinterpret_cast<double *>(smem + bins_lds_bytes(a.ws_stride));
-    int8_t *sp = reinterpret_cast<int8_t *>(hf + a.np);
+    int8_t *sp = reinterpret_cast<int8_t *>(hf + lds_sites(a.n));
     load_bins(a, w, ws, v);
     const size_t base = size_t(st.slot) * a.np;
     for (uint32_t i = lane; i < a.n; i += kWave) {
@@ -698,14 +719,34 @@ __device__ inline void lds_walk(const Args &a, uint32_t w, uint64_t request, uin
     Xoshiro rng{st.rng[0], st.rng[1], st.rng[2], st.rng[3]};
     const uint64_t trace_base = uint64_t(w) * a.trace_cap;
     int64_t cur = grid_index(a.grid, energy);
+    // Philox: the proposal of the next attempt is known, so its CSR row is loaded one attempt
+    // ahead, behind the current decision
+    const bool ahead = PHILOX && a.csr_offsets != nullptr;
     Philox4 block{};
+    uint32_t next_site = 0;
+    RowAhead row{};
+    if (PHILOX && count > 0) {
+        block = philox4x64(st.attempted >> 1, kPhiloxWalkTag, 0, 0, a.seed, uint64_t(w));
+        next_site = uint32_t(
+            __umul64hi((st.attempted & 1) ? block.v[2] : block.v[0], uint64_t(a.n)));
+        if (ahead) row = load_row_ahead(a, next_site);
+    }
     for (uint64_t t = 0; t < count; ++t) {
         uint64_t site;
+        double u_philox = 0.0;
+        RowAhead this_row{};
         if (PHILOX) {
-            if (t == 0 || (st.attempted & 1) == 0)
-                block = philox4x64(st.attempted >> 1, kPhiloxWalkTag, 0, 0, a.seed,
-                                   uint64_t(w));
-            site = __umul64hi((st.attempted & 1) ? block.v[2] : block.v[0], uint64_t(a.n));
+            site = next_site;
+            u_philox = double(((st.attempted & 1) ? block.v[3] : block.v[1]) >> 11) * 0x1.0p-53;
+            this_row = row;
+            if (t + 1 < count) { // the next attempt's proposal and, on CSR systems, its row
+                const uint64_t na = st.attempted + 1;
+                if ((na & 1) == 0)
+                    block = philox4x64(na >> 1, kPhiloxWalkTag, 0, 0, a.seed, uint64_t(w));
+                next_site = uint32_t(__umul64hi((na & 1) ? block.v[2] : block.v[0],
+                                                uint64_t(a.n)));
+                if (ahead) row = load_row_ahead(a, next_site);
+            }
         } else {
             site = draw_bounded(rng, a.bounded);
         }
@@ -730,10 +771,7 @@ __device__ inline void lds_walk(const Args &a, uint32_t w, uint64_t request, uin
             else if (force_due)
                 accepted = forced = true;
             else {
-                const double u =
-                    PHILOX ? double(((st.attempted & 1) ? block.v[3] : block.v[1]) >> 11) *
-                                 0x1.0p-53
-                           : rng.uniform();
+                const double u = PHILOX ? u_philox : rng.uniform();
                 drew = true;
                 const double lu = log(fmax(u, 0x1.0p-53));
                 accepted = lu < log_ratio;
@@ -743,15 +781,22 @@ __device__ inline void lds_walk(const Args &a, uint32_t w, uint64_t request, uin
         ++st.attempted;
         if (accepted) {
             const double m = -2.0 * sigma;
-            if (a.csr_offsets) {
+            if (ahead) {
+                if (this_row.begin + lane < this_row.end)
+                    hf[this_row.column] = fma(m, this_row.value, hf[this_row.column]);
+                for (uint64_t k = this_row.begin + kWave + lane; k < this_row.end; k += kWave) {
+                    const uint32_t c = a.csr_columns[k];
+                    hf[c] = fma(m, a.csr_values[k], hf[c]);
+                }
+            } else if (a.csr_offsets) {
                 const uint64_t row_end = a.csr_offsets[s + 1];
                 for (uint64_t k = a.csr_offsets[s] + lane; k < row_end; k += kWave) {
                     const uint32_t c = a.csr_columns[k];
                     hf[c] = fma(m, a.csr_values[k], hf[c]);
                 }
             } else {
-                const double *row = a.J + size_t(s) * a.np;
-                for (uint32_t j = lane; j < a.n; j += kWave) hf[j] = fma(m, row[j], hf[j]);
+                const double *jrow = a.J + size_t(s) * a.np;
+                for (uint32_t j = lane; j < a.n; j += kWave) hf[j] = fma(m, jrow[j], hf[j]);
             }
             if (lane == 0) sp[s] = sigma < 0.0 ? int8_t(1) : int8_t(-1);
             __syncthreads(); // the scattered fields before the next attempt reads one
@@ -2766,7 +2811,7 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
     if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.init), e->lds_init))
         return status;
     { // the LDS walker stages the configuration in LDS beside the bins
-        e->lds_walker = lds_walker_bytes(e->ws_stride, uint32_t(sys->np));
+        e->lds_walker = lds_walker_bytes(e->ws_stride, uint32_t(sys->n));
         if (e->lds_walker <= 160 * 1024)
             for (SampleFn fn : {e->kernels.lds_xo, e->kernels.lds_ph})
                 if (int status = set_lds_limit(reinterpret_cast<const void *>(fn), e->lds_walker))
