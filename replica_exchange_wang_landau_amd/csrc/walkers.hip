@@ -719,32 +719,41 @@ __device__ inline void lds_walk(const Args &a, uint32_t w, uint64_t request, uin
     Xoshiro rng{st.rng[0], st.rng[1], st.rng[2], st.rng[3]};
     const uint64_t trace_base = uint64_t(w) * a.trace_cap;
     int64_t cur = grid_index(a.grid, energy);
-    // Philox: the proposal of the next attempt is known, so its CSR row is loaded one attempt
-    // ahead, behind the current decision
+    // Philox: lane j draws the proposal site and log(u) of attempt first + j for 64 attempts
+    // at a time, so the proposal of the next attempt is known and its CSR row is loaded one
+    // attempt ahead, behind the current decision
     const bool ahead = PHILOX && a.csr_offsets != nullptr;
-    Philox4 block{};
+    uint64_t first = st.attempted; // attempt of lane 0 in p_site / p_lu
+    uint32_t p_site = 0;
+    double p_lu = 0.0;
+    auto draw_block = [&](uint64_t from) {
+        const uint64_t t = from + lane;
+        const Philox4 blk = philox4x64(t >> 1, kPhiloxWalkTag, 0, 0, a.seed, uint64_t(w));
+        const bool odd = t & 1;
+        p_site = uint32_t(__umul64hi(odd ? blk.v[2] : blk.v[0], uint64_t(a.n)));
+        const double u = double((odd ? blk.v[3] : blk.v[1]) >> 11) * 0x1.0p-53;
+        p_lu = log(fmax(u, 0x1.0p-53));
+        first = from;
+    };
     uint32_t next_site = 0;
     RowAhead row{};
     if (PHILOX && count > 0) {
-        block = philox4x64(st.attempted >> 1, kPhiloxWalkTag, 0, 0, a.seed, uint64_t(w));
-        next_site = uint32_t(
-            __umul64hi((st.attempted & 1) ? block.v[2] : block.v[0], uint64_t(a.n)));
+        draw_block(st.attempted);
+        next_site = __builtin_amdgcn_readlane(p_site, 0);
         if (ahead) row = load_row_ahead(a, next_site);
     }
     for (uint64_t t = 0; t < count; ++t) {
         uint64_t site;
-        double u_philox = 0.0;
+        double lu_philox = 0.0;
         RowAhead this_row{};
         if (PHILOX) {
             site = next_site;
-            u_philox = double(((st.attempted & 1) ? block.v[3] : block.v[1]) >> 11) * 0x1.0p-53;
+            lu_philox = readlane_f64(p_lu, uint32_t(st.attempted - first));
             this_row = row;
             if (t + 1 < count) { // the next attempt's proposal and, on CSR systems, its row
                 const uint64_t na = st.attempted + 1;
-                if ((na & 1) == 0)
-                    block = philox4x64(na >> 1, kPhiloxWalkTag, 0, 0, a.seed, uint64_t(w));
-                next_site = uint32_t(__umul64hi((na & 1) ? block.v[2] : block.v[0],
-                                                uint64_t(a.n)));
+                if (na - first == kWave) draw_block(na);
+                next_site = __builtin_amdgcn_readlane(p_site, uint32_t(na - first));
                 if (ahead) row = load_row_ahead(a, next_site);
             }
         } else {
@@ -771,9 +780,8 @@ __device__ inline void lds_walk(const Args &a, uint32_t w, uint64_t request, uin
             else if (force_due)
                 accepted = forced = true;
             else {
-                const double u = PHILOX ? u_philox : rng.uniform();
                 drew = true;
-                const double lu = log(fmax(u, 0x1.0p-53));
+                const double lu = PHILOX ? lu_philox : log(fmax(rng.uniform(), 0x1.0p-53));
                 accepted = lu < log_ratio;
                 if (near_threshold(lu, log_ratio)) ++st.ambiguous;
             }
