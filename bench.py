@@ -23,6 +23,9 @@ on the same workload with OpenMP on the host cores, bounded to a fixed attempt b
 time_to_accuracy runs a fresh C2 engine and reports the device time until its stitched DOS
 is within max|dlog g| < 0.01 of a converged C2 reference DOS, after mean-offset alignment
 (tests/golden/c2_reference_dos.json, made by tests/golden/make_c2_reference_dos.py).
+time_to_accuracy.replicas does the same for 8 independent C2 runs (seeds 1234..1241) batched
+into one lookahead-walker launch per epoch, whose stitched DOS are averaged: the 128-walker
+workload fills an eighth of the chip's wave slots, the batch fills them with replicas.
 
 `--c4` runs only the C4 precision study instead: a 16x16x16 dense system with FP32 Kahan
 fields, the FP64 matrix-core resync every 10 MCS, and the drift it measures.
@@ -67,11 +70,11 @@ def traffic_bytes(kernel: str):
     return None
 
 
-def c2_config(max_attempts=0, rng=0):
+def c2_config(max_attempts=0, rng=0, seed=1234):
     from replica_exchange_wang_landau_amd import RunConfig, WlParameters
 
     return RunConfig(emin=C2_GRID[0], emax=C2_GRID[1], bin_width=C2_GRID[2], windows=8,
-                     walkers_per_window=16, overlap=0.75, seed=1234,
+                     walkers_per_window=16, overlap=0.75, seed=seed,
                      exchange_interval_attempts=512, max_attempts=max_attempts, rng=rng,
                      wl=WlParameters(final_factor=1e-7, check_interval_attempts=512 * 100))
 
@@ -353,6 +356,60 @@ def c3_leg(epochs: int, warmup: int, graph: int, pilot_mcs: int = 10_000):
         "setup_seconds": {"pilot": pilot_s, "total": setup_s},
     }
 
+def tta_replicas_leg(budget_seconds: float, interval_epochs: int, replicas: int,
+                     graph: int = 16):
+    """Time to accuracy of `replicas` independent C2 runs (Philox, seeds 1234 + r) run as one
+    batch (one lookahead-walker launch over all their walkers per epoch, HIP-graph replays of
+    `graph` epochs): the clock (Batch.run only) stops at the first snapshot where the average
+    of their stitched DOS (analysis.average_dos) holds every reference bin and lies within
+    max|dlog g| < 0.01 of the converged reference."""
+    from replica_exchange_wang_landau_amd import Batch, Engine, System, analysis
+
+    if not REFERENCE_DOS.exists() or replicas < 1:
+        return None
+    import numpy as np
+
+    ref = json.loads(REFERENCE_DOS.read_text())
+    ref_lg = np.array([float.fromhex(v) for v in ref["log_g"]])
+    ref_valid = np.array(ref["valid"], dtype=np.uint8)
+    engines = []
+    for r in range(replicas):
+        e = Engine(System.simple_cubic(8, 8, 8, 1.0, (0.0, 0.0, 1.0), True),
+                   c2_config(rng=1, seed=1234 + r))
+        e.initialize()
+        engines.append(e)
+    batch = Batch(engines)
+    batch.set_graph(graph)
+    kernel = batch.kernel()
+    device, reached, best, snaps = 0.0, None, float("inf"), 0
+    while device < budget_seconds:
+        t0 = time.perf_counter()
+        batch.run(interval_epochs)
+        device += time.perf_counter() - t0
+        try:
+            lg, valid = analysis.average_dos([analysis.engine_dos(e) for e in engines])
+        except (RuntimeError, ValueError):
+            continue
+        snaps += 1
+        if np.any((ref_valid != 0) & (valid == 0)):
+            print(f"tta replicas snapshot {snaps}: {device:.1f} s, reference bins not all "
+                  "found yet", file=sys.stderr, flush=True)
+            continue
+        dev = analysis.max_deviation(lg, valid, ref_lg, ref_valid)
+        best = min(best, dev)
+        print(f"tta replicas snapshot {snaps}: {device:.1f} s, max|dlog g| {dev:.4f}",
+              file=sys.stderr, flush=True)
+        if dev < 0.01:
+            reached = device
+            break
+    attempted = sum(e.status().attempted for e in engines)
+    batch.close()
+    return {"replicas": replicas, "kernel": kernel, "threshold": 0.01, "seconds": reached,
+            "attempted": int(attempted), "rate": attempted / device if device else None,
+            "best_max_dev": best if best != float("inf") else None, "snapshots": snaps,
+            "interval_epochs": interval_epochs, "graph_epochs": graph,
+            "budget_seconds": budget_seconds}
+
 
 def accuracy_time(engine, ref_log_g, ref_valid, budget_s: float, interval: int,
                   threshold: float = 0.01):
@@ -528,6 +585,11 @@ def main():
     ap.add_argument("--tta-seconds", type=float, default=150.0,
                     help="device-time budget of the time-to-accuracy leg (0 skips it)")
     ap.add_argument("--tta-interval", type=int, default=4000, help="epochs between snapshots")
+    ap.add_argument("--tta-replicas", type=int, default=8,
+                    help="independent C2 replicas batched for the averaged time to accuracy "
+                         "(0 skips it)")
+    ap.add_argument("--tta-replicas-seconds", type=float, default=60.0,
+                    help="device-time budget of the replica leg")
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
                     help="only run a plain copy of MIB MiB (counter calibration)")
     ap.add_argument("--c5", action="store_true",
@@ -569,6 +631,10 @@ def main():
     if tta and cpu and tta["seconds"] is not None:
         # the reference CPU path at its measured rate over the same number of attempts
         tta["cpu_estimate_seconds"] = tta["attempted"] / cpu["value"]
+    if tta is not None and args.tta_replicas > 0:
+        tta["replicas"] = tta_replicas_leg(args.tta_replicas_seconds, args.tta_interval,
+                                           args.tta_replicas)
+        print("bench: replica leg done", file=sys.stderr, flush=True)
     line = {
         "metric": "attempted flips/s (all walkers)",
         "value": g["rate"],

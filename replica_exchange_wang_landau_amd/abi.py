@@ -167,6 +167,7 @@ EXPORTS = [
     "wlx_batch_destroy",
     "wlx_batch_set_graph",
     "wlx_batch_run",
+    "wlx_batch_kernel",
     "wlx_mlp_params",
     "wlx_mlp_create",
     "wlx_mlp_destroy",

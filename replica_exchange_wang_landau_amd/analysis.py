@@ -177,3 +177,28 @@ def max_deviation(log_g, valid, ref_log_g, ref_valid) -> float:
         return math.inf
     d = log_g[both] - ref_log_g[both]
     return float(np.max(np.abs(d - d.mean())))
+
+
+def average_dos(curves) -> tuple[np.ndarray, np.ndarray]:
+    """Mean of several stitched log g curves of one system (independent replicas): each curve
+    is shifted to zero mean over the bins valid in every curve, the shifted curves are
+    averaged there, and the result is returned with its peak at 0 and that common mask. No
+    reference counterpart: the reference runs one REWL job per system (parallel.cpp:456-676)."""
+    curves = [(np.asarray(lg, dtype=np.float64), np.asarray(v)) for lg, v in curves]
+    if not curves:
+        raise ValueError("No curves to average")
+    bins = len(curves[0][0])
+    if any(len(lg) != bins or len(v) != bins for lg, v in curves):
+        raise ValueError("Curves of different grids")
+    common = np.ones(bins, dtype=bool)
+    for lg, v in curves:
+        common &= (v != 0) & np.isfinite(lg)
+    if not common.any():
+        raise RuntimeError("Replica curves share no valid bin")
+    total = np.zeros(int(common.sum()))
+    for lg, _ in curves:
+        total += lg[common] - lg[common].mean()
+    out = np.full(bins, -np.inf)
+    out[common] = total / len(curves)
+    out[common] -= out[common].max()
+    return out, common.astype(np.uint8)

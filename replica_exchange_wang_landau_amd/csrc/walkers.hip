@@ -22,6 +22,9 @@
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
 #include <memory>
 
 namespace rewl {
@@ -107,6 +110,12 @@ struct Args {
     uint64_t exchange_interval, max_attempts, check_interval, force_after, min_visits;
     double final_factor, flatness;
     uint32_t inverse_time, nalivaiko, philox, unlimited;
+};
+
+// one wave of a sampling launch: the engine (index into the launch's Args table) and walker.
+// An engine alone launches with the identity list {0, w}; a batch with its sorted list.
+struct BatchSlot {
+    uint32_t engine, walker;
 };
 
 enum : uint32_t { kGate = 1, kCheck = 2, kTrace = 4 };
@@ -548,10 +557,13 @@ __device__ inline void finish_walker(const Args &a, uint32_t w, WalkerState &st,
 // with kCheck the flatness cadence of parallel.cpp:395-407 follows the batch.
 // ---------------------------------------------------------------------------------------------
 template <int NPL, bool PHILOX>
-__global__ __launch_bounds__(kWave) void sample_kernel(Args a, uint64_t request, uint32_t flags)
+__global__ __launch_bounds__(kWave) void sample_kernel(const Args *table, const BatchSlot *slots, uint64_t request,
+                   uint32_t flags)
 {
+    const BatchSlot slot = slots[blockIdx.x];
+    const Args a = table[slot.engine];
     extern __shared__ __align__(16) unsigned char smem[];
-    const uint32_t w = blockIdx.x, lane = lane_id();
+    const uint32_t w = slot.walker, lane = lane_id();
     if ((flags & kGate) && a.glob->done) return;
     WalkerState st = a.state[w];
     if (st.escaped) return;
@@ -846,11 +858,13 @@ __device__ inline void lds_walk(const Args &a, uint32_t w, uint64_t request, uin
 }
 
 template <bool PHILOX>
-__global__ __launch_bounds__(kWave) void lds_sample_kernel(Args a, uint64_t request,
-                                                           uint32_t flags)
+__global__ __launch_bounds__(kWave) void lds_sample_kernel(const Args *table, const BatchSlot *slots, uint64_t request,
+                   uint32_t flags)
 {
+    const BatchSlot slot = slots[blockIdx.x];
+    const Args a = table[slot.engine];
     extern __shared__ __align__(16) unsigned char smem[];
-    lds_walk<PHILOX>(a, blockIdx.x, request, flags, smem);
+    lds_walk<PHILOX>(a, slot.walker, request, flags, smem);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -933,11 +947,14 @@ __device__ inline void apply_rows(const Args &a, uint32_t p_site, uint32_t m_hi,
 constexpr int kLookaheadBatch = 16;
 
 template <int NPL>
-__global__ __launch_bounds__(kWave) void lookahead_kernel(Args a, uint64_t request, uint32_t flags)
+__global__ __launch_bounds__(kWave) void lookahead_kernel(const Args *table, const BatchSlot *slots, uint64_t request,
+                   uint32_t flags)
 {
+    const BatchSlot slot = slots[blockIdx.x];
+    const Args a = table[slot.engine];
     constexpr int KB = kLookaheadBatch;
     extern __shared__ __align__(16) unsigned char smem[];
-    const uint32_t w = blockIdx.x, lane = lane_id();
+    const uint32_t w = slot.walker, lane = lane_id();
     if ((flags & kGate) && a.glob->done) return;
     WalkerState st = a.state[w];
     if (st.escaped) return;
@@ -1162,12 +1179,12 @@ __device__ inline double batch_factors(uint32_t stage, double factor, uint32_t a
 }
 
 template <int NPL>
-__global__ __launch_bounds__(kWave) void lookahead_fast_kernel(Args a, uint64_t request,
-                                                               uint32_t flags)
+__device__ __attribute__((always_inline)) inline void fast_walk(const Args &a, uint32_t w,
+                                                                uint64_t request, uint32_t flags,
+                                                                unsigned char *smem)
 {
     constexpr int KB = kLookaheadBatch;
-    extern __shared__ __align__(16) unsigned char smem[];
-    const uint32_t w = blockIdx.x, lane = lane_id();
+    const uint32_t lane = lane_id();
     if ((flags & kGate) && a.glob->done) return;
     WalkerState st = a.state[w];
     if (st.escaped) return;
@@ -1317,6 +1334,16 @@ __global__ __launch_bounds__(kWave) void lookahead_fast_kernel(Args a, uint64_t 
     finish_walker<NPL>(a, w, st, v, ws, neg, h, energy, flags);
 }
 
+template <int NPL>
+__global__ __launch_bounds__(kWave) void lookahead_fast_kernel(const Args *table, const BatchSlot *slots, uint64_t request,
+                   uint32_t flags)
+{
+    const BatchSlot slot = slots[blockIdx.x];
+    const Args a = table[slot.engine];
+    extern __shared__ __align__(16) unsigned char smem[];
+    fast_walk<NPL>(a, slot.walker, request, flags, smem);
+}
+
 
 // ---------------------------------------------------------------------------------------------
 // lookahead sampling on the reference's Xoshiro256** stream (common configuration: no trace,
@@ -1335,12 +1362,14 @@ __device__ inline void xoshiro_advance(Xoshiro &r, uint32_t steps)
 }
 
 template <int NPL>
-__global__ __launch_bounds__(kWave) void lookahead_xo_kernel(Args a, uint64_t request,
-                                                             uint32_t flags)
+__global__ __launch_bounds__(kWave) void lookahead_xo_kernel(const Args *table, const BatchSlot *slots, uint64_t request,
+                   uint32_t flags)
 {
+    const BatchSlot slot = slots[blockIdx.x];
+    const Args a = table[slot.engine];
     constexpr int KB = kLookaheadBatch;
     extern __shared__ __align__(16) unsigned char smem[];
-    const uint32_t w = blockIdx.x, lane = lane_id();
+    const uint32_t w = slot.walker, lane = lane_id();
     if ((flags & kGate) && a.glob->done) return;
     WalkerState st = a.state[w];
     if (st.escaped) return;
@@ -1624,12 +1653,14 @@ __device__ inline void rows_kahan(const Args &a, uint32_t p_site, uint32_t m_hi,
 }
 
 template <int NPL>
-__global__ __launch_bounds__(kWave) void lookahead_f32_kernel(Args a, uint64_t request,
-                                                              uint32_t flags)
+__global__ __launch_bounds__(kWave) void lookahead_f32_kernel(const Args *table, const BatchSlot *slots, uint64_t request,
+                   uint32_t flags)
 {
+    const BatchSlot slot = slots[blockIdx.x];
+    const Args a = table[slot.engine];
     constexpr int KB = kLookaheadBatch;
     extern __shared__ __align__(16) unsigned char smem[];
-    const uint32_t w = blockIdx.x, lane = lane_id();
+    const uint32_t w = slot.walker, lane = lane_id();
     if ((flags & kGate) && a.glob->done) return;
     WalkerState st = a.state[w];
     if (st.escaped) return;
@@ -2280,25 +2311,10 @@ __global__ __launch_bounds__(kExchangeBlock) void exchange_kernel(Args a, uint32
 }
 
 // ---------------------------------------------------------------------------------------------
-// batches (wlx_batch): the walkers of several engines in one sampling launch (one wave per
-// walker, the LDS walker, engine parameters from a device table) and their replica exchanges
-// in one launch (one workgroup per engine). Each engine keeps its own gate, epoch and counters.
+// batches (wlx_batch): the walkers of several engines in one sampling launch (the sampling
+// kernels above with the batch's Args table and slot list) and their replica exchanges in one
+// launch (one workgroup per engine). Each engine keeps its own gate, epoch and counters.
 // ---------------------------------------------------------------------------------------------
-struct BatchSlot {
-    uint32_t engine, walker;
-};
-
-template <bool PHILOX>
-__global__ __launch_bounds__(kWave) void batch_sample_kernel(const Args *table,
-                                                             const BatchSlot *slots,
-                                                             uint32_t flags)
-{
-    extern __shared__ __align__(16) unsigned char smem[];
-    const BatchSlot b = slots[blockIdx.x];
-    const Args a = table[b.engine];
-    lds_walk<PHILOX>(a, b.walker, 0, flags, smem);
-}
-
 __global__ __launch_bounds__(kExchangeBlock) void batch_exchange_kernel(const Args *table)
 {
     const Args a = table[blockIdx.x];
@@ -2333,7 +2349,7 @@ __global__ __launch_bounds__(kWave) void walker_op_kernel(Args a, uint32_t w, ui
 }
 
 // ---- dispatch ------------------------------------------------------------------------------
-using SampleFn = void (*)(Args, uint64_t, uint32_t);
+using SampleFn = void (*)(const Args *, const BatchSlot *, uint64_t, uint32_t);
 using InitFn = void (*)(Args, const int8_t *, const uint8_t *, InitParams, InitReport *);
 using FieldsFn = void (*)(Args, const int8_t *, double *, double *);
 using PilotFn = void (*)(Args, uint64_t, uint64_t, double *);
@@ -2368,10 +2384,18 @@ KernelSet kernels_for(uint32_t npl)
     }
 }
 
+// raises a kernel's dynamic-LDS limit to `bytes`; never lowers it, since the kernels are
+// shared by every engine and batch of the process
 int set_lds_limit(const void *fn, size_t bytes)
 {
-    if (bytes > 64 * 1024)
-        REWL_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+    if (bytes <= 64 * 1024) return WLX_OK;
+    static std::mutex guard;
+    static std::map<const void *, size_t> limits;
+    std::lock_guard<std::mutex> lock(guard);
+    size_t &limit = limits[fn];
+    if (bytes <= limit) return WLX_OK;
+    REWL_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+    limit = bytes;
     return WLX_OK;
 }
 
@@ -2446,6 +2470,10 @@ struct wlx_engine {
     uint8_t *d_tflag = nullptr;
     uint64_t trace_cap = 0;
     Globals *d_glob = nullptr;
+    Args *d_args = nullptr;       // the sampling kernels' parameters (a table of one)
+    BatchSlot *d_slots = nullptr; // {0, w} for every walker w
+    Args uploaded{};
+    bool args_uploaded = false;
     InitReport *d_report = nullptr;
     uint64_t *d_op = nullptr;
     hipStream_t stream = nullptr;
@@ -2475,7 +2503,8 @@ struct wlx_engine {
 
     Args args(uint32_t flags_trace = 0) const
     {
-        Args a{};
+        Args a;
+        std::memset(&a, 0, sizeof a); // padding too: upload_args compares bytes
         a.J = sys->d_J64;
         a.J32 = sys->d_J32;
         a.fields32 = d_fields32;
@@ -2526,6 +2555,18 @@ struct wlx_engine {
         return a;
     }
 
+    // the sampling kernels read Args from d_args, refreshed when the parameters change
+    int upload_args()
+    {
+        const Args a = args();
+        if (args_uploaded && std::memcmp(&a, &uploaded, sizeof a) == 0) return WLX_OK;
+        REWL_HIP(hipStreamSynchronize(stream)); // the previous copy has left `uploaded`
+        std::memcpy(&uploaded, &a, sizeof a);
+        REWL_HIP(hipMemcpyAsync(d_args, &uploaded, sizeof uploaded, hipMemcpyHostToDevice, stream));
+        args_uploaded = true;
+        return WLX_OK;
+    }
+
     SampleFn sample_fn() const
     {
         if (f32) return kernels.sample_f32;
@@ -2548,8 +2589,9 @@ struct wlx_engine {
         if (trace_cap) flags |= kTrace;
         const SampleFn fn = sample_fn();
         const bool lds_fn = fn == kernels.lds_xo || fn == kernels.lds_ph;
-        hipLaunchKernelGGL(fn, dim3(walkers), dim3(kWave), lds_fn ? lds_walker : lds_sample, stream, args(),
-                           request, flags);
+        if (int status = upload_args()) return status;
+        hipLaunchKernelGGL(fn, dim3(walkers), dim3(kWave), lds_fn ? lds_walker : lds_sample, stream,
+                           d_args, d_slots, request, flags);
         REWL_HIP(hipGetLastError());
         return WLX_OK;
     }
@@ -2840,6 +2882,15 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
     if (int st = device_alloc(&e->d_active, W * S)) return st;
     if (int st = device_alloc(&e->d_refine, W * S)) return st;
     if (int st = device_alloc(&e->d_glob, 1)) return st;
+    if (int st = device_alloc(&e->d_args, 1)) return st;
+    if (int st = device_alloc(&e->d_slots, e->walkers)) return st;
+    {
+        std::vector<BatchSlot> identity(e->walkers);
+        for (uint32_t w = 0; w < e->walkers; ++w) identity[w] = BatchSlot{0, w};
+        REWL_HIP(hipMemcpyAsync(e->d_slots, identity.data(), identity.size() * sizeof(BatchSlot),
+                                hipMemcpyHostToDevice, e->stream));
+        REWL_HIP(hipStreamSynchronize(e->stream));
+    }
     if (int st = device_alloc(&e->d_report, W)) return st;
     if (int st = device_alloc(&e->d_op, 8)) return st;
     if (c->fields == WLX_FIELDS_FP32_KAHAN) {
@@ -2889,7 +2940,7 @@ int wlx_engine_destroy(wlx_engine *e)
                        e->d_hist,  e->d_active, e->d_refine, e->d_tsite,  e->d_tbin,
                        e->d_tflag, e->d_glob,   e->d_report, e->d_op,     e->d_fields32,
                        e->d_S,     e->d_H,      e->d_drift,  e->d_sq,     e->d_cnt,
-                       e->d_sampling, e->d_rep};
+                       e->d_sampling, e->d_rep, e->d_args, e->d_slots};
     for (void *p : buffers)
         if (p) (void)hipFree(p);
     for (hipEvent_t ev : e->events) (void)hipEventDestroy(ev);
@@ -3655,19 +3706,20 @@ struct wlx_batch {
     uint32_t slots = 0;
     size_t lds = 0;
     bool philox = false;
+    SampleFn sample = nullptr; // lookahead_fast_kernel<NPL> or the LDS walker
     hipStream_t stream = nullptr;
     uint64_t graph_epochs = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::string kernel_name;
     std::vector<Globals> globs;
     double sample_seconds = 0.0;
 
     int enqueue_epochs(uint64_t count)
     {
-        const auto sample = philox ? batch_sample_kernel<true> : batch_sample_kernel<false>;
         for (uint64_t k = 0; k < count; ++k) {
             hipLaunchKernelGGL(sample, dim3(slots), dim3(kWave), lds, stream, d_table, d_slots,
-                               uint32_t(kGate | kCheck));
+                               uint64_t(0), uint32_t(kGate | kCheck));
             REWL_HIP(hipGetLastError());
             hipLaunchKernelGGL(batch_exchange_kernel, dim3(uint32_t(engines.size())),
                                dim3(kExchangeBlock), 0, stream, d_table);
@@ -3711,6 +3763,10 @@ int wlx_batch_create(wlx_engine *const *engines, uint64_t count, wlx_batch **out
     auto b = std::make_unique<wlx_batch>();
     std::vector<Args> table;
     std::vector<std::pair<uint64_t, BatchSlot>> order; // heaviest systems first
+    // every engine on sample_ph_fast of one register layout: the lookahead walker runs the
+    // batch (the same trajectories as the LDS walker, which runs every other batch)
+    SampleFn fast = engines[0] ? engines[0]->kernels.sample_ph_fast : nullptr;
+    size_t fast_lds = 0;
     for (uint64_t k = 0; k < count; ++k) {
         wlx_engine *e = engines[k];
         if (int status = require_engine(e)) return status;
@@ -3723,14 +3779,24 @@ int wlx_batch_create(wlx_engine *const *engines, uint64_t count, wlx_batch **out
         if (e->f32 || e->collect || e->trace_cap)
             return set_error(WLX_INVALID_ARGUMENT,
                              "Batched engines run FP64 fields without statistics or traces");
-        if (e->lds_walker > 160 * 1024)
-            return set_error(WLX_INVALID_ARGUMENT, "Fields and window bins exceed the LDS of a CU");
+        if (e->sample_fn() != fast) fast = nullptr;
         REWL_HIP(hipStreamSynchronize(e->stream));
         table.push_back(e->args());
         b->lds = std::max(b->lds, e->lds_walker);
+        fast_lds = std::max(fast_lds, e->lds_sample);
         for (uint32_t w = 0; w < e->walkers; ++w)
             order.push_back({e->sys->n, BatchSlot{uint32_t(k), w}});
         b->engines.push_back(e);
+    }
+    if (fast) {
+        b->sample = fast;
+        b->lds = fast_lds;
+        b->kernel_name = "lookahead_fast_kernel<" + std::to_string(engines[0]->sys->npl) + ">";
+    } else if (b->lds > 160 * 1024) {
+        return set_error(WLX_INVALID_ARGUMENT, "Fields and window bins exceed the LDS of a CU");
+    } else {
+        b->sample = b->philox ? engines[0]->kernels.lds_ph : engines[0]->kernels.lds_xo;
+        b->kernel_name = b->philox ? "lds_sample_kernel<true>" : "lds_sample_kernel<false>";
     }
     std::stable_sort(order.begin(), order.end(),
                      [](const auto &x, const auto &y) { return x.first > y.first; });
@@ -3747,9 +3813,8 @@ int wlx_batch_create(wlx_engine *const *engines, uint64_t count, wlx_batch **out
     REWL_HIP(hipMemcpyAsync(b->d_slots, slots.data(), slots.size() * sizeof(BatchSlot),
                             hipMemcpyHostToDevice, b->stream));
     REWL_HIP(hipStreamSynchronize(b->stream));
-    for (const void *fn : {reinterpret_cast<const void *>(batch_sample_kernel<true>),
-                           reinterpret_cast<const void *>(batch_sample_kernel<false>)})
-        if (int status = set_lds_limit(fn, b->lds)) return status;
+    if (int status = set_lds_limit(reinterpret_cast<const void *>(b->sample), b->lds))
+        return status;
     *out = b.release();
     return WLX_OK;
 }
@@ -3785,6 +3850,13 @@ int wlx_batch_set_graph(wlx_batch *b, uint64_t epochs_per_graph)
     REWL_HIP(end);
     b->graph = graph;
     REWL_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return WLX_OK;
+}
+
+int wlx_batch_kernel(const wlx_batch *b, char *name, size_t capacity)
+{
+    if (!b || !name || capacity == 0) return set_error(WLX_INVALID_ARGUMENT, "Null argument");
+    std::snprintf(name, capacity, "%s", b->kernel_name.c_str());
     return WLX_OK;
 }
 

@@ -575,6 +575,12 @@ class Batch:
                           ctypes.byref(epochs), ctypes.byref(done))
         return epochs.value, done.value
 
+    def kernel(self) -> str:
+        """Name of the sampling kernel the batch launches."""
+        name = ctypes.create_string_buffer(64)
+        self.backend.call("batch_kernel", self.handle, name, ctypes.c_size_t(len(name)))
+        return name.value.decode()
+
     def close(self):
         if getattr(self, "handle", None):
             self.backend.call("batch_destroy", self.handle)

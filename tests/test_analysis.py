@@ -145,3 +145,30 @@ def test_checkpoint_resume_equals_uninterrupted_run(orc, tmp_path, rng_mode):
     from harness import assert_walkers_equal
     for k in range(a.walkers):
         assert_walkers_equal(a, c, k)
+
+
+def test_average_dos_removes_offsets_and_averages_noise():
+    from replica_exchange_wang_landau_amd.analysis import average_dos, max_deviation
+
+    rng = np.random.default_rng(5)
+    bins = 40
+    truth = -0.01 * (np.arange(bins) - 17.0) ** 2
+    curves = []
+    for r in range(16):
+        valid = np.ones(bins, dtype=np.uint8)
+        valid[: r % 3] = 0                   # ragged low edges
+        lg = truth + 3.0 * r + rng.normal(0.0, 0.1, bins)
+        lg[valid == 0] = -np.inf
+        curves.append((lg, valid))
+    avg, common = average_dos(curves)
+    assert common[:2].sum() == 0 and common[2:].all()
+    assert avg[common != 0].max() == 0.0
+    # offsets cancel exactly without noise; with it the error falls like 1/sqrt(16)
+    exact, _ = average_dos([(truth + 7.0, np.ones(bins, np.uint8)), (truth - 2.0,
+                                                                       np.ones(bins, np.uint8))])
+    assert max_deviation(exact, np.ones(bins), truth, np.ones(bins)) < 1e-12
+    single = max_deviation(curves[5][0], curves[5][1], truth, np.ones(bins))
+    assert max_deviation(avg, common, truth, np.ones(bins)) < 0.5 * single
+    with pytest.raises(RuntimeError, match="share no valid bin"):
+        average_dos([(truth, np.r_[np.ones(20), np.zeros(20)]),
+                     (truth, np.r_[np.zeros(20), np.ones(20)])])

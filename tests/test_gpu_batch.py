@@ -127,3 +127,92 @@ def test_batched_pilot_equals_single_pilots(gpu):
     for k, s in enumerate(systems):
         a, b = s.pilot(1234, attempts[k])
         assert (lo[k].hex(), hi[k].hex()) == (a.hex(), b.hex()), k
+
+
+def replicas(backend, lx, grid, seeds, walkers=4, max_epochs=0):
+    """Dense simple-cubic replicas that differ only in their seed (Philox, default kernel)."""
+    out = []
+    for seed in seeds:
+        system = System(*simple_cubic(lx, lx, lx, 1.0, (0, 0, 1), True), backend=backend)
+        cfg = RunConfig(emin=grid[0], emax=grid[1], bin_width=grid[2], windows=4,
+                        walkers_per_window=walkers, overlap=0.75, seed=seed,
+                        exchange_interval_attempts=system.n,
+                        max_attempts=system.n * max_epochs if max_epochs else 0,
+                        rng=abi.RNG_PHILOX, wl=WlParameters(check_interval_attempts=8 * system.n))
+        e = Engine(system, cfg)
+        e.initialize()
+        out.append(e)
+    return out
+
+
+GRID_8 = (-273.0, 354.5, 0.5)    # 8x8x8 dipolar cube, D = 1
+
+
+def grid_4(gpu):
+    """A 4x4x4 grid from a pilot sweep, padded 20 %, bins of 0.5."""
+    system = System(*simple_cubic(4, 4, 4, 1.0, (0, 0, 1), True), backend=gpu)
+    emin, emax = grid_for(system, system.n)
+    return float(emin), float(emax), 0.5
+
+
+@pytest.mark.parametrize("lx,npl", [(4, 1), (8, 8)])
+def test_replica_batch_runs_the_lookahead_walker_bit_exact(gpu, lx, npl):
+    """Replicas of one dense system on the default Philox kernel batch onto
+    lookahead_fast_kernel<NPL> (with the batch's slot list) and end up identical to each
+    replica run alone, over plain launches and graph replays."""
+    seeds = [11, 12, 13]
+    grid = GRID_8 if lx == 8 else grid_4(gpu)
+    batched, alone = replicas(gpu, lx, grid, seeds), replicas(gpu, lx, grid, seeds)
+    assert alone[0].kernel()[1] == f"lookahead_fast_kernel<{npl}>"
+    batch = Batch(batched)
+    assert batch.kernel() == f"lookahead_fast_kernel<{npl}>"
+    assert batch.run(5) == (5, 0)
+    batch.set_graph(3)
+    assert batch.run(6) == (6, 0)
+    for e in alone:
+        e.run(11)
+    for a, b in zip(batched, alone):
+        sa, sb = a.status(), b.status()
+        assert (sa.epoch, sa.attempted, sa.accepted, sa.exchange_accepted) == (
+            sb.epoch, sb.attempted, sb.accepted, sb.exchange_accepted)
+        assert sa.accepted > 0
+        for k in range(a.walkers):
+            assert_walkers_equal(a, b, k)
+
+
+def test_mixed_layouts_batch_onto_the_lds_walker(gpu):
+    """Replicas of two sizes (register layouts 1 and 8) fall back to the LDS walker, whose
+    trajectories are the same."""
+    g4 = grid_4(gpu)
+    small, big = replicas(gpu, 4, g4, [3]), replicas(gpu, 8, GRID_8, [4])
+    small_alone, big_alone = replicas(gpu, 4, g4, [3]), replicas(gpu, 8, GRID_8, [4])
+    batch = Batch(small + big)
+    assert batch.kernel() == "lds_sample_kernel<true>"
+    assert batch.run(4) == (4, 0)
+    for a, b in zip(small + big, small_alone + big_alone):
+        b.run(4)
+        for k in range(a.walkers):
+            assert_walkers_equal(a, b, k)
+
+
+def test_smaller_engine_keeps_a_larger_engines_lds_limit(gpu):
+    """Kernels are shared by every engine: creating an engine that needs less LDS (here one
+    window of 5120 bins, ~92 KB) after one that needs more (8192 bins, ~148 KB) must leave
+    the larger engine runnable, alone and in a batch."""
+    def engine(width, seed):
+        system = System(*simple_cubic(8, 8, 8, 1.0, (0, 0, 1), True), backend=gpu)
+        e = Engine(system, RunConfig(emin=-280.0, emax=360.0, bin_width=width, windows=1,
+                                     walkers_per_window=2, seed=seed,
+                                     exchange_interval_attempts=512, rng=abi.RNG_PHILOX))
+        e.initialize()
+        return e
+
+    big = engine(0.078125, 1)
+    assert big.bins == 8192
+    small = engine(0.125, 2)
+    assert small.bins == 5120
+    small.run(2)
+    assert big.run(2).epoch == 2
+    batch = Batch([big, engine(0.078125, 3)])
+    engine(0.125, 4)
+    assert batch.run(2) == (2, 0)
