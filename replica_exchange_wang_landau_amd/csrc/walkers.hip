@@ -906,6 +906,17 @@ __device__ inline double hi_double(uint32_t hi)
     return __hiloint2double(int(hi), 0);
 }
 
+// A value every active lane holds, marked wave-uniform for the compiler: what is computed from
+// it stays uniform, so its branches and selects are scalar instead of exec-mask sequences.
+__device__ inline double uniform_f64(double x)
+{
+    const uint64_t bits = __double_as_longlong(x);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(bits));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(bits >> 32));
+    return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
+}
+__device__ inline uint32_t uniform_u32(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
 // h += m_b J[s_b, :] for the proposals of the batch, in order. m_hi holds the high word of
 // m_b = -2 sigma_b (+-2.0; the low word is zero) in the lane of every accepted proposal and 0
 // elsewhere. m_b J is exact (a power-of-two scaling), so fma(m_b, J, h) rounds exactly like
@@ -1196,7 +1207,7 @@ __device__ __attribute__((always_inline)) inline void fast_walk(const Args &a, u
     uint64_t neg;
     double h[NPL];
     load_config<NPL>(a, st.slot, neg, h);
-    double energy = a.energy[st.slot];
+    double energy = uniform_f64(a.energy[st.slot]);
     __syncthreads();
     const uint64_t count = batch_count(a, st, request, flags);
     const int64_t start_bin = grid_index(a.grid, energy);
@@ -1206,10 +1217,10 @@ __device__ __attribute__((always_inline)) inline void fast_walk(const Args &a, u
         const uint32_t begin = st.begin;
         const uint64_t att0 = st.attempted, clk0 = att0 + st.clock_offset;
         uint32_t cur = uint32_t(start_bin);
-        double lg_cur = v.lg[cur - begin];
-        bool cur_seen = v.act[cur - begin] != 0;
-        uint32_t stage = st.stage, active = st.active_count;
-        double factor = st.factor;
+        double lg_cur = uniform_f64(v.lg[cur - begin]);
+        bool cur_seen = uniform_u32(v.act[cur - begin]) != 0;
+        uint32_t stage = uniform_u32(st.stage), active = uniform_u32(st.active_count);
+        double factor = uniform_f64(st.factor);
         uint32_t done_total = 0, n_acc = 0, last_acc = 0; // last_acc: 1 + index, 0 = none
         uint32_t p_site = 0, used = kWave;
         double p_lu = 0.0;
@@ -1239,13 +1250,22 @@ __device__ __attribute__((always_inline)) inline void fast_walk(const Args &a, u
                     p_site, min(used + uint32_t(q), uint32_t(kWave - 1)));
                 jc[q] = a.J[size_t(sq) * a.np + p_site];
             }
+            // log_g increments of the batch's attempts, lane-parallel: the WL factor, the 1/t
+            // factors, or 0 from the attempt whose 1/t factor reaches final_factor (it freezes
+            // the stage, wang_landau.cpp:309-315) on. A first visit recomputes them for the
+            // later attempts with the new active count.
+            const bool inverse_time = stage == WLX_STAGE_INVERSE_TIME;
             double inc = batch_factors(stage, factor, active, clk0 + done_total, used);
+            uint32_t freeze = kWave; // lane of the freezing attempt; kWave: none
+            if (inverse_time) {
+                const uint64_t fz = __ballot(lane >= used && inc <= a.final_factor);
+                freeze = fz ? uint32_t(__ffsll((unsigned long long)fz) - 1) : kWave;
+            }
+            double incz = lane >= freeze ? 0.0 : inc;
             uint32_t m_hi = 0, cur_lane = 0;
             double lr_lane = 0.0;
-#pragma unroll
-            for (int b = 0; b < KB; ++b) {
-                if (uint32_t(b) >= K) continue;
-                const uint32_t L = used + uint32_t(b);
+            auto decide = [&](uint32_t b) __attribute__((always_inline)) {
+                const uint32_t L = used + b;
                 const double hs = readlane_f64(h_run, L);
                 const uint32_t ms_hi = __builtin_amdgcn_readlane(dsig_hi, L);
                 const double ms = hi_double(ms_hi); // -2 sigma
@@ -1258,8 +1278,8 @@ __device__ __attribute__((always_inline)) inline void fast_walk(const Args &a, u
                 const uint32_t rel = uint32_t(idx) - begin; // idx < end <= count when rel < ws
                 const bool inside = in && rel < ws;
                 const uint32_t rr = inside ? rel : 0u;
-                const double lgn = v.lg[rr];
-                const bool seen_n = v.act[rr] != 0;
+                const double lgn = uniform_f64(v.lg[rr]);
+                const bool seen_n = uniform_u32(v.act[rr]) != 0;
                 const double log_ratio = lg_cur - lgn;
                 const double lu = readlane_f64(p_lu, L);
                 const bool up = log_ratio >= 0.0;
@@ -1275,29 +1295,44 @@ __device__ __attribute__((always_inline)) inline void fast_walk(const Args &a, u
                 cur = accepted ? uint32_t(idx) : cur;
                 lg_cur = accepted ? lgn : lg_cur;
                 cur_seen = accepted ? seen_n : cur_seen;
-                n_acc += accepted ? 1u : 0u;
-                last_acc = accepted ? done_total + uint32_t(b) + 1 : last_acc;
                 const uint32_t crel = cur - begin;
-                const double f = readlane_f64(inc, L);
-                if (stage == WLX_STAGE_INVERSE_TIME) { // wang_landau.cpp:309-315
-                    factor = f;
-                    if (factor <= a.final_factor) stage = WLX_STAGE_FROZEN;
-                }
                 if (!cur_seen) { // first visit: update_current_bin (wang_landau.cpp:294-297)
                     ++active;
                     cur_seen = true;
-                    if (lane == 0) v.act[crel] = 1;
-                    if (stage == WLX_STAGE_INVERSE_TIME) // later attempts see the new count
-                        inc = lane > L ? batch_factors(stage, factor, active,
+                    v.act[crel] = 1; // every lane writes the same byte
+                    if (inverse_time && L < freeze) { // later attempts see the new count
+                        inc = lane > L ? batch_factors(WLX_STAGE_INVERSE_TIME, factor, active,
                                                        clk0 + done_total + b + 1, L + 1)
                                        : inc;
+                        const uint64_t fz = __ballot(lane > L && inc <= a.final_factor);
+                        freeze = fz ? uint32_t(__ffsll((unsigned long long)fz) - 1) : kWave;
+                        incz = lane > L ? (lane >= freeze ? 0.0 : inc) : incz;
+                    }
                 }
-                lg_cur = lg_cur + (stage != WLX_STAGE_FROZEN ? f : 0.0);
-                if (lane == 0) v.lg[crel] = lg_cur;
+                lg_cur = lg_cur + readlane_f64(incz, L);
+                v.lg[crel] = lg_cur; // every lane writes the same value
                 cur_lane = writelane_u32(cur_lane, crel, L);
+            };
+            if (K == uint32_t(KB)) {
+#pragma unroll
+                for (int b = 0; b < KB; ++b) decide(uint32_t(b));
+            } else { // the last batch of a call
+#pragma unroll 1
+                for (uint32_t b = 0; b < K; ++b) decide(b);
+            }
+            if (inverse_time) { // the factor of the last 1/t attempt, and a freeze
+                const uint32_t last_lane = used + K - 1;
+                factor = readlane_f64(inc, min(freeze, last_lane));
+                if (freeze <= last_lane) stage = WLX_STAGE_FROZEN;
             }
             const uint32_t last = used + K - 1;
             const bool mine = lane >= used && lane <= last;
+            { // accepted proposals of the batch: the count and the last one
+                const uint64_t acc_lanes = __ballot(mine && m_hi != 0u);
+                n_acc += uint32_t(__popcll(acc_lanes));
+                if (acc_lanes)
+                    last_acc = done_total + (63u - uint32_t(__clzll(acc_lanes))) - used + 1u;
+            }
             if (mine) // H[cur] += 1 of every attempt of the batch
                 __hip_atomic_fetch_add(&v.H[cur_lane], uint64_t(1), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
