@@ -557,8 +557,9 @@ __device__ inline void finish_walker(const Args &a, uint32_t w, WalkerState &st,
 // with kCheck the flatness cadence of parallel.cpp:395-407 follows the batch.
 // ---------------------------------------------------------------------------------------------
 template <int NPL, bool PHILOX>
-__global__ __launch_bounds__(kWave) void sample_kernel(const Args *table, const BatchSlot *slots, uint64_t request,
-                   uint32_t flags)
+__global__ __launch_bounds__(kWave) void sample_kernel(const Args *table,
+                                                  const BatchSlot *slots,
+                                                  uint64_t request, uint32_t flags)
 {
     const BatchSlot slot = slots[blockIdx.x];
     const Args a = table[slot.engine];
@@ -858,8 +859,9 @@ __device__ inline void lds_walk(const Args &a, uint32_t w, uint64_t request, uin
 }
 
 template <bool PHILOX>
-__global__ __launch_bounds__(kWave) void lds_sample_kernel(const Args *table, const BatchSlot *slots, uint64_t request,
-                   uint32_t flags)
+__global__ __launch_bounds__(kWave) void lds_sample_kernel(const Args *table,
+                                                  const BatchSlot *slots,
+                                                  uint64_t request, uint32_t flags)
 {
     const BatchSlot slot = slots[blockIdx.x];
     const Args a = table[slot.engine];
@@ -958,8 +960,9 @@ __device__ inline void apply_rows(const Args &a, uint32_t p_site, uint32_t m_hi,
 constexpr int kLookaheadBatch = 16;
 
 template <int NPL>
-__global__ __launch_bounds__(kWave) void lookahead_kernel(const Args *table, const BatchSlot *slots, uint64_t request,
-                   uint32_t flags)
+__global__ __launch_bounds__(kWave) void lookahead_kernel(const Args *table,
+                                                  const BatchSlot *slots,
+                                                  uint64_t request, uint32_t flags)
 {
     const BatchSlot slot = slots[blockIdx.x];
     const Args a = table[slot.engine];
@@ -1370,8 +1373,9 @@ __device__ __attribute__((always_inline)) inline void fast_walk(const Args &a, u
 }
 
 template <int NPL>
-__global__ __launch_bounds__(kWave) void lookahead_fast_kernel(const Args *table, const BatchSlot *slots, uint64_t request,
-                   uint32_t flags)
+__global__ __launch_bounds__(kWave) void lookahead_fast_kernel(const Args *table,
+                                                  const BatchSlot *slots,
+                                                  uint64_t request, uint32_t flags)
 {
     const BatchSlot slot = slots[blockIdx.x];
     const Args a = table[slot.engine];
@@ -1397,8 +1401,9 @@ __device__ inline void xoshiro_advance(Xoshiro &r, uint32_t steps)
 }
 
 template <int NPL>
-__global__ __launch_bounds__(kWave) void lookahead_xo_kernel(const Args *table, const BatchSlot *slots, uint64_t request,
-                   uint32_t flags)
+__global__ __launch_bounds__(kWave) void lookahead_xo_kernel(const Args *table,
+                                                  const BatchSlot *slots,
+                                                  uint64_t request, uint32_t flags)
 {
     const BatchSlot slot = slots[blockIdx.x];
     const Args a = table[slot.engine];
@@ -1688,8 +1693,9 @@ __device__ inline void rows_kahan(const Args &a, uint32_t p_site, uint32_t m_hi,
 }
 
 template <int NPL>
-__global__ __launch_bounds__(kWave) void lookahead_f32_kernel(const Args *table, const BatchSlot *slots, uint64_t request,
-                   uint32_t flags)
+__global__ __launch_bounds__(kWave) void lookahead_f32_kernel(const Args *table,
+                                                  const BatchSlot *slots,
+                                                  uint64_t request, uint32_t flags)
 {
     const BatchSlot slot = slots[blockIdx.x];
     const Args a = table[slot.engine];
