@@ -356,6 +356,7 @@ def c3_leg(epochs: int, warmup: int, graph: int, pilot_mcs: int = 10_000):
         "setup_seconds": {"pilot": pilot_s, "total": setup_s},
     }
 
+
 def tta_replicas_leg(budget_seconds: float, interval_epochs: int, replicas: int,
                      graph: int = 16):
     """Time to accuracy of `replicas` independent C2 runs (Philox, seeds 1234 + r) run as one

@@ -1692,16 +1692,16 @@ __device__ inline void rows_kahan(const Args &a, uint32_t p_site, uint32_t m_hi,
     }
 }
 
+// Parameters by value (the kernel-argument segment), not from a table like the other sampling
+// kernels: FP32-field engines never join a batch, and at 64 register slots per lane the table
+// copy costs registers (C4 ran 5 % slower with it).
 template <int NPL>
-__global__ __launch_bounds__(kWave) void lookahead_f32_kernel(const Args *table,
-                                                  const BatchSlot *slots,
-                                                  uint64_t request, uint32_t flags)
+__global__ __launch_bounds__(kWave) void lookahead_f32_kernel(Args a, uint64_t request,
+                                                              uint32_t flags)
 {
-    const BatchSlot slot = slots[blockIdx.x];
-    const Args a = table[slot.engine];
     constexpr int KB = kLookaheadBatch;
     extern __shared__ __align__(16) unsigned char smem[];
-    const uint32_t w = slot.walker, lane = lane_id();
+    const uint32_t w = blockIdx.x, lane = lane_id();
     if ((flags & kGate) && a.glob->done) return;
     WalkerState st = a.state[w];
     if (st.escaped) return;
@@ -2391,13 +2391,15 @@ __global__ __launch_bounds__(kWave) void walker_op_kernel(Args a, uint32_t w, ui
 
 // ---- dispatch ------------------------------------------------------------------------------
 using SampleFn = void (*)(const Args *, const BatchSlot *, uint64_t, uint32_t);
+using F32Fn = void (*)(Args, uint64_t, uint32_t);
 using InitFn = void (*)(Args, const int8_t *, const uint8_t *, InitParams, InitReport *);
 using FieldsFn = void (*)(Args, const int8_t *, double *, double *);
 using PilotFn = void (*)(Args, uint64_t, uint64_t, double *);
 
 struct KernelSet {
-    SampleFn sample_xo, sample_xo_fast, sample_ph, sample_ph_fast, sample_f32, sample_ph_seq;
+    SampleFn sample_xo, sample_xo_fast, sample_ph, sample_ph_fast, sample_ph_seq;
     SampleFn lds_xo, lds_ph; // any N
+    F32Fn sample_f32;
     InitFn init;
     FieldsFn fields;
     PilotFn pilot;
@@ -2406,9 +2408,8 @@ struct KernelSet {
 template <int NPL> KernelSet kernel_set()
 {
     return {sample_kernel<NPL, false>, lookahead_xo_kernel<NPL>, lookahead_kernel<NPL>,
-            lookahead_fast_kernel<NPL>,
-            lookahead_f32_kernel<NPL>, sample_kernel<NPL, true>,
-            lds_sample_kernel<false>, lds_sample_kernel<true>,
+            lookahead_fast_kernel<NPL>, sample_kernel<NPL, true>,
+            lds_sample_kernel<false>, lds_sample_kernel<true>, lookahead_f32_kernel<NPL>,
             init_kernel<NPL>, fields_kernel<NPL>, pilot_kernel<NPL>};
 }
 
@@ -2610,7 +2611,7 @@ struct wlx_engine {
 
     SampleFn sample_fn() const
     {
-        if (f32) return kernels.sample_f32;
+        if (f32) return nullptr; // lookahead_f32_kernel, launched by value (launch_sample)
         const bool philox = cfg.rng == WLX_RNG_PHILOX;
         // the statistics are recorded by the one-flip-at-a-time kernels
         if (collect) return philox ? kernels.sample_ph_seq : kernels.sample_xo;
@@ -2628,6 +2629,12 @@ struct wlx_engine {
     int launch_sample(uint64_t request, uint32_t flags)
     {
         if (trace_cap) flags |= kTrace;
+        if (f32) {
+            hipLaunchKernelGGL(kernels.sample_f32, dim3(walkers), dim3(kWave), lds_sample, stream,
+                               args(), request, flags);
+            REWL_HIP(hipGetLastError());
+            return WLX_OK;
+        }
         const SampleFn fn = sample_fn();
         const bool lds_fn = fn == kernels.lds_xo || fn == kernels.lds_ph;
         if (int status = upload_args()) return status;
@@ -2896,9 +2903,12 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
                                    e->lds_sample))
         return status;
     for (SampleFn fn : {e->kernels.sample_xo_fast, e->kernels.sample_ph, e->kernels.sample_ph_fast,
-                        e->kernels.sample_f32, e->kernels.sample_ph_seq})
+                        e->kernels.sample_ph_seq})
         if (int status = set_lds_limit(reinterpret_cast<const void *>(fn), e->lds_sample))
             return status;
+    if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.sample_f32),
+                                   e->lds_sample))
+        return status;
     if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.init), e->lds_init))
         return status;
     { // the LDS walker stages the configuration in LDS beside the bins
@@ -3211,7 +3221,7 @@ int wlx_engine_kernel(const wlx_engine *e, uint64_t *kernel, char *name, size_t 
         base = "lookahead_xo_kernel";
     } else if (fn == k.sample_ph_fast) {
         base = "lookahead_fast_kernel";
-    } else if (fn == k.sample_f32) {
+    } else if (fn == nullptr) { // FP32 fields
         base = "lookahead_f32_kernel";
     }
     if (kernel) *kernel = family;
