@@ -26,6 +26,8 @@ is within max|dlog g| < 0.01 of a converged C2 reference DOS, after mean-offset 
 time_to_accuracy.replicas does the same for 8 independent C2 runs (seeds 1234..1241) batched
 into one lookahead-walker launch per epoch, whose stitched DOS are averaged: the 128-walker
 workload fills an eighth of the chip's wave slots, the batch fills them with replicas.
+time_to_accuracy.windows runs the same system, grid and schedule split into 48 windows of 16
+walkers (768 walkers in one launch per epoch), with snapshots every 500 epochs.
 
 `--c4` runs only the C4 precision study instead: a 16x16x16 dense system with FP32 Kahan
 fields, the FP64 matrix-core resync every 10 MCS, and the drift it measures.
@@ -70,10 +72,10 @@ def traffic_bytes(kernel: str):
     return None
 
 
-def c2_config(max_attempts=0, rng=0, seed=1234):
+def c2_config(max_attempts=0, rng=0, seed=1234, windows=8):
     from replica_exchange_wang_landau_amd import RunConfig, WlParameters
 
-    return RunConfig(emin=C2_GRID[0], emax=C2_GRID[1], bin_width=C2_GRID[2], windows=8,
+    return RunConfig(emin=C2_GRID[0], emax=C2_GRID[1], bin_width=C2_GRID[2], windows=windows,
                      walkers_per_window=16, overlap=0.75, seed=seed,
                      exchange_interval_attempts=512, max_attempts=max_attempts, rng=rng,
                      wl=WlParameters(final_factor=1e-7, check_interval_attempts=512 * 100))
@@ -412,6 +414,30 @@ def tta_replicas_leg(budget_seconds: float, interval_epochs: int, replicas: int,
             "budget_seconds": budget_seconds}
 
 
+def tta_windows_leg(budget_seconds: float, interval_epochs: int, windows: int):
+    """Time to accuracy of the C2 system, grid and WL schedule split into `windows` windows of
+    16 walkers (overlap 0.75, seed 1234, Philox) instead of C2's 8: the same clock and
+    threshold as tta_leg, snapshots every interval_epochs."""
+    from replica_exchange_wang_landau_amd import Engine, System
+
+    if not REFERENCE_DOS.exists():
+        return None
+    import numpy as np
+
+    ref = json.loads(REFERENCE_DOS.read_text())
+    ref_lg = np.array([float.fromhex(v) for v in ref["log_g"]])
+    ref_valid = np.array(ref["valid"], dtype=np.uint8)
+    engine = Engine(System.simple_cubic(8, 8, 8, 1.0, (0.0, 0.0, 1.0), True),
+                    c2_config(rng=1, windows=windows))
+    engine.initialize()
+    seconds, attempted, best = accuracy_time(engine, ref_lg, ref_valid, budget_seconds,
+                                             interval_epochs)
+    return {"windows": windows, "walkers": engine.walkers, "kernel": engine.kernel()[1],
+            "threshold": 0.01, "seconds": seconds, "attempted": int(attempted),
+            "best_max_dev": best if best != float("inf") else None,
+            "interval_epochs": interval_epochs, "budget_seconds": budget_seconds}
+
+
 def accuracy_time(engine, ref_log_g, ref_valid, budget_s: float, interval: int,
                   threshold: float = 0.01):
     """Device seconds (Engine.run only) until the stitched DOS holds every reference bin and
@@ -454,7 +480,8 @@ def c5_leg(curves: int, steps: int, budget_s: float):
     max|dlog g| < 0.01 from a flat start and from a warm start (network log g; ln f from the
     network's validation error) on
     the held-out 12x12x4 slab at dilution 0.2 (reference: a run of this engine to ln f = 1e-5)
-    and on C2 (reference: tests/golden/c2_reference_dos.json)."""
+    and on C2 (reference: tests/golden/c2_reference_dos.json), with C2's 8 windows and split
+    into 48."""
     import numpy as np
 
     from replica_exchange_wang_landau_amd import Engine, System, abi, analysis
@@ -538,7 +565,7 @@ def c5_leg(curves: int, steps: int, budget_s: float):
     print(f"c5: held-out reference ({hs.n} spins) in {ref_s:.0f} s", file=sys.stderr, flush=True)
     held_result = evaluate(held_engine, ref_lg, ref_valid, held.features(hs.n), 500)
 
-    c2_result = None
+    c2_result = c2w_result = None
     if REFERENCE_DOS.exists():
         ref = json.loads(REFERENCE_DOS.read_text())
         c2_lg = np.array([float.fromhex(v) for v in ref["log_g"]])
@@ -549,6 +576,13 @@ def c5_leg(curves: int, steps: int, budget_s: float):
             e.initialize()
             return e
         c2_result = evaluate(c2_engine, c2_lg, c2_valid, (512.0, 1.0, 0.0, 8.0, 0.0), 4000)
+
+        def c2_windows_engine():  # the many-window form: 48 windows x 16 walkers
+            e = Engine(c2_sys, c2_config(rng=abi.RNG_PHILOX, windows=48))
+            e.initialize()
+            return e
+        c2w_result = evaluate(c2_windows_engine, c2_lg, c2_valid, (512.0, 1.0, 0.0, 8.0, 0.0),
+                              500)
     return {
         "workload": "C5: learned warm start (4 x 128 BF16 MLP, FP32 accumulate)",
         "training": {"curves": split, "validation_curves": len(keep) - split,
@@ -558,6 +592,7 @@ def c5_leg(curves: int, steps: int, budget_s: float):
         "held_out": {"slab": "12x12x4, dilution 0.2, +z axes", "spins": int(hs.n),
                      "reference_seconds": ref_s, **held_result},
         "c2": c2_result,
+        "c2_48_windows": c2w_result,
     }
 
 
@@ -591,6 +626,8 @@ def main():
                          "(0 skips it)")
     ap.add_argument("--tta-replicas-seconds", type=float, default=60.0,
                     help="device-time budget of the replica leg")
+    ap.add_argument("--tta-windows", type=int, default=48,
+                    help="windows of the many-window time to accuracy (0 skips it)")
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
                     help="only run a plain copy of MIB MiB (counter calibration)")
     ap.add_argument("--c5", action="store_true",
@@ -636,6 +673,9 @@ def main():
         tta["replicas"] = tta_replicas_leg(args.tta_replicas_seconds, args.tta_interval,
                                            args.tta_replicas)
         print("bench: replica leg done", file=sys.stderr, flush=True)
+    if tta is not None and args.tta_windows > 0:
+        tta["windows"] = tta_windows_leg(60.0, 500, args.tta_windows)
+        print("bench: many-window leg done", file=sys.stderr, flush=True)
     line = {
         "metric": "attempted flips/s (all walkers)",
         "value": g["rate"],
