@@ -28,6 +28,8 @@ into one lookahead-walker launch per epoch, whose stitched DOS are averaged: the
 workload fills an eighth of the chip's wave slots, the batch fills them with replicas.
 time_to_accuracy.windows runs the same system, grid and schedule split into 48 windows of 16
 walkers (768 walkers in one launch per epoch), with snapshots every 500 epochs.
+time_to_accuracy.warm trains the warm-start network (256 slab curves, 60 000 steps) and times
+warm-started runs of both layouts.
 
 `--c4` runs only the C4 precision study instead: a 16x16x16 dense system with FP32 Kahan
 fields, the FP64 matrix-core resync every 10 MCS, and the drift it measures.
@@ -438,6 +440,37 @@ def tta_windows_leg(budget_seconds: float, interval_epochs: int, windows: int):
             "interval_epochs": interval_epochs, "budget_seconds": budget_seconds}
 
 
+def tta_warm_leg(curves: int, steps: int, budget_seconds: float):
+    """Time to accuracy of warm-started C2 runs (8 windows, snapshots every 4000 epochs; and 48
+    windows, every 500): the network of train_warm_start presets log g and ln f
+    (wlx_engine_warm_start); the clock and threshold are tta_leg's. The flat-start figures of
+    the same layouts are tta_leg's and tta_windows_leg's."""
+    from replica_exchange_wang_landau_amd import Engine, System
+    from replica_exchange_wang_landau_amd import warmstart as ws
+
+    if not REFERENCE_DOS.exists():
+        return None
+    import numpy as np
+
+    ref = json.loads(REFERENCE_DOS.read_text())
+    ref_lg = np.array([float.fromhex(v) for v in ref["log_g"]])
+    ref_valid = np.array(ref["valid"], dtype=np.uint8)
+    mlp, val_rms, training = train_warm_start(curves, steps, "warm")
+    features = (512.0, 1.0, 0.0, 8.0, 0.0)  # N, density, dilution, thickness, axis mode
+    factor = warm_initial_factor(val_rms, features[0])
+    out = {"training": training, "initial_ln_f": factor}
+    for name, windows, interval in (("c2", 8, 4000), ("c2_48_windows", 48, 500)):
+        engine = Engine(System.simple_cubic(8, 8, 8, 1.0, (0.0, 0.0, 1.0), True),
+                        c2_config(rng=1, windows=windows))
+        engine.initialize()
+        engine.warm_start(ws.predicted_log_g(mlp, engine, features), factor)
+        seconds, attempted, best = accuracy_time(engine, ref_lg, ref_valid, budget_seconds,
+                                                 interval)
+        out[name] = {"windows": windows, "seconds": seconds, "attempted": int(attempted),
+                     "best_max_dev": best if best != float("inf") else None}
+    return out
+
+
 def accuracy_time(engine, ref_log_g, ref_valid, budget_s: float, interval: int,
                   threshold: float = 0.01):
     """Device seconds (Engine.run only) until the stitched DOS holds every reference bin and
@@ -472,41 +505,36 @@ def accuracy_time(engine, ref_log_g, ref_valid, budget_s: float, interval: int,
     return None, attempted, best
 
 
-def c5_leg(curves: int, steps: int, budget_s: float):
-    """Learned warm start. Training: `curves` slabs (Lx = Ly in 6..12, Lz in 2..6, dilution
-    in [0, 0.4], +z or random in-plane axes; seed 1234) run to ln f = 1e-3 together in one
-    batch; the 4 x 128 BF16 network is trained on (E/N, N, density, dilution, Lz, axis mode)
-    -> log g / N by the project's kernels (Adam, lr 1e-3, batch 1024). Evaluation: time to
-    max|dlog g| < 0.01 from a flat start and from a warm start (network log g; ln f from the
-    network's validation error) on
-    the held-out 12x12x4 slab at dilution 0.2 (reference: a run of this engine to ln f = 1e-5)
-    and on C2 (reference: tests/golden/c2_reference_dos.json), with C2's 8 windows and split
-    into 48."""
+def train_warm_start(curves: int, steps: int, tag: str = "c5"):
+    """The warm-start network: `curves` slabs (Lx = Ly in 6..12, Lz in 2..6, dilution in
+    [0, 0.4], +z or random in-plane axes; seed 1234) run to ln f = 1e-3 together in one batch,
+    and the 4 x 128 BF16 network trained on (E/N, N, density, dilution, Lz, axis mode) ->
+    log g / N by the project's kernels (Adam 1e-3, then 3e-4 and 1e-4 for the last quarter
+    each, batch 1024). The last tenth of the curves validates it. Returns (network, per-spin
+    validation RMS after removing each curve's mean offset, training record)."""
     import numpy as np
 
-    from replica_exchange_wang_landau_amd import Engine, System, abi, analysis
     from replica_exchange_wang_landau_amd import warmstart as ws
 
     t0 = time.perf_counter()
     slabs = ws.slab_family(curves, 1234)
     data = ws.converged_curves(
         slabs, final_factor=1e-3,
-        report=lambda d, n: print(f"c5: {d}/{n} training curves converged", file=sys.stderr,
+        report=lambda d, n: print(f"{tag}: {d}/{n} training curves converged", file=sys.stderr,
                                   flush=True))
     keep = [(s, c) for s, c in zip(slabs, data) if (c[2] != 0).sum() > 4]
-    split = len(keep) * 9 // 10  # the last tenth of the curves validates the network
+    split = len(keep) * 9 // 10
     x, y = ws.training_set([s for s, _ in keep[:split]], [c for _, c in keep[:split]])
     vx, vy = ws.training_set([s for s, _ in keep[split:]], [c for _, c in keep[split:]])
     curve_s = time.perf_counter() - t0
     mlp = ws.Mlp(1234)
     t1 = time.perf_counter()
-    losses = np.concatenate([  # Adam at 1e-3, then 3e-4 and 1e-4 for the last quarter each
+    losses = np.concatenate([
         mlp.train(x, y, steps=steps // 2, batch=1024, learning_rate=1e-3, seed=5),
         mlp.train(x, y, steps=steps // 4, batch=1024, learning_rate=3e-4, seed=6),
         mlp.train(x, y, steps=steps - steps // 2 - steps // 4, batch=1024, learning_rate=1e-4,
                   seed=7)])
     train_s = time.perf_counter() - t1
-    # per-spin RMS error on unseen curves, after removing each curve's mean offset
     pred = mlp.predict(vx)
     offsets, at = [], 0
     for _, c in keep[split:]:
@@ -515,14 +543,37 @@ def c5_leg(curves: int, steps: int, budget_s: float):
         offsets.append(d - d.mean())
         at += m
     val_rms = float(np.sqrt(np.mean(np.concatenate(offsets) ** 2)))
+    print(f"{tag}: {len(keep)} curves, {len(x)} samples in {curve_s:.0f} s; trained {steps} "
+          f"steps in {train_s:.1f} s, loss {losses[0]:.3g} -> {losses[-1]:.3g}", file=sys.stderr,
+          flush=True)
+    record = {"curves": split, "validation_curves": len(keep) - split,
+              "validation_rms_per_spin": val_rms, "samples": int(len(x)), "steps": steps,
+              "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+              "curve_seconds": curve_s, "train_seconds": train_s}
+    return mlp, val_rms, record
+
+
+def warm_initial_factor(val_rms: float, n: float) -> float:
+    """ln f of a warm start: the square of the expected worst log g error of the network
+    (3 x the validation RMS per spin x N), within [1e-4, 1]."""
+    return float(min(1.0, max(1e-4, (3.0 * val_rms * n) ** 2)))
+
+
+def c5_leg(curves: int, steps: int, budget_s: float):
+    """Learned warm start (train_warm_start). Evaluation: time to max|dlog g| < 0.01 from a
+    flat start and from a warm start (network log g; ln f from the network's validation
+    error) on the held-out 12x12x4 slab at dilution 0.2 (reference: a run of this engine to
+    ln f = 1e-5) and on C2 (reference: tests/golden/c2_reference_dos.json), with C2's 8
+    windows and split into 48."""
+    import numpy as np
+
+    from replica_exchange_wang_landau_amd import Engine, System, abi, analysis
+    from replica_exchange_wang_landau_amd import warmstart as ws
+
+    mlp, val_rms, training = train_warm_start(curves, steps)
 
     def initial_factor(n):
-        """ln f of a warm start: the square of the expected worst log g error of the network
-        (3 x the validation RMS per spin x N), within [1e-4, 1]."""
-        return float(min(1.0, max(1e-4, (3.0 * val_rms * n) ** 2)))
-    print(f"c5: {len(keep)} curves, {len(x)} samples in {curve_s:.0f} s; trained {steps} steps "
-          f"in {train_s:.1f} s, loss {losses[0]:.3g} -> {losses[-1]:.3g}", file=sys.stderr,
-          flush=True)
+        return warm_initial_factor(val_rms, n)
 
     def evaluate(make_engine, ref_lg, ref_valid, features, interval):
         out = {}
@@ -585,10 +636,7 @@ def c5_leg(curves: int, steps: int, budget_s: float):
                               500)
     return {
         "workload": "C5: learned warm start (4 x 128 BF16 MLP, FP32 accumulate)",
-        "training": {"curves": split, "validation_curves": len(keep) - split,
-                     "validation_rms_per_spin": val_rms, "samples": int(len(x)), "steps": steps,
-                     "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
-                     "curve_seconds": curve_s, "train_seconds": train_s},
+        "training": training,
         "held_out": {"slab": "12x12x4, dilution 0.2, +z axes", "spins": int(hs.n),
                      "reference_seconds": ref_s, **held_result},
         "c2": c2_result,
@@ -628,6 +676,8 @@ def main():
                     help="device-time budget of the replica leg")
     ap.add_argument("--tta-windows", type=int, default=48,
                     help="windows of the many-window time to accuracy (0 skips it)")
+    ap.add_argument("--tta-warm-curves", type=int, default=256,
+                    help="training curves of the warm-started time to accuracy (0 skips it)")
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
                     help="only run a plain copy of MIB MiB (counter calibration)")
     ap.add_argument("--c5", action="store_true",
@@ -676,6 +726,9 @@ def main():
     if tta is not None and args.tta_windows > 0:
         tta["windows"] = tta_windows_leg(60.0, 500, args.tta_windows)
         print("bench: many-window leg done", file=sys.stderr, flush=True)
+    if tta is not None and args.tta_warm_curves > 0:
+        tta["warm"] = tta_warm_leg(args.tta_warm_curves, 60_000, 60.0)
+        print("bench: warm-start leg done", file=sys.stderr, flush=True)
     line = {
         "metric": "attempted flips/s (all walkers)",
         "value": g["rate"],
