@@ -1,0 +1,45 @@
+"""Host-side pieces of bench.py that need no GPU: the one-GPU refusal, the counter summary it
+reads the traffic figure from, the warm-start ln f rule and the C2 run configurations."""
+from __future__ import annotations
+
+import subprocess
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(ROOT))
+
+import bench  # noqa: E402
+
+
+def test_more_than_one_gpu_is_refused_with_a_message():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr and r.stdout == ""
+
+
+def test_traffic_figures_come_from_the_committed_counter_summary():
+    for kernel in ("lookahead_fast_kernel<8>", "lookahead_xo_kernel<8>"):
+        value = bench.traffic_bytes(kernel)
+        assert value is not None and 1e6 < value < 1e9
+    assert bench.traffic_bytes("no_such_kernel<1>") is None
+
+
+@pytest.mark.parametrize("rms,n,expected", [(1e-6, 512, 1e-4), (1e-4, 512, (3e-4 * 512) ** 2),
+                                            (1e-3, 512, 1.0)])
+def test_warm_start_factor_is_the_squared_error_bound_clamped(rms, n, expected):
+    assert bench.warm_initial_factor(rms, n) == pytest.approx(expected, rel=1e-12)
+
+
+def test_c2_configurations():
+    plain = bench.c2_config(rng=1)
+    many = bench.c2_config(rng=1, seed=77, windows=48)
+    assert (plain.windows, plain.walkers_per_window, plain.seed) == (8, 16, 1234)
+    assert (many.windows, many.walkers_per_window, many.seed) == (48, 16, 77)
+    for c in (plain, many):
+        assert (c.emin, c.emax, c.bin_width) == bench.C2_GRID
+        assert c.exchange_interval_attempts == 512 and c.overlap == 0.75
+        assert round((c.emax - c.emin) / c.bin_width) == 1255
