@@ -192,6 +192,22 @@ def test_c2_epochs_bit_exact(gpu, orc, rng_mode):
     np.testing.assert_allclose(energies, exact, rtol=1e-10, atol=1e-10)
 
 
+@pytest.mark.parametrize("rng_mode", [abi.RNG_XOSHIRO, abi.RNG_PHILOX])
+def test_c2_many_windows_bit_exact(gpu, orc, rng_mode):
+    """The C2 system split into 48 windows of 16 walkers (about 100 bins each, 768 walkers),
+    the layout of the fastest time to accuracy: 20 exchange epochs, every walker
+    bit-identical."""
+    cfg = c2_config(max_attempts=512 * 20, rng=rng_mode, windows=48)
+    eg, eo = pair_engines(gpu, orc, simple_cubic(8, 8, 8, 1.0, (0, 0, 1), True), cfg)
+    sg, so = eg.run(), eo.run()
+    for key in ("epoch", "done", "attempted", "accepted", "exchange_attempted",
+                "exchange_accepted"):
+        assert getattr(sg, key) == getattr(so, key), key
+    assert eg.walkers == 768 and sg.attempted == 768 * 512 * 20 and sg.exchange_accepted > 0
+    for k in range(eg.walkers):
+        assert_walkers_equal(eg, eo, k)
+
+
 def test_exchange_schedule_bit_exact(gpu, orc):
     cfg = c2_config(max_attempts=0)
     eg, eo = pair_engines(gpu, orc, simple_cubic(8, 8, 8, 1.0, (0, 0, 1), True), cfg)
