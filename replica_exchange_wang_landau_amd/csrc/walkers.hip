@@ -1478,8 +1478,16 @@ __global__ __launch_bounds__(kWave) void lookahead_xo_kernel(const Args *table,
                     p_site, min(pos + uint32_t(q), uint32_t(kWave - 1)));
                 jc[q] = a.J[size_t(sq) * a.np + p_site];
             }
-            // lane j: the log_g increment of the batch's j-th attempt
+            // lane j: the log_g increment of the batch's j-th attempt; from the attempt whose
+            // 1/t factor reaches final_factor (it freezes the stage) on, 0
+            const bool inverse_time = stage == WLX_STAGE_INVERSE_TIME;
             double inc = batch_factors(stage, factor, active, clk0 + done_total, 0);
+            uint32_t freeze = kWave; // attempt that freezes the stage; kWave: none
+            if (inverse_time) {
+                const uint64_t fz = __ballot(inc <= a.final_factor);
+                freeze = fz ? uint32_t(__ffsll((unsigned long long)fz) - 1) : kWave;
+            }
+            double incz = lane >= freeze ? 0.0 : inc;
             uint32_t m_hi = 0, cur_lane = 0, attempts = 0;
             double lr_lane = 0.0;
             uint64_t attempt_lanes = 0;
@@ -1520,32 +1528,42 @@ __global__ __launch_bounds__(kWave) void lookahead_xo_kernel(const Args *table,
                 cur = accepted ? uint32_t(idx) : cur;
                 lg_cur = accepted ? lgn : lg_cur;
                 cur_seen = accepted ? seen_n : cur_seen;
-                n_acc += accepted ? 1u : 0u;
-                last_acc = accepted ? done_total + attempts + 1 : last_acc;
                 const uint32_t crel = cur - begin;
-                const double f = readlane_f64(inc, attempts);
-                if (stage == WLX_STAGE_INVERSE_TIME) {
-                    factor = f;
-                    if (factor <= a.final_factor) stage = WLX_STAGE_FROZEN;
-                }
-                if (!cur_seen) {
+                if (!cur_seen) { // first visit: update_current_bin (wang_landau.cpp:294-297)
                     ++active;
                     cur_seen = true;
-                    if (lane == 0) v.act[crel] = 1;
-                    if (stage == WLX_STAGE_INVERSE_TIME)
+                    v.act[crel] = 1; // every lane writes the same byte
+                    if (inverse_time && attempts < freeze) { // later attempts see the new count
                         inc = lane > attempts
-                                  ? batch_factors(stage, factor, active,
+                                  ? batch_factors(WLX_STAGE_INVERSE_TIME, factor, active,
                                                   clk0 + done_total + attempts + 1, attempts + 1)
                                   : inc;
+                        const uint64_t fz = __ballot(lane > attempts && inc <= a.final_factor);
+                        freeze = fz ? uint32_t(__ffsll((unsigned long long)fz) - 1) : kWave;
+                        incz = lane > attempts ? (lane >= freeze ? 0.0 : inc) : incz;
+                    }
                 }
-                lg_cur = lg_cur + (stage != WLX_STAGE_FROZEN ? f : 0.0);
-                if (lane == 0) v.lg[crel] = lg_cur;
+                lg_cur = lg_cur + readlane_f64(incz, attempts);
+                v.lg[crel] = lg_cur; // every lane writes the same value
                 cur_lane = writelane_u32(cur_lane, crel, L);
                 attempt_lanes |= uint64_t(1) << L;
                 ++attempts;
             }
+            if (inverse_time && attempts > 0) { // the factor of the last 1/t attempt, a freeze
+                factor = readlane_f64(inc, min(freeze, attempts - 1));
+                if (freeze < attempts) stage = WLX_STAGE_FROZEN;
+            }
             // the batch consumed the words before `next`; from position 63 a new block follows
             const bool mine = (attempt_lanes >> lane) & 1;
+            { // accepted proposals: the count and the last one (its attempt index in the batch)
+                const uint64_t acc_lanes = __ballot(mine && m_hi != 0u);
+                n_acc += uint32_t(__popcll(acc_lanes));
+                if (acc_lanes) {
+                    const uint32_t top = 63u - uint32_t(__clzll(acc_lanes));
+                    const uint64_t below = top ? attempt_lanes & ((uint64_t(1) << top) - 1) : 0;
+                    last_acc = done_total + uint32_t(__popcll(below)) + 1u;
+                }
+            }
             if (mine)
                 __hip_atomic_fetch_add(&v.H[cur_lane], uint64_t(1), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
