@@ -1774,7 +1774,15 @@ __global__ __launch_bounds__(kWave) void lookahead_f32_kernel(Args a, uint64_t r
                     p_site, min(used + uint32_t(q), uint32_t(kWave - 1)));
                 jc[q] = a.J32[size_t(sq) * a.np + p_site];
             }
+            // log_g increments, lane-parallel, 0 from the freezing 1/t attempt on (fast_walk)
+            const bool inverse_time = stage == WLX_STAGE_INVERSE_TIME;
             double inc = batch_factors(stage, factor, active, clk0 + done_total, used);
+            uint32_t freeze = kWave;
+            if (inverse_time) {
+                const uint64_t fz = __ballot(lane >= used && inc <= a.final_factor);
+                freeze = fz ? uint32_t(__ffsll((unsigned long long)fz) - 1) : kWave;
+            }
+            double incz = lane >= freeze ? 0.0 : inc;
             uint32_t m_hi = 0, cur_lane = 0;
             double lr_lane = 0.0;
 #pragma unroll
@@ -1813,29 +1821,37 @@ __global__ __launch_bounds__(kWave) void lookahead_f32_kernel(Args a, uint64_t r
                 cur = accepted ? uint32_t(idx) : cur;
                 lg_cur = accepted ? lgn : lg_cur;
                 cur_seen = accepted ? seen_n : cur_seen;
-                n_acc += accepted ? 1u : 0u;
-                last_acc = accepted ? done_total + uint32_t(b) + 1 : last_acc;
                 const uint32_t crel = cur - begin;
-                const double f = readlane_f64(inc, L);
-                if (stage == WLX_STAGE_INVERSE_TIME) {
-                    factor = f;
-                    if (factor <= a.final_factor) stage = WLX_STAGE_FROZEN;
-                }
-                if (!cur_seen) {
+                if (!cur_seen) { // first visit: update_current_bin (wang_landau.cpp:294-297)
                     ++active;
                     cur_seen = true;
-                    if (lane == 0) v.act[crel] = 1;
-                    if (stage == WLX_STAGE_INVERSE_TIME)
-                        inc = lane > L ? batch_factors(stage, factor, active,
+                    v.act[crel] = 1; // every lane writes the same byte
+                    if (inverse_time && L < freeze) { // later attempts see the new count
+                        inc = lane > L ? batch_factors(WLX_STAGE_INVERSE_TIME, factor, active,
                                                        clk0 + done_total + b + 1, L + 1)
                                        : inc;
+                        const uint64_t fz = __ballot(lane > L && inc <= a.final_factor);
+                        freeze = fz ? uint32_t(__ffsll((unsigned long long)fz) - 1) : kWave;
+                        incz = lane > L ? (lane >= freeze ? 0.0 : inc) : incz;
+                    }
                 }
-                lg_cur = lg_cur + (stage != WLX_STAGE_FROZEN ? f : 0.0);
-                if (lane == 0) v.lg[crel] = lg_cur;
+                lg_cur = lg_cur + readlane_f64(incz, L);
+                v.lg[crel] = lg_cur; // every lane writes the same value
                 cur_lane = writelane_u32(cur_lane, crel, L);
+            }
+            if (inverse_time) { // the factor of the last 1/t attempt, and a freeze
+                const uint32_t last_lane = used + K - 1;
+                factor = readlane_f64(inc, min(freeze, last_lane));
+                if (freeze <= last_lane) stage = WLX_STAGE_FROZEN;
             }
             const uint32_t last = used + K - 1;
             const bool mine = lane >= used && lane <= last;
+            { // accepted proposals of the batch: the count and the last one
+                const uint64_t acc_lanes = __ballot(mine && m_hi != 0u);
+                n_acc += uint32_t(__popcll(acc_lanes));
+                if (acc_lanes)
+                    last_acc = done_total + (63u - uint32_t(__clzll(acc_lanes))) - used + 1u;
+            }
             if (mine)
                 __hip_atomic_fetch_add(&v.H[cur_lane], uint64_t(1), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
