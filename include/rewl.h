@@ -282,9 +282,9 @@ int wlx_engine_autotune(wlx_engine *engine, uint64_t epochs, uint64_t *chosen, d
  * No reference counterpart (the reference runs one system per process). A batch runs the
  * epochs of several initialized engines together: per epoch one sampling launch over every
  * walker of every engine (one wave per walker) and one exchange launch (one workgroup per
- * engine). The sampling launch runs the lookahead walker when every engine would run
- * lookahead_fast_kernel with one register layout (Philox, same padded N, no forced
- * acceptance or Nalivaiko rule), else the LDS walker. Each engine keeps its own schedule,
+ * engine). The sampling launch runs the kernel every engine would run alone when that is one
+ * register kernel for all of them (one family and padded N, e.g. lookahead_fast_kernel<8> for
+ * replicas of one system), else the LDS walker. Each engine keeps its own schedule,
  * gate, counters and results, identical to running it alone. The engines must share the
  * proposal stream and use FP64 fields without statistics or traces; the batch captures their
  * parameters at creation, and an engine must not be run on its own while the batch runs. */
@@ -297,7 +297,8 @@ int wlx_batch_set_graph(wlx_batch *batch, uint64_t epochs_per_graph);
  * enqueued and how many engines are done. */
 int wlx_batch_run(wlx_batch *batch, uint64_t max_epochs, uint64_t *epochs,
                   uint64_t *done_engines);
-/* Name of the batch's sampling kernel ("lookahead_fast_kernel<8>", "lds_sample_kernel<true>"). */
+/* Name of the batch's sampling kernel ("lookahead_fast_kernel<8>", "lds_sample_kernel<true>",
+ * as wlx_engine_kernel names them). */
 int wlx_batch_kernel(const wlx_batch *batch, char *name, size_t capacity);
 /* ---- warm-start network (no reference counterpart) ---------------------------------------
  * A 4-layer, 128-wide MLP, 6 features -> 128 -> 128 -> 128 -> 1 with SiLU hidden

@@ -3848,10 +3848,10 @@ int wlx_batch_create(wlx_engine *const *engines, uint64_t count, wlx_batch **out
     auto b = std::make_unique<wlx_batch>();
     std::vector<Args> table;
     std::vector<std::pair<uint64_t, BatchSlot>> order; // heaviest systems first
-    // every engine on sample_ph_fast of one register layout: the lookahead walker runs the
-    // batch (the same trajectories as the LDS walker, which runs every other batch)
-    SampleFn fast = engines[0] ? engines[0]->kernels.sample_ph_fast : nullptr;
-    size_t fast_lds = 0;
+    // every engine on one register kernel (one family and register layout): that kernel runs
+    // the batch; any other mix runs the LDS walker (the trajectories are the same)
+    SampleFn common = engines[0] ? engines[0]->sample_fn() : nullptr;
+    size_t common_lds = 0;
     for (uint64_t k = 0; k < count; ++k) {
         wlx_engine *e = engines[k];
         if (int status = require_engine(e)) return status;
@@ -3864,19 +3864,21 @@ int wlx_batch_create(wlx_engine *const *engines, uint64_t count, wlx_batch **out
         if (e->f32 || e->collect || e->trace_cap)
             return set_error(WLX_INVALID_ARGUMENT,
                              "Batched engines run FP64 fields without statistics or traces");
-        if (e->sample_fn() != fast) fast = nullptr;
+        if (e->sample_fn() != common) common = nullptr;
         REWL_HIP(hipStreamSynchronize(e->stream));
         table.push_back(e->args());
         b->lds = std::max(b->lds, e->lds_walker);
-        fast_lds = std::max(fast_lds, e->lds_sample);
+        common_lds = std::max(common_lds, e->lds_sample);
         for (uint32_t w = 0; w < e->walkers; ++w)
             order.push_back({e->sys->n, BatchSlot{uint32_t(k), w}});
         b->engines.push_back(e);
     }
-    if (fast) {
-        b->sample = fast;
-        b->lds = fast_lds;
-        b->kernel_name = "lookahead_fast_kernel<" + std::to_string(engines[0]->sys->npl) + ">";
+    if (common && common != engines[0]->kernels.lds_ph && common != engines[0]->kernels.lds_xo) {
+        b->sample = common;
+        b->lds = common_lds;
+        char name[64];
+        if (int status = wlx_engine_kernel(engines[0], nullptr, name, sizeof name)) return status;
+        b->kernel_name = name;
     } else if (b->lds > 160 * 1024) {
         return set_error(WLX_INVALID_ARGUMENT, "Fields and window bins exceed the LDS of a CU");
     } else {

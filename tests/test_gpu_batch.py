@@ -129,8 +129,8 @@ def test_batched_pilot_equals_single_pilots(gpu):
         assert (lo[k].hex(), hi[k].hex()) == (a.hex(), b.hex()), k
 
 
-def replicas(backend, lx, grid, seeds, walkers=4, max_epochs=0):
-    """Dense simple-cubic replicas that differ only in their seed (Philox, default kernel)."""
+def replicas(backend, lx, grid, seeds, walkers=4, max_epochs=0, rng=abi.RNG_PHILOX):
+    """Dense simple-cubic replicas that differ only in their seed (default kernel)."""
     out = []
     for seed in seeds:
         system = System(*simple_cubic(lx, lx, lx, 1.0, (0, 0, 1), True), backend=backend)
@@ -138,7 +138,7 @@ def replicas(backend, lx, grid, seeds, walkers=4, max_epochs=0):
                         walkers_per_window=walkers, overlap=0.75, seed=seed,
                         exchange_interval_attempts=system.n,
                         max_attempts=system.n * max_epochs if max_epochs else 0,
-                        rng=abi.RNG_PHILOX, wl=WlParameters(check_interval_attempts=8 * system.n))
+                        rng=rng, wl=WlParameters(check_interval_attempts=8 * system.n))
         e = Engine(system, cfg)
         e.initialize()
         out.append(e)
@@ -155,17 +155,20 @@ def grid_4(gpu):
     return float(emin), float(emax), 0.5
 
 
+@pytest.mark.parametrize("rng,kernel", [(abi.RNG_PHILOX, "lookahead_fast_kernel"),
+                                        (abi.RNG_XOSHIRO, "lookahead_xo_kernel")])
 @pytest.mark.parametrize("lx,npl", [(4, 1), (8, 8)])
-def test_replica_batch_runs_the_lookahead_walker_bit_exact(gpu, lx, npl):
-    """Replicas of one dense system on the default Philox kernel batch onto
-    lookahead_fast_kernel<NPL> (with the batch's slot list) and end up identical to each
-    replica run alone, over plain launches and graph replays."""
+def test_replica_batch_runs_the_lookahead_walker_bit_exact(gpu, lx, npl, rng, kernel):
+    """Replicas of one dense system on the default kernel of either stream batch onto that
+    lookahead kernel (with the batch's slot list) and end up identical to each replica run
+    alone, over plain launches and graph replays."""
     seeds = [11, 12, 13]
     grid = GRID_8 if lx == 8 else grid_4(gpu)
-    batched, alone = replicas(gpu, lx, grid, seeds), replicas(gpu, lx, grid, seeds)
-    assert alone[0].kernel()[1] == f"lookahead_fast_kernel<{npl}>"
+    batched = replicas(gpu, lx, grid, seeds, rng=rng)
+    alone = replicas(gpu, lx, grid, seeds, rng=rng)
+    assert alone[0].kernel()[1] == f"{kernel}<{npl}>"
     batch = Batch(batched)
-    assert batch.kernel() == f"lookahead_fast_kernel<{npl}>"
+    assert batch.kernel() == f"{kernel}<{npl}>"
     assert batch.run(5) == (5, 0)
     batch.set_graph(3)
     assert batch.run(6) == (6, 0)
