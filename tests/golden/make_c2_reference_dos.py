@@ -4,7 +4,7 @@ time-to-accuracy is measured against.
 C2 is the 8x8x8 periodic dense system with 8 windows x 16 walkers, exchanges every MCS and
 the 1/t stage down to ln f = 1e-7. Its exact DOS is out of reach (2^512 states), so the
 reference is a long plain-REWL run of this engine to convergence: every walker frozen at
-ln f <= 1e-7. It uses its own seed, so it is independent of the runs it judges. The run takes
+ln f <= 1e-7. It uses its own seed, so it is independent of the runs measured against it. The run takes
 roughly 6e11 attempts, which needs several GPU sessions. Each invocation runs for at most
 --seconds, checkpoints every walker (Engine.save_checkpoint) and resumes from that checkpoint
 on the next call, restarting each configuration's energy and fields from an FP64 recompute. When all walkers are frozen, the stitched log g (stitch_dos normalisation:
