@@ -455,18 +455,23 @@ def tta_warm_leg(curves: int, steps: int, budget_seconds: float):
     ref = json.loads(REFERENCE_DOS.read_text())
     ref_lg = np.array([float.fromhex(v) for v in ref["log_g"]])
     ref_valid = np.array(ref["valid"], dtype=np.uint8)
-    mlp, val_rms, training = train_warm_start(curves, steps, "warm")
+    mlp, val_rms, training, validation = train_warm_start(curves, steps, "warm")
     features = (512.0, 1.0, 0.0, 8.0, 0.0)  # N, density, dilution, thickness, axis mode
-    factor = warm_initial_factor(val_rms, features[0])
-    out = {"training": training, "initial_ln_f": factor}
+    out = {"training": training}
     for name, windows, interval in (("c2", 8, 4000), ("c2_48_windows", 48, 500)):
         engine = Engine(System.simple_cubic(8, 8, 8, 1.0, (0.0, 0.0, 1.0), True),
                         c2_config(rng=1, windows=windows))
         engine.initialize()
+        # ln f from the error the network leaves inside windows of this layout's width
+        info = engine.read_walker(0).info
+        width = (info.window_end - info.window_begin) * engine.config.bin_width / features[0]
+        rms = window_rms(validation, width)
+        factor = warm_initial_factor(rms, features[0])
         engine.warm_start(ws.predicted_log_g(mlp, engine, features), factor)
         seconds, attempted, best = accuracy_time(engine, ref_lg, ref_valid, budget_seconds,
                                                  interval)
-        out[name] = {"windows": windows, "seconds": seconds, "attempted": int(attempted),
+        out[name] = {"windows": windows, "window_rms_per_spin": rms, "initial_ln_f": factor,
+                     "seconds": seconds, "attempted": int(attempted),
                      "best_max_dev": best if best != float("inf") else None}
     return out
 
@@ -537,10 +542,12 @@ def train_warm_start(curves: int, steps: int, tag: str = "c5"):
     train_s = time.perf_counter() - t1
     pred = mlp.predict(vx)
     offsets, at = [], 0
+    validation = []  # per unseen curve: E/N of its bins and the network's error there
     for _, c in keep[split:]:
         m = int((c[2] != 0).sum())
         d = pred[at:at + m] - vy[at:at + m]
         offsets.append(d - d.mean())
+        validation.append((2.0 * vx[at:at + m, 0].astype(np.float64), d.astype(np.float64)))
         at += m
     val_rms = float(np.sqrt(np.mean(np.concatenate(offsets) ** 2)))
     print(f"{tag}: {len(keep)} curves, {len(x)} samples in {curve_s:.0f} s; trained {steps} "
@@ -550,7 +557,22 @@ def train_warm_start(curves: int, steps: int, tag: str = "c5"):
               "validation_rms_per_spin": val_rms, "samples": int(len(x)), "steps": steps,
               "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
               "curve_seconds": curve_s, "train_seconds": train_s}
-    return mlp, val_rms, record
+    return mlp, val_rms, record, validation
+
+
+def window_rms(validation, width_per_spin: float) -> float:
+    """Per-spin RMS of the network's error on the unseen curves after removing its mean over
+    each window: consecutive E/N ranges of `width_per_spin`. Stitching removes each window's
+    offset, so this is the error a warm start leaves inside windows of that width."""
+    import numpy as np
+
+    parts = []
+    for e, d in validation:
+        tile = np.floor((e - e.min()) / width_per_spin).astype(np.int64)
+        for k in np.unique(tile):
+            part = d[tile == k]
+            parts.append(part - part.mean())
+    return float(np.sqrt(np.mean(np.concatenate(parts) ** 2)))
 
 
 def warm_initial_factor(val_rms: float, n: float) -> float:
@@ -570,7 +592,7 @@ def c5_leg(curves: int, steps: int, budget_s: float):
     from replica_exchange_wang_landau_amd import Engine, System, abi, analysis
     from replica_exchange_wang_landau_amd import warmstart as ws
 
-    mlp, val_rms, training = train_warm_start(curves, steps)
+    mlp, val_rms, training, _ = train_warm_start(curves, steps)
 
     def initial_factor(n):
         return warm_initial_factor(val_rms, n)

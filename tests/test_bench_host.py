@@ -43,3 +43,16 @@ def test_c2_configurations():
         assert (c.emin, c.emax, c.bin_width) == bench.C2_GRID
         assert c.exchange_interval_attempts == 512 and c.overlap == 0.75
         assert round((c.emax - c.emin) / c.bin_width) == 1255
+
+
+def test_window_rms_removes_each_window_mean():
+    """A linear error d = k e, sampled densely, leaves k w / sqrt(12) inside windows of width w
+    once each window's mean is removed; a constant error leaves nothing."""
+    import numpy as np
+
+    e = np.linspace(0.0, 4.0, 400001)[:-1]
+    k = 2e-3
+    for w in (1.0, 0.5, 0.25):
+        rms = bench.window_rms([(e, k * e)], w)
+        assert rms == pytest.approx(k * w / np.sqrt(12.0), rel=1e-3)
+    assert bench.window_rms([(e, np.full_like(e, 0.7))], 0.5) == pytest.approx(0.0, abs=1e-12)
