@@ -455,7 +455,7 @@ def tta_warm_leg(curves: int, steps: int, budget_seconds: float):
     ref = json.loads(REFERENCE_DOS.read_text())
     ref_lg = np.array([float.fromhex(v) for v in ref["log_g"]])
     ref_valid = np.array(ref["valid"], dtype=np.uint8)
-    mlp, val_rms, training, validation = train_warm_start(curves, steps, "warm")
+    mlp, _, training, validation = train_warm_start(curves, steps, "warm")
     features = (512.0, 1.0, 0.0, 8.0, 0.0)  # N, density, dilution, thickness, axis mode
     out = {"training": training}
     for name, windows, interval in (("c2", 8, 4000), ("c2_48_windows", 48, 500)):
@@ -584,7 +584,7 @@ def warm_initial_factor(val_rms: float, n: float) -> float:
 def c5_leg(curves: int, steps: int, budget_s: float):
     """Learned warm start (train_warm_start). Evaluation: time to max|dlog g| < 0.01 from a
     flat start and from a warm start (network log g; ln f from the network's validation
-    error) on the held-out 12x12x4 slab at dilution 0.2 (reference: a run of this engine to
+    error inside windows of the run's width) on the held-out 12x12x4 slab at dilution 0.2 (reference: a run of this engine to
     ln f = 1e-5) and on C2 (reference: tests/golden/c2_reference_dos.json), with C2's 8
     windows and split into 48."""
     import numpy as np
@@ -592,17 +592,20 @@ def c5_leg(curves: int, steps: int, budget_s: float):
     from replica_exchange_wang_landau_amd import Engine, System, abi, analysis
     from replica_exchange_wang_landau_amd import warmstart as ws
 
-    mlp, val_rms, training, _ = train_warm_start(curves, steps)
+    mlp, _, training, validation = train_warm_start(curves, steps)
 
-    def initial_factor(n):
-        return warm_initial_factor(val_rms, n)
+    def initial_factor(engine, n):
+        """ln f from the network's error inside windows of the engine's width."""
+        info = engine.read_walker(0).info
+        width = (info.window_end - info.window_begin) * engine.config.bin_width / n
+        return warm_initial_factor(window_rms(validation, width), n)
 
     def evaluate(make_engine, ref_lg, ref_valid, features, interval):
         out = {}
         for mode in ("flat", "warm"):
             engine = make_engine()
             if mode == "warm":
-                f0 = initial_factor(features[0])
+                f0 = initial_factor(engine, features[0])
                 engine.warm_start(ws.predicted_log_g(mlp, engine, features), f0)
             seconds, attempted, best = accuracy_time(engine, ref_lg, ref_valid, budget_s, interval)
             out[mode] = {"seconds": seconds, "attempted": int(attempted), "best_max_dev": best}
