@@ -15,7 +15,8 @@ the reference's) is timed beside it and reported under config.xoshiro.
 
 The roofline object prices the sampling kernel's dense row stream: each accepted flip
 streams one row J[i, :] of N doubles (N * 8 bytes); achieved = accepted flips per launch *
-N * 8 bytes / mean launch duration (HIP events on the engine stream).
+N * 8 bytes / mean launch duration (HIP events on the engine stream around every 8th launch of
+the timed region).
 
 cpu_baseline runs the reference's own run_rewl (compiled from its sources into oracle/_ref)
 on the same workload with OpenMP on the host cores, bounded to a fixed attempt budget.
@@ -56,6 +57,9 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "tests"))
 
 HBM_PEAK_GBS = 8000.0
+# HIP events bracket every TIMING_EVERY-th sampling launch of the timed region: the mean
+# launch duration for the roofline, without an event pair around every launch of the rate
+TIMING_EVERY = 8
 C2_GRID = (-273.0, 354.5, 0.5)
 # HBM bytes per launch of the sampling kernels on this workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes scaled by a copy-kernel calibration; the summary (with its commands) is
@@ -92,7 +96,7 @@ def gpu_leg(steps: int, warmup: int, rng: int):
     engine.initialize()
     engine.run(warmup)
     before = engine.status()
-    engine.set_timing(True)
+    engine.set_timing(True, every=TIMING_EVERY)
     t0 = time.perf_counter()
     after = engine.run(steps)  # returns after the device has finished (status read)
     t1 = time.perf_counter()

@@ -333,9 +333,10 @@ int wlx_engine_next_iteration(wlx_engine *engine, uint64_t walker);
 int wlx_engine_trace(wlx_engine *engine, uint64_t capacity);
 int wlx_engine_read_trace(wlx_engine *engine, uint64_t walker, uint64_t *count,
                           uint32_t *sites, uint32_t *bins, uint8_t *flags);
-/* Times every sampling-kernel launch of wlx_engine_run with HIP events on the engine stream
- * (enable != 0) and resets the accumulators; wlx_engine_kernel_time returns the mean launch
- * duration in ms and the number of timed launches. No reference counterpart. */
+/* Times sampling-kernel launches of wlx_engine_run with HIP events on the engine stream and
+ * resets the accumulators: enable = k > 0 times every k-th launch (1: every launch), 0 stops.
+ * wlx_engine_kernel_time returns the mean duration of the timed launches in ms and their
+ * number. No reference counterpart. */
 int wlx_engine_set_timing(wlx_engine *engine, int enable);
 int wlx_engine_kernel_time(wlx_engine *engine, double *sample_ms, uint64_t *launches);
 

@@ -2557,6 +2557,7 @@ struct wlx_engine {
     size_t lds_sample = 0, lds_init = 0, lds_walker = 0;
     uint64_t kernel_choice = WLX_KERNEL_AUTO;
     bool timing = false;
+    uint64_t timing_every = 1, timing_count = 0; // time launches timing_count % every == 0
     std::vector<hipEvent_t> events;
     double sample_ms = 0.0;
     uint64_t launches = 0;
@@ -3437,10 +3438,12 @@ int wlx_engine_run(wlx_engine *e, uint64_t max_epochs, wlx_run_status *status)
                 e->events.push_back(ev);
             }
         }
+        uint64_t timed = 0; // event pairs recorded in this chunk
         for (uint64_t k = 0; k < n; ++k) {
-            if (e->timing) REWL_HIP(hipEventRecord(e->events[2 * k], e->stream));
+            const bool time_it = e->timing && e->timing_count++ % e->timing_every == 0;
+            if (time_it) REWL_HIP(hipEventRecord(e->events[2 * timed], e->stream));
             if (int st = e->launch_sample(0, kGate | kCheck)) return st;
-            if (e->timing) REWL_HIP(hipEventRecord(e->events[2 * k + 1], e->stream));
+            if (time_it) REWL_HIP(hipEventRecord(e->events[2 * timed++ + 1], e->stream));
             if (e->resync_every && ++e->since_resync >= e->resync_every) {
                 if (int st = e->launch_resync()) return st;
                 e->since_resync = 0;
@@ -3451,7 +3454,7 @@ int wlx_engine_run(wlx_engine *e, uint64_t max_epochs, wlx_run_status *status)
         REWL_HIP(hipMemcpyAsync(&g, e->d_glob, sizeof g, hipMemcpyDeviceToHost, e->stream));
         REWL_HIP(hipStreamSynchronize(e->stream));
         if (e->timing)
-            for (uint64_t k = 0; k < n; ++k) {
+            for (uint64_t k = 0; k < timed; ++k) {
                 float ms = 0.0f;
                 REWL_HIP(hipEventElapsedTime(&ms, e->events[2 * k], e->events[2 * k + 1]));
                 e->sample_ms += ms;
@@ -3767,7 +3770,10 @@ int wlx_engine_read_trace(wlx_engine *e, uint64_t w, uint64_t *count, uint32_t *
 int wlx_engine_set_timing(wlx_engine *e, int enable)
 {
     if (!e) return set_error(WLX_INVALID_ARGUMENT, "Null engine");
+    if (enable < 0) return set_error(WLX_INVALID_ARGUMENT, "Timing period must be >= 0");
     e->timing = enable != 0;
+    e->timing_every = enable > 0 ? uint64_t(enable) : 1;
+    e->timing_count = 0;
     e->sample_ms = 0.0;
     e->launches = 0;
     return WLX_OK;

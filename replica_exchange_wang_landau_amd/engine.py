@@ -527,8 +527,13 @@ class Engine:
         c = count.value
         return sites[:c], bins[:c], flags[:c]
 
-    def set_timing(self, enable: bool = True):
-        self.backend.call("engine_set_timing", self.handle, ctypes.c_int(int(enable)))
+    def set_timing(self, enable: bool = True, every: int = 1):
+        """HIP events around every `every`-th sampling launch of run() (wlx_engine_set_timing);
+        kernel_time() then averages the timed launches."""
+        if every < 1:
+            raise InvalidArgument("Timing period must be >= 1")
+        self.backend.call("engine_set_timing", self.handle,
+                          ctypes.c_int(int(every) if enable else 0))
 
     def kernel_time(self) -> tuple[float, int]:
         ms, launches = ctypes.c_double(), ctypes.c_uint64()

@@ -219,3 +219,23 @@ def test_smaller_engine_keeps_a_larger_engines_lds_limit(gpu):
     batch = Batch([big, engine(0.078125, 3)])
     engine(0.125, 4)
     assert batch.run(2) == (2, 0)
+
+
+def test_sampled_launch_timing(gpu):
+    """wlx_engine_set_timing(k): events around every k-th sampling launch; the mean of the
+    sampled launches is close to the mean of all of them, and 0 switches timing off."""
+    e = replicas(gpu, 8, GRID_8, [5])[0]
+    e.run(10)
+    e.set_timing(True)
+    e.run(40)
+    every_ms, every_n = e.kernel_time()
+    e.set_timing(True, every=4)
+    e.run(40)
+    sampled_ms, sampled_n = e.kernel_time()
+    assert (every_n, sampled_n) == (40, 10)
+    assert every_ms > 0.0 and abs(sampled_ms - every_ms) < 0.25 * every_ms
+    e.set_timing(False)
+    e.run(5)
+    assert e.kernel_time() == (0.0, 0)
+    with pytest.raises(InvalidArgument):
+        e.set_timing(True, every=0)
