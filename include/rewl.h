@@ -217,6 +217,11 @@ int wlx_engine_exchange(wlx_engine *engine, uint64_t epoch, uint64_t *attempted,
  * when every walker is frozen or the attempt limit is reached. Resumable. */
 int wlx_engine_run(wlx_engine *engine, uint64_t max_epochs, wlx_run_status *status);
 int wlx_engine_status(wlx_engine *engine, wlx_run_status *status);
+/* Replays `epochs_per_graph` epochs of wlx_engine_run (the sampling and exchange launches of
+ * parallel.cpp:376-421) from one HIP graph, recaptured when the launch parameters or the
+ * walker kernel change; 0 = plain launches (the default). Timed runs (wlx_engine_set_timing)
+ * and FP32-field runs with a resync cadence launch plainly. No reference counterpart. */
+int wlx_engine_set_graph(wlx_engine *engine, uint64_t epochs_per_graph);
 /* Restores the run counters of a resumed run: the exchange epoch (which drives the even/odd
  * parity and the exchange random stream, parallel.cpp:410-421) and the exchange tallies, as
  * stored by a multi-walker checkpoint next to the per-walker WLCHKP3 records. Clears the

@@ -150,6 +150,7 @@ EXPORTS = [
     "wlx_engine_exchange",
     "wlx_engine_run",
     "wlx_engine_status",
+    "wlx_engine_set_graph",
     "wlx_engine_set_run_state",
     "wlx_engine_walkers",
     "wlx_engine_read_walker",

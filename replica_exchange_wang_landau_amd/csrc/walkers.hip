@@ -2562,7 +2562,13 @@ struct wlx_engine {
     double sample_ms = 0.0;
     uint64_t launches = 0;
     uint64_t epochs_run = 0;
-    uint64_t chunk = 1;
+    // HIP graph of graph_epochs epochs (sampling and exchange launches) that run replays;
+    // recaptured when the launch parameters or the sampling kernel change
+    uint64_t graph_epochs = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    Args graph_args{};
+    SampleFn graph_fn = nullptr;
     // FP32 fields (WLX_FIELDS_FP32_KAHAN)
     bool f32 = false;
     float2 *d_fields32 = nullptr;
@@ -2702,6 +2708,45 @@ struct wlx_engine {
         hipLaunchKernelGGL(exchange_kernel, dim3(1), dim3(kExchangeBlock), 0, stream, args(),
                            gated, epoch);
         REWL_HIP(hipGetLastError());
+        return WLX_OK;
+    }
+
+    void release_graph()
+    {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_fn = nullptr;
+    }
+
+    // the graph of graph_epochs epochs for the current parameters and kernel: kernel nodes
+    // only (the parameter table is uploaded before the capture, outside the graph)
+    int ensure_graph()
+    {
+        if (int status = upload_args()) return status;
+        const Args a = args();
+        const SampleFn fn = sample_fn();
+        if (graph_exec && fn == graph_fn && std::memcmp(&a, &graph_args, sizeof a) == 0)
+            return WLX_OK;
+        release_graph();
+        REWL_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int status = WLX_OK;
+        for (uint64_t k = 0; k < graph_epochs && status == WLX_OK; ++k) {
+            status = launch_sample(0, kGate | kCheck);
+            if (status == WLX_OK) status = launch_exchange(1, 0);
+        }
+        hipGraph_t captured = nullptr;
+        const hipError_t end = hipStreamEndCapture(stream, &captured);
+        if (status) {
+            if (captured) (void)hipGraphDestroy(captured);
+            return status;
+        }
+        REWL_HIP(end);
+        graph = captured;
+        REWL_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+        std::memcpy(&graph_args, &a, sizeof a);
+        graph_fn = fn;
         return WLX_OK;
     }
 
@@ -3029,6 +3074,7 @@ int wlx_engine_destroy(wlx_engine *e)
                        e->d_sampling, e->d_rep, e->d_args, e->d_slots};
     for (void *p : buffers)
         if (p) (void)hipFree(p);
+    e->release_graph();
     for (hipEvent_t ev : e->events) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
@@ -3100,7 +3146,6 @@ int wlx_engine_initialize(wlx_engine *e, const int8_t *initial, const uint8_t *s
     if (d_initial) (void)hipFree(d_initial);
     if (d_supplied) (void)hipFree(d_supplied);
     e->epochs_run = 0;
-    e->chunk = 1;
     if (e->f32) { // the FP32 pairs start from the FP64 local fields
         hipLaunchKernelGGL(narrow_fields_kernel, dim3(256), dim3(256), 0, e->stream, e->d_fields,
                            e->d_fields32, size_t(W) * s->np);
@@ -3392,7 +3437,6 @@ int wlx_engine_set_run_state(wlx_engine *e, uint64_t epoch, uint64_t exchange_at
     g.exchange_accepted = exchange_accepted;
     REWL_HIP(hipMemcpyAsync(e->d_glob, &g, sizeof g, hipMemcpyHostToDevice, e->stream));
     REWL_HIP(hipStreamSynchronize(e->stream));
-    e->chunk = 1;
     return WLX_OK;
 }
 
@@ -3427,16 +3471,22 @@ int wlx_engine_status(wlx_engine *e, wlx_run_status *st)
 int wlx_engine_run(wlx_engine *e, uint64_t max_epochs, wlx_run_status *status)
 {
     if (int st = require_engine(e)) return st;
+    // the device flags (done, escaped) are polled after every chunk of epochs; a run that
+    // finishes inside a chunk leaves the rest of its launches gated (they return at once)
+    constexpr uint64_t kChunk = 64;
+    // timed launches and the host-side resync cadence run as plain launches
+    const bool graphs = e->graph_epochs > 0 && !e->timing && e->resync_every == 0;
+    const uint64_t chunk = std::max(kChunk, graphs ? e->graph_epochs : 0);
+    if (e->timing && e->events.size() < 2 * kChunk)
+        return set_error(WLX_LOGIC_ERROR, "Timing events were not created");
     uint64_t remaining = max_epochs;
-    constexpr uint64_t kMaxChunk = 64;
     while (remaining > 0) {
-        const uint64_t n = std::min(remaining, e->chunk);
-        if (e->timing && e->events.size() < 2 * n) {
-            while (e->events.size() < 2 * kMaxChunk) {
-                hipEvent_t ev;
-                REWL_HIP(hipEventCreate(&ev));
-                e->events.push_back(ev);
-            }
+        uint64_t n = std::min(remaining, chunk);
+        remaining -= n;
+        if (graphs && n >= e->graph_epochs) {
+            if (int st = e->ensure_graph()) return st;
+            for (; n >= e->graph_epochs; n -= e->graph_epochs)
+                REWL_HIP(hipGraphLaunch(e->graph_exec, e->stream));
         }
         uint64_t timed = 0; // event pairs recorded in this chunk
         for (uint64_t k = 0; k < n; ++k) {
@@ -3453,19 +3503,27 @@ int wlx_engine_run(wlx_engine *e, uint64_t max_epochs, wlx_run_status *status)
         Globals g{};
         REWL_HIP(hipMemcpyAsync(&g, e->d_glob, sizeof g, hipMemcpyDeviceToHost, e->stream));
         REWL_HIP(hipStreamSynchronize(e->stream));
-        if (e->timing)
-            for (uint64_t k = 0; k < timed; ++k) {
-                float ms = 0.0f;
-                REWL_HIP(hipEventElapsedTime(&ms, e->events[2 * k], e->events[2 * k + 1]));
-                e->sample_ms += ms;
-                ++e->launches;
-            }
-        remaining -= n;
+        for (uint64_t k = 0; k < timed; ++k) {
+            float ms = 0.0f;
+            REWL_HIP(hipEventElapsedTime(&ms, e->events[2 * k], e->events[2 * k + 1]));
+            e->sample_ms += ms;
+            ++e->launches;
+        }
         if (g.escaped) return set_error(WLX_RUNTIME_ERROR, "Walker escaped its energy window");
         if (g.done) break;
-        e->chunk = std::min(kMaxChunk, e->chunk * 2);
     }
     if (status) return wlx_engine_status(e, status);
+    return WLX_OK;
+}
+
+int wlx_engine_set_graph(wlx_engine *e, uint64_t epochs_per_graph)
+{
+    if (!e) return set_error(WLX_INVALID_ARGUMENT, "Null engine");
+    if (epochs_per_graph > 4096)
+        return set_error(WLX_INVALID_ARGUMENT, "At most 4096 epochs per graph");
+    if (e->stream) REWL_HIP(hipStreamSynchronize(e->stream));
+    e->release_graph();
+    e->graph_epochs = epochs_per_graph;
     return WLX_OK;
 }
 
@@ -3771,6 +3829,13 @@ int wlx_engine_set_timing(wlx_engine *e, int enable)
 {
     if (!e) return set_error(WLX_INVALID_ARGUMENT, "Null engine");
     if (enable < 0) return set_error(WLX_INVALID_ARGUMENT, "Timing period must be >= 0");
+    // one event pair per launch of a polling chunk (64 epochs), created here so that no
+    // timed region pays for their creation
+    while (enable > 0 && e->events.size() < 2 * 64) {
+        hipEvent_t ev;
+        REWL_HIP(hipEventCreate(&ev));
+        e->events.push_back(ev);
+    }
     e->timing = enable != 0;
     e->timing_every = enable > 0 ? uint64_t(enable) : 1;
     e->timing_count = 0;

@@ -535,6 +535,11 @@ class Engine:
         self.backend.call("engine_set_timing", self.handle,
                           ctypes.c_int(int(every) if enable else 0))
 
+    def set_graph(self, epochs_per_graph: int):
+        """run() replays `epochs_per_graph` captured epochs per graph launch
+        (wlx_engine_set_graph; 0: plain launches)."""
+        self.backend.call("engine_set_graph", self.handle, ctypes.c_uint64(epochs_per_graph))
+
     def kernel_time(self) -> tuple[float, int]:
         ms, launches = ctypes.c_double(), ctypes.c_uint64()
         self.backend.call("engine_kernel_time", self.handle, ctypes.byref(ms),

@@ -239,3 +239,32 @@ def test_sampled_launch_timing(gpu):
     assert e.kernel_time() == (0.0, 0)
     with pytest.raises(InvalidArgument):
         e.set_timing(True, every=0)
+
+
+@pytest.mark.parametrize("rng", [abi.RNG_PHILOX, abi.RNG_XOSHIRO])
+def test_engine_graph_replays_equal_plain_launches(gpu, rng):
+    """wlx_engine_set_graph(k): run() replays k captured epochs per graph launch. Replays with
+    host reads in between, a run length that is not a multiple of k, a kernel change (the
+    graph is recaptured) and a run that finishes inside a graph all end bit-identical to plain
+    launches of the same engine."""
+    graphed = replicas(gpu, 8, GRID_8, [21], rng=rng, max_epochs=40)[0]
+    plain = replicas(gpu, 8, GRID_8, [21], rng=rng, max_epochs=40)[0]
+    graphed.set_graph(4)
+    for epochs in (8, 5, 4):
+        sg, sp = graphed.run(epochs), plain.run(epochs)
+        assert (sg.epoch, sg.attempted, sg.accepted, sg.exchange_accepted) == (
+            sp.epoch, sp.attempted, sp.accepted, sp.exchange_accepted)
+        for k in range(graphed.walkers):
+            assert_walkers_equal(graphed, plain, k)
+    for e in (graphed, plain):
+        e.set_kernel(abi.KERNEL_SEQUENTIAL)
+    assert graphed.run(6).epoch == plain.run(6).epoch == 23
+    for e in (graphed, plain):
+        e.set_kernel(abi.KERNEL_AUTO)
+    sg, sp = graphed.run(100), plain.run(100)  # the attempt limit falls at epoch 40
+    assert sg.done and sp.done and sg.epoch == sp.epoch == 40
+    assert sg.attempted == sp.attempted
+    for k in range(graphed.walkers):
+        assert_walkers_equal(graphed, plain, k)
+    with pytest.raises(InvalidArgument):
+        graphed.set_graph(5000)
