@@ -127,6 +127,10 @@ typedef struct wlx_walker_info {
     uint64_t rng_state[4];
     uint64_t last_flatness_check; /* parallel.cpp:364 */
     double last_flatness;         /* parallel.cpp:365 */
+    /* 1/t clock offset set by wlx_engine_warm_start (0 otherwise): the 1/t factor is
+     * active / (attempted + clock_offset). No reference counterpart; restored by
+     * wlx_engine_write_walker so that a resumed warm-started run continues exactly. */
+    uint64_t clock_offset;
 } wlx_walker_info;
 
 /* HistogramStatistics, wang_landau.hpp:60-67 */

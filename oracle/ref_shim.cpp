@@ -143,6 +143,28 @@ int ref_simple_cubic(uint64_t nx, uint64_t ny, uint64_t nz, double spacing, cons
     }
 }
 
+// Geometry::load_xyz_axes (csv = 0, physics.cpp:77-97) or Geometry::load_csv (csv = 1,
+// physics.cpp:99-149) of `path`: *n sites; positions/axes (N x 3) are written when *n <= capacity
+int ref_load_geometry(const char *path, int csv, const double *box, int periodic,
+                      uint64_t capacity, double *positions, double *axes, uint64_t *n)
+{
+    try {
+        const wl::Box b{{box[0], box[1], box[2]}, periodic != 0};
+        const auto g = csv ? wl::Geometry::load_csv(path, b) : wl::Geometry::load_xyz_axes(path, b);
+        *n = g.size();
+        if (g.size() <= capacity)
+            for (size_t i = 0; i < g.size(); ++i) {
+                const double p[3] = {g.positions[i].x, g.positions[i].y, g.positions[i].z};
+                const double a[3] = {g.axes[i].x, g.axes[i].y, g.axes[i].z};
+                std::memcpy(positions + 3 * i, p, sizeof p);
+                std::memcpy(axes + 3 * i, a, sizeof a);
+            }
+        return WLX_OK;
+    } catch (const std::exception &e) {
+        return report(e);
+    }
+}
+
 void *ref_system_create(const double *positions, const double *axes, uint64_t n,
                         const double *box, int periodic, double scale, double cutoff)
 {

@@ -945,10 +945,15 @@ static int run_batch(orc_engine *e, walker_t *w, uint64_t count)
     return 0;
 }
 
+/* attempts per parallel region below which the walkers run on one thread: the fork/join of
+ * a region costs more than a few thousand attempts (C1: 160 attempts per epoch) */
+#define PARALLEL_ATTEMPTS 32768u
+
 int orc_engine_sample(orc_engine *e, uint64_t attempts)
 {
     int escaped = 0;
-#pragma omp parallel for schedule(static) reduction(| : escaped)
+#pragma omp parallel for schedule(static) reduction(| : escaped) \
+    if (e->walkers * attempts >= PARALLEL_ATTEMPTS)
     for (int64_t k = 0; k < (int64_t)e->walkers; ++k) escaped |= run_batch(e, &e->w[k], attempts) != 0;
     if (escaped) return fail(WLX_RUNTIME_ERROR, "Walker escaped its energy window");
     return WLX_OK;
@@ -1145,7 +1150,8 @@ int orc_engine_run(orc_engine *e, uint64_t max_epochs, wlx_run_status *status)
     const int unlimited = e->cfg.max_attempts == 0;
     for (uint64_t done_epochs = 0; !e->done && done_epochs < max_epochs; ++done_epochs) {
         int limit_reached = !unlimited, escaped = 0;
-#pragma omp parallel for schedule(static) reduction(| : escaped)
+#pragma omp parallel for schedule(static) reduction(| : escaped) \
+    if (e->walkers * e->cfg.exchange_interval_attempts >= PARALLEL_ATTEMPTS)
         for (int64_t k = 0; k < (int64_t)e->walkers; ++k) {
             walker_t *w = &e->w[k];
             uint64_t batch = e->cfg.exchange_interval_attempts;
@@ -1210,6 +1216,7 @@ int orc_engine_read_walker(orc_engine *e, uint64_t k, wlx_walker_info *info, int
         memcpy(info->rng_state, w->rng, sizeof w->rng);
         info->last_flatness_check = w->last_check;
         info->last_flatness = w->last_flatness;
+        info->clock_offset = w->clock_offset;
     }
     if (spins) memcpy(spins, w->spins, n);
     if (fields) memcpy(fields, w->fields, n * sizeof(double));
@@ -1258,7 +1265,7 @@ int orc_engine_write_walker(orc_engine *e, uint64_t k, const wlx_walker_info *s,
     w->forced = s->forced_accepted;
     w->last_accepted = s->last_accepted_attempt;
     w->stage = (int)s->stage;
-    w->clock_offset = 0;
+    w->clock_offset = s->clock_offset;
     memcpy(w->rng, s->rng_state, sizeof w->rng);
     double *exact = malloc(n * sizeof(double));
     local_fields(e->sys, w->spins, exact);

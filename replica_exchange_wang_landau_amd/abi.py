@@ -103,6 +103,7 @@ class WalkerInfo(ctypes.Structure):
         ("rng_state", c_u64 * 4),
         ("last_flatness_check", c_u64),
         ("last_flatness", c_f64),
+        ("clock_offset", c_u64),
     ]
 
 

@@ -3581,6 +3581,7 @@ int wlx_engine_read_walker(wlx_engine *e, uint64_t w, wlx_walker_info *info, int
         for (int k = 0; k < 4; ++k) info->rng_state[k] = st.rng[k];
         info->last_flatness_check = st.last_check;
         info->last_flatness = st.last_flatness;
+        info->clock_offset = st.clock_offset;
     }
     if (spins) std::memcpy(spins, sp.data(), s->n);
     if (fields) std::memcpy(fields, fl.data(), s->n * sizeof(double));
@@ -3634,6 +3635,7 @@ int wlx_engine_read_infos(wlx_engine *e, wlx_walker_info *infos)
         for (int k = 0; k < 4; ++k) info.rng_state[k] = st.rng[k];
         info.last_flatness_check = st.last_check;
         info.last_flatness = st.last_flatness;
+        info.clock_offset = st.clock_offset;
     }
     return WLX_OK;
 }
@@ -3681,7 +3683,7 @@ int wlx_engine_write_walker(wlx_engine *e, uint64_t w, const wlx_walker_info *in
     st.refine_count = refine_count;
     for (int k = 0; k < 4; ++k) st.rng[k] = info->rng_state[k];
     st.escaped = 0;
-    st.clock_offset = 0;
+    st.clock_offset = info->clock_offset;
     std::vector<int8_t> sp(s->np, 1);
     std::vector<double> fl(s->np, 0.0);
     std::memcpy(sp.data(), spins, s->n);

@@ -401,7 +401,7 @@ class Engine:
     _SNAP_FIELDS = ("walker_id", "window", "window_begin", "window_end", "energy", "factor",
                     "attempted", "accepted", "forced_accepted", "last_accepted_attempt",
                     "stage", "active_bins", "refinement_active_bins", "last_flatness_check",
-                    "last_flatness")
+                    "last_flatness", "clock_offset")
 
     def save_checkpoint(self, path) -> None:
         """Writes every walker's WalkerSnapshot (wang_landau.hpp:74-89) and the run counters
@@ -429,6 +429,8 @@ class Engine:
         differs from an uninterrupted one in the last bits."""
         with np.load(path, allow_pickle=False) as z:
             data = {k: z[k] for k in z.files}
+        if "clock_offset" not in data:  # files written before warm starts were checkpointed
+            data["clock_offset"] = np.zeros(self.walkers, dtype=np.uint64)
         if tuple(int(x) for x in data["shape"]) != (self.walkers, self.n, self.bins):
             raise InvalidArgument("Checkpoint does not match the engine's walkers, spins or bins")
         if recompute_fields:

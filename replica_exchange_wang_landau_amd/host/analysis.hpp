@@ -29,6 +29,7 @@ struct WalkerState {
     std::uint64_t rng_state[4]{};
     std::uint64_t last_flatness_check{}; // driver cadence (parallel.cpp:364-365)
     double last_flatness{};
+    std::uint64_t clock_offset{}; // 1/t clock offset of a warm start (0 without one)
 };
 
 struct HistogramStatistics { // wang_landau.hpp:60-67

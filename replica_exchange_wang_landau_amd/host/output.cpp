@@ -371,7 +371,7 @@ WalkerState load_checkpoint(const std::string &path)
 void save_run_checkpoint(const std::string &path, const RunCheckpoint &run)
 {
     write_atomically(path, [&](std::ostream &out) {
-        const std::array<char, 8> magic{'W', 'L', 'C', 'H', 'K', 'M', '1', '\0'};
+        const std::array<char, 8> magic{'W', 'L', 'C', 'H', 'K', 'M', '2', '\0'};
         out.write(magic.data(), magic.size());
         put(out, static_cast<std::uint64_t>(run.walkers.size()));
         put(out, run.epoch);
@@ -380,6 +380,7 @@ void save_run_checkpoint(const std::string &path, const RunCheckpoint &run)
         for (const WalkerState &w : run.walkers) {
             put(out, w.last_flatness_check);
             put(out, w.last_flatness);
+            put(out, w.clock_offset);
         }
         for (const WalkerState &w : run.walkers) put_record(out, w);
     });
@@ -391,8 +392,10 @@ RunCheckpoint load_run_checkpoint(const std::string &path)
     if (!in) throw std::runtime_error("Cannot open checkpoint " + path);
     std::array<char, 8> magic{};
     in.read(magic.data(), magic.size());
-    if (!in || std::string_view(magic.data(), 7) != "WLCHKM1")
+    const std::string_view tag(magic.data(), 7);
+    if (!in || (tag != "WLCHKM1" && tag != "WLCHKM2"))
         throw std::runtime_error("Invalid multi-walker checkpoint format");
+    const bool offsets = tag == "WLCHKM2"; // WLCHKM1 files predate warm starts: offset 0
     RunCheckpoint run;
     std::uint64_t count{};
     get(in, count);
@@ -402,14 +405,17 @@ RunCheckpoint load_run_checkpoint(const std::string &path)
     get(in, run.exchange_accepted);
     std::vector<std::uint64_t> checks(count);
     std::vector<double> flatness(count);
+    std::vector<std::uint64_t> clock(count, 0);
     for (std::uint64_t k = 0; k < count; ++k) {
         get(in, checks[k]);
         get(in, flatness[k]);
+        if (offsets) get(in, clock[k]);
     }
     for (std::uint64_t k = 0; k < count; ++k) {
         run.walkers.push_back(get_record(in));
         run.walkers.back().last_flatness_check = checks[k];
         run.walkers.back().last_flatness = flatness[k];
+        run.walkers.back().clock_offset = clock[k];
     }
     return run;
 }

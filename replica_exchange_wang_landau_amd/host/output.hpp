@@ -51,9 +51,11 @@ void save_checkpoint(const std::string &path, const WalkerState &walker);
 WalkerState load_checkpoint(const std::string &path);
 
 // Checkpoint of a whole run with any window x walker layout (the reference checkpoints 1x1 runs
-// only): "WLCHKM1\0", u64 walker count, u64 epoch, u64 exchange attempts, u64 accepted
+// only): "WLCHKM2\0", u64 walker count, u64 epoch, u64 exchange attempts, u64 accepted
 // exchanges, then per walker u64 last flatness check and f64 last flatness (the driver's
-// cadence, parallel.cpp:364-365), then one WLCHKP3 record per walker in walker order.
+// cadence, parallel.cpp:364-365) and u64 1/t clock offset (a warm start's; 0 otherwise), then
+// one WLCHKP3 record per walker in walker order. WLCHKM1 files (no clock offsets) still load,
+// with offset 0.
 struct RunCheckpoint {
     std::uint64_t epoch{}, exchange_attempted{}, exchange_accepted{};
     std::vector<WalkerState> walkers; // last_flatness_check / last_flatness filled

@@ -82,6 +82,7 @@ WalkerState download(wlx_engine *e, std::uint64_t k, std::size_t n, std::size_t 
     std::memcpy(w.rng_state, info.rng_state, sizeof w.rng_state);
     w.last_flatness_check = info.last_flatness_check;
     w.last_flatness = info.last_flatness;
+    w.clock_offset = info.clock_offset;
     return w;
 }
 
@@ -122,11 +123,12 @@ void restore(wlx_engine *e, const WalkerState &s, std::size_t n, std::size_t bin
     info.last_accepted_attempt = s.last_accepted_attempt;
     info.stage = std::uint64_t(s.stage);
     std::memcpy(info.rng_state, s.rng_state, sizeof info.rng_state);
+    info.clock_offset = s.clock_offset;
     check(wlx_engine_write_walker(e, k, &info, s.spins.data(), s.fields.data(), s.log_g.data(),
                                   s.histogram.data(), s.active.data()));
 }
 
-// every walker, the driver's cadence and the run counters from a WLCHKM1 file
+// every walker, the driver's cadence and the run counters from a WLCHKM1/WLCHKM2 file
 void restore_run(wlx_engine *e, const RunCheckpoint &run, std::size_t walkers, std::size_t n,
                  std::size_t bins)
 {

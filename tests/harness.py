@@ -117,6 +117,24 @@ class Reference:
                                      _ptr(out, ctypes.c_uint64))
         return out
 
+    def load_geometry(self, path, box, periodic, csv=False):
+        """Geometry::load_xyz_axes / load_csv of `path`: (positions, axes), or the exception
+        text as a string when the loader throws."""
+        bx = np.ascontiguousarray(box, dtype=np.float64)
+        n = ctypes.c_uint64(0)
+        args = (str(path).encode(), ctypes.c_int(1 if csv else 0), _ptr(bx, ctypes.c_double),
+                ctypes.c_int(1 if periodic else 0))
+        null = ctypes.cast(None, ctypes.POINTER(ctypes.c_double))
+        if self.lib.ref_load_geometry(*args, ctypes.c_uint64(0), null, null, ctypes.byref(n)):
+            return self.error()
+        pos = np.empty((n.value, 3))
+        ax = np.empty((n.value, 3))
+        status = self.lib.ref_load_geometry(*args, ctypes.c_uint64(n.value),
+                                            _ptr(pos, ctypes.c_double), _ptr(ax, ctypes.c_double),
+                                            ctypes.byref(n))
+        assert status == 0, self.error()
+        return pos, ax
+
     def system(self, positions, axes, box, periodic, scale=1.0, cutoff=0.0):
         pos = np.ascontiguousarray(positions, dtype=np.float64)
         ax = np.ascontiguousarray(axes, dtype=np.float64)

@@ -21,9 +21,9 @@ import pytest
 from harness import _ptr
 
 ROOT = Path(__file__).resolve().parents[1]
-SELFTEST = ROOT / "replica_exchange_wang_landau_amd" / "bin" / "host_selftest"
+SELFTEST = ROOT / "tests" / "host" / "_bin" / "host_checks"
 
-pytestmark = pytest.mark.skipif(not SELFTEST.exists(), reason="host_selftest not built")
+pytestmark = pytest.mark.skipif(not SELFTEST.exists(), reason="tests/host/_bin/host_checks not built")
 
 DEFAULTS = dict(smoothing_width=0.0, minimum_width=0.0, diffusivity_floor_fraction=0.05,
                 curvature_weight=0.25, round_trip_target=2.0, maximum_round_trip_penalty=3.0)

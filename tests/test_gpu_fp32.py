@@ -209,3 +209,103 @@ def test_wl_run_fp32_fields_from_environment(tmp_path):
     assert "converged=yes" in r.stdout
     meta = json.loads((tmp_path / "f32_metadata.json").read_text())
     assert meta["engine"]["fields"] == "fp32-kahan" and meta["engine"]["rng"] == "philox4x64-10"
+
+
+# ---- C4 shape: simple-cubic 16x16x16 periodic, axes drawn from +-x, +-y, +-z --------------
+def c4_geometry():
+    """16x16x16 simple cubic (N = 4096), periodic box 16, each site's axis drawn from
+    {+x, -x, +y, -y, +z, -z} by numpy's default_rng(1234)."""
+    pos, _, box, per = simple_cubic(16, 16, 16, 1.0, (0.0, 0.0, 1.0), True)
+    choice = np.random.default_rng(1234).integers(0, 6, size=len(pos))
+    unit = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], float)
+    return pos, unit[choice], box, per
+
+
+@pytest.fixture(scope="module")
+def c4(gpu):
+    """The C4 system and its grid: bin 2.0 over a 10^3-MCS pilot range padded by 5 %."""
+    system = System(*c4_geometry(), backend=gpu)
+    lo, hi = system.pilot(1234, 1000 * system.n)
+    pad = 0.05 * (hi - lo)
+    return system, 2.0 * math.floor((lo - pad) / 2.0), 2.0 * math.ceil((hi + pad) / 2.0)
+
+
+def c4_config(emin, emax, windows, per_window, **kw):
+    n = 4096
+    return RunConfig(emin=emin, emax=emax, bin_width=2.0, windows=windows,
+                     walkers_per_window=per_window, overlap=0.75, seed=1234,
+                     exchange_interval_attempts=n, rng=abi.RNG_PHILOX,
+                     fields=abi.FIELDS_FP32_KAHAN, **kw)
+
+
+def test_fp32_lookahead_c4_shape_equals_sequential_kahan_walker(orc, c4):
+    """lookahead_f32_kernel<64> (64 register slots per lane) at N = 4096 with the C4 axes: two
+    walkers' spins, FP32 field pairs, energies, log g and histograms after 2500 attempts equal
+    the one-flip-at-a-time FP32 Kahan walker bit for bit."""
+    system, emin, emax = c4
+    cfg = c4_config(emin, emax, 2, 1, wl=WlParameters(check_interval_attempts=10**9))
+    e = Engine(system, cfg)
+    e.initialize()
+    assert e.kernel()[1] == "lookahead_f32_kernel<64>"
+    J32 = system.dense().astype(np.float32)
+    refs = []
+    for k in range(2):
+        snap = e.read_walker(k)
+        spins = snap.spins.copy()
+        f64, _ = system.fields(spins[None, :])
+        walker = {"id": k, "begin": int(snap.window_begin), "end": int(snap.window_end)}
+        refs.append(sequential_f32_walk(orc, J32, spins, f64[0], snap.energy,
+                                        snap.log_g.copy(), snap.histogram.copy(),
+                                        snap.active.copy(), cfg, walker, int(snap.attempted),
+                                        2500, snap.factor))
+    for chunk in (17, 1000, 1483):
+        e.sample(chunk)
+    for k, ref in enumerate(refs):
+        got = e.read_walker(k)
+        assert got.accepted > 1000
+        assert np.array_equal(got.spins, ref[0]), k
+        assert got.fields.tobytes() == ref[1].tobytes(), k
+        assert got.energy.hex() == float(ref[2]).hex(), k
+        assert got.log_g.tobytes() == ref[3].tobytes(), k
+        assert np.array_equal(got.histogram, ref[4]), k
+
+
+def test_resync_c4_4096_sites_256_slots(c4):
+    """The matrix-core resync at C4's shape (H = J.S for 256 configurations of 4096 spins):
+    every configuration's fields within 1e-12 relative of the sequential local_fields
+    (physics.cpp:238-251, computed by the fields kernel in the reference's order) and its
+    energy within 1e-12 relative of total_energy."""
+    system, emin, emax = c4
+    e = Engine(system, c4_config(emin, emax, 32, 8,
+                                 wl=WlParameters(final_factor=1e-7,
+                                                 check_interval_attempts=100 * 4096)))
+    e.initialize()
+    assert e.walkers == 256
+    e.run(3)
+    e.resync()
+    spins = np.stack([e.read_walker(k).spins for k in range(e.walkers)])
+    assert len({s.tobytes() for s in spins}) == 256
+    fields, energies = system.fields(spins)
+    worst = 0.0
+    for k in range(e.walkers):
+        w = e.read_walker(k)
+        worst = max(worst, float(np.max(np.abs(w.fields - fields[k]) /
+                                        np.maximum(1.0, np.abs(fields[k])))))
+        assert abs(w.energy - energies[k]) <= 1e-12 * max(1.0, abs(energies[k])), k
+    assert worst <= 1e-12, worst
+
+
+def test_fp32_drift_bounds_c4_shape(c4):
+    """C4 as configured (32 windows x 8 walkers, resync every 10 MCS) for 120 epochs: 12
+    resyncs, field drift below 1e-4 absolute and energy drift below 1e-2 of a bin (2.0) at
+    every resync."""
+    system, emin, emax = c4
+    e = Engine(system, c4_config(emin, emax, 32, 8, resync_interval_attempts=10 * 4096,
+                                 wl=WlParameters(final_factor=1e-7,
+                                                 check_interval_attempts=100 * 4096)))
+    e.initialize()
+    st = e.run(120)
+    assert st.epoch == 120
+    f, en, count = e.drift()
+    assert count == 12
+    assert 0.0 < f < 1e-4 and en < 1e-2, (f, en)

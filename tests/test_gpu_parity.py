@@ -461,3 +461,35 @@ def test_xoshiro_lookahead_rejected_site_draw_bit_exact(gpu, orc):
         eo.sample(chunk)
         for k in range(2):
             assert_walkers_equal(eg, eo, k)
+
+
+@pytest.mark.parametrize("rng_mode", [abi.RNG_XOSHIRO, abi.RNG_PHILOX])
+def test_warm_started_checkpoint_resume_bit_exact(gpu, orc, tmp_path, rng_mode):
+    """A warm-started C2 run (per-walker 1/t clock offset) saved after 400 epochs, restored
+    into a fresh engine and run 400 more equals the oracle's uninterrupted 800 epochs: the
+    clock offset travels with the checkpoint, so the switch to 1/t and its factors agree."""
+    cfg = c2_config(rng=rng_mode)
+    cfg.wl.check_interval_attempts = 512 * 4
+    geo = simple_cubic(8, 8, 8, 1.0, (0, 0, 1), True)
+    eg, eo = pair_engines(gpu, orc, geo, cfg)
+    ref = json.loads((GOLDEN / "c2_reference_dos.json").read_text())
+    start = np.array([float.fromhex(v) for v in ref["log_g"]])
+    valid = np.array(ref["valid"], dtype=bool)
+    start = np.interp(np.arange(eg.bins), np.nonzero(valid)[0], start[valid])  # finite everywhere
+    for e in (eg, eo):
+        e.warm_start(start, 1e-3)
+    eg.run(400)
+    offsets = [eg.read_walker(k).info.clock_offset for k in range(eg.walkers)]
+    assert all(o > 0 for o in offsets)
+    eg.save_checkpoint(tmp_path / "warm.npz")
+    resumed = Engine(System(*geo, backend=gpu), cfg)
+    resumed.initialize()
+    resumed.load_checkpoint(tmp_path / "warm.npz")
+    assert [resumed.read_walker(k).info.clock_offset for k in range(eg.walkers)] == offsets
+    sg, so = resumed.run(400), eo.run(800)
+    assert (sg.epoch, sg.attempted, sg.accepted, sg.exchange_accepted) == (
+        so.epoch, so.attempted, so.accepted, so.exchange_accepted)
+    stages = {resumed.read_walker(k).info.stage for k in range(eg.walkers)}
+    assert abi.STAGE_INVERSE_TIME in stages, stages  # the 1/t switch happened after the resume
+    for k in range(eo.walkers):
+        assert_walkers_equal(resumed, eo, k)
