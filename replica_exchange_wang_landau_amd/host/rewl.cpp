@@ -237,7 +237,7 @@ RewlResult run_rewl(std::shared_ptr<const Couplings> couplings, const RunConfig 
                                 supplied.empty() ? nullptr : supplied.data()));
 
     // 1x1 runs keep the reference's WLCHKP3 file (parallel.cpp:354-360, 522-526); other
-    // layouts write a WLCHKM1 file of every walker at the same cadence (walker 0's attempts)
+    // layouts write a WLCHKM2 file of every walker at the same cadence (walker 0's attempts)
     const bool checkpointing = !c.checkpoint_path.empty() && walkers == 1;
     const bool run_checkpointing = !c.checkpoint_path.empty() && walkers > 1;
     if (checkpointing) {

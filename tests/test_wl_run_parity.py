@@ -136,7 +136,7 @@ def test_adaptive_windows_identical(tmp_path, args):
 
 
 def test_multi_walker_checkpoint_resume_identical(tmp_path):
-    """A 2x3 run checkpointed (WLCHKM1: every walker's WLCHKP3 record, the driver's cadence
+    """A 2x3 run checkpointed (WLCHKM2: every walker's WLCHKP3 record, the driver's cadence
     and the exchange counters) at 600 MCS and resumed to 1000 MCS writes the same DOS,
     window, thermodynamics and walker-statistics files as an uninterrupted 1000-MCS run. The
     reference checkpoints 1x1 runs only (parallel.cpp:354,522), so this compares this
@@ -150,7 +150,7 @@ def test_multi_walker_checkpoint_resume_identical(tmp_path):
     first = run(OURS, base + ["--max-mcs", 600, "--checkpoint", "run.chk",
                               "--checkpoint-interval-mcs", 100], a)
     assert first.returncode == 2, first.stderr
-    assert (a / "run.chk").read_bytes()[:8] == b"WLCHKM1\0"
+    assert (a / "run.chk").read_bytes()[:8] == b"WLCHKM2\0"
     resumed = run(OURS, base + ["--max-mcs", 1000, "--checkpoint", "run.chk",
                                 "--checkpoint-interval-mcs", 100], a)
     straight = run(OURS, base + ["--max-mcs", 1000, "--checkpoint-interval-mcs", 100], b)
