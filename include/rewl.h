@@ -45,6 +45,11 @@ enum wlx_rng { WLX_RNG_XOSHIRO = 0, WLX_RNG_PHILOX = 1 };
 /* local-field precision: FP64 (reference arithmetic), or FP32 fields with Kahan-compensated
  * incremental updates over FP32 couplings, resynchronised periodically from an FP64 H = J.S */
 enum wlx_fields { WLX_FIELDS_FP64 = 0, WLX_FIELDS_FP32_KAHAN = 1 };
+/* coupling precision of a system: FP64 as the reference builds it, or every J_ij rounded to the
+ * nearest binary32 value after the FP64 build (CSR values then stream as 4-byte floats).
+ * Fields, energies and log g stay FP64 either way, so an FP32-coupling run is the reference
+ * algorithm on the rounded couplings. */
+enum wlx_couplings { WLX_COUPLINGS_FP64 = 0, WLX_COUPLINGS_FP32 = 1 };
 /* Walker kernel families (all give the same trajectory):
  * AUTO       lookahead for runs without statistics collection;
  * LOOKAHEAD  batches of proposals resolved together, fields in registers (N <= 4096);
@@ -168,10 +173,10 @@ size_t wlx_last_error(char *buffer, size_t capacity);
  * physics.cpp:172-184), cutoff > 0 the CSR backend with norm(d) <= cutoff (CsrCouplings,
  * physics.cpp:197-223). positions/axes are N x 3 row-major; box holds the three box lengths
  * (0 = open axis), periodic as in Box (physics.hpp:17-20). Validates like Geometry::validate
- * (physics.cpp:34-56). */
+ * (physics.cpp:34-56). precision: enum wlx_couplings (the reference has FP64 only). */
 int wlx_system_create(const double *positions, const double *axes, uint64_t n,
                       const double box[3], int periodic, double coupling_scale, double cutoff,
-                      wlx_system **out);
+                      int precision, wlx_system **out);
 int wlx_system_destroy(wlx_system *system);
 /* Couplings::size (physics.hpp:44) and backend_name: 0 = dense, 1 = csr */
 int wlx_system_info(const wlx_system *system, uint64_t *n, uint64_t *backend, uint64_t *nnz);
@@ -234,6 +239,11 @@ int wlx_engine_set_run_state(wlx_engine *engine, uint64_t epoch, uint64_t exchan
                              uint64_t exchange_accepted);
 int wlx_engine_walkers(const wlx_engine *engine, uint64_t *walkers, uint64_t *spins,
                        uint64_t *bins);
+/* Device layout of an engine's walkers: the LDS bytes of one LDS-walker workgroup (fields,
+ * spins and, unless bins_in_hbm, the window bins) and whether the window bins stay in HBM
+ * because the widest window does not fit in LDS (then only the LDS walker samples). No
+ * reference counterpart. */
+int wlx_engine_layout(const wlx_engine *engine, uint64_t *lds_bytes, uint64_t *bins_in_hbm);
 /* WangLandauWalker::snapshot (wang_landau.cpp:415-418). Arrays are full-grid (bins); any may
  * be NULL. */
 int wlx_engine_read_walker(wlx_engine *engine, uint64_t walker, wlx_walker_info *info,

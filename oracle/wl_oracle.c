@@ -190,9 +190,14 @@ static int pair_value(const double *pos, const double *axes, uint64_t i, uint64_
 }
 
 orc_system *orc_system_create(const double *pos, const double *axes, uint64_t n,
-                              const double box[3], int periodic, double scale, double cutoff)
+                              const double box[3], int periodic, double scale, double cutoff,
+                              int precision)
 {
     if (validate_geometry(pos, axes, n, box, periodic) != WLX_OK) return NULL;
+    if (precision != WLX_COUPLINGS_FP64 && precision != WLX_COUPLINGS_FP32) {
+        fail(WLX_INVALID_ARGUMENT, "Unknown coupling precision");
+        return NULL;
+    }
     orc_system *s = calloc(1, sizeof *s);
     s->n = n;
     if (!(cutoff > 0.0)) { /* DenseCouplings, physics.cpp:172-184 */
@@ -204,6 +209,7 @@ orc_system *orc_system_create(const double *pos, const double *axes, uint64_t n,
                     orc_system_destroy(s);
                     return NULL;
                 }
+                if (precision == WLX_COUPLINGS_FP32) v = (double)(float)v;
                 s->dense[i * n + j] = v;
                 s->dense[j * n + i] = v;
             }
@@ -233,7 +239,7 @@ orc_system *orc_system_create(const double *pos, const double *axes, uint64_t n,
                 s->values = realloc(s->values, capacity * sizeof(double));
             }
             s->columns[nnz] = j;
-            s->values[nnz] = v;
+            s->values[nnz] = precision == WLX_COUPLINGS_FP32 ? (double)(float)v : v;
             ++nnz;
         }
         s->offsets[i + 1] = nnz;

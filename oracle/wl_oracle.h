@@ -51,7 +51,8 @@ int64_t orc_grid_index(double emin, double emax, double width, double energy);
 int orc_partition_windows(uint64_t bins, uint64_t count, double overlap, uint64_t *begin_end);
 
 orc_system *orc_system_create(const double *positions, const double *axes, uint64_t n,
-                              const double box[3], int periodic, double scale, double cutoff);
+                              const double box[3], int periodic, double scale, double cutoff,
+                              int precision);
 void orc_system_destroy(orc_system *system);
 int orc_system_info(const orc_system *system, uint64_t *n, uint64_t *backend, uint64_t *nnz);
 int orc_system_read_dense(const orc_system *system, double *out);

@@ -19,6 +19,8 @@ WLX_LOGIC_ERROR = 3
 WLX_OUT_OF_RANGE = 4
 WLX_DEVICE_ERROR = 5
 
+COUPLINGS_FP64 = 0
+COUPLINGS_FP32 = 1
 STAGE_WANG_LANDAU = 0
 STAGE_INVERSE_TIME = 1
 STAGE_FROZEN = 2
@@ -154,6 +156,7 @@ EXPORTS = [
     "wlx_engine_set_graph",
     "wlx_engine_set_run_state",
     "wlx_engine_walkers",
+    "wlx_engine_layout",
     "wlx_engine_read_walker",
     "wlx_engine_read_infos",
     "wlx_engine_write_walker",

@@ -27,6 +27,18 @@ __global__ void narrow_couplings_kernel(const double *J64, float *J32, size_t co
         J32[k] = float(J64[k]);
 }
 
+// FP32 couplings: each value rounded to the nearest binary32 in place (the FP64 array then holds
+// exactly representable FP32 values), with an optional float copy
+__global__ void round_couplings_kernel(double *values, float *copy, size_t count)
+{
+    for (size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x; k < count;
+         k += size_t(gridDim.x) * blockDim.x) {
+        const float v = float(values[k]);
+        values[k] = double(v);
+        if (copy) copy[k] = v;
+    }
+}
+
 int system_j32(wlx_system *s)
 {
     if (s->d_J32) return WLX_OK;
@@ -250,11 +262,13 @@ size_t wlx_last_error(char *buffer, size_t capacity)
 
 int wlx_system_create(const double *positions, const double *axes, uint64_t n,
                       const double box[3], int periodic, double coupling_scale, double cutoff,
-                      wlx_system **out)
+                      int precision, wlx_system **out)
 {
     if (!out || (n && (!positions || !axes)) || !box)
         return set_error(WLX_INVALID_ARGUMENT, "Null argument");
     *out = nullptr;
+    if (precision != WLX_COUPLINGS_FP64 && precision != WLX_COUPLINGS_FP32)
+        return set_error(WLX_INVALID_ARGUMENT, "Unknown coupling precision");
     if (int status = validate_geometry(positions, axes, n, box, periodic)) return status;
     const bool csr = cutoff > 0.0;
     if (n > uint64_t(kWave) * kMaxSitesPerLane)
@@ -266,6 +280,7 @@ int wlx_system_create(const double *positions, const double *axes, uint64_t n,
     s->npl = sites_per_lane(n);
     s->np = uint64_t(s->npl) * kWave;
     s->csr = csr;
+    s->precision = precision;
     REWL_HIP(hipStreamCreate(&s->stream));
 
     // SoA site arrays
@@ -311,9 +326,20 @@ int wlx_system_create(const double *positions, const double *axes, uint64_t n,
                            sites, b, uint32_t(n), cutoff, coupling_scale, s->d_offsets,
                            s->d_columns, s->d_values, coincident);
         REWL_HIP(hipGetLastError());
+        if (precision == WLX_COUPLINGS_FP32 && s->nnz) {
+            REWL_HIP(hipMalloc(&s->d_values32, s->nnz * sizeof(float)));
+            hipLaunchKernelGGL(round_couplings_kernel, dim3(1024), dim3(256), 0, s->stream,
+                               s->d_values, s->d_values32, size_t(s->nnz));
+            REWL_HIP(hipGetLastError());
+        }
         REWL_HIP(hipMemsetAsync(s->d_J64, 0, n * s->np * sizeof(double), s->stream));
         hipLaunchKernelGGL(csr_expand_kernel, dim3(unsigned(n)), dim3(kBuildBlock), 0, s->stream,
                            s->d_offsets, s->d_columns, s->d_values, uint32_t(s->np), s->d_J64);
+        REWL_HIP(hipGetLastError());
+    }
+    if (precision == WLX_COUPLINGS_FP32 && !csr) { // the CSR expansion copied rounded values
+        hipLaunchKernelGGL(round_couplings_kernel, dim3(1024), dim3(256), 0, s->stream, s->d_J64,
+                           static_cast<float *>(nullptr), size_t(n) * s->np);
         REWL_HIP(hipGetLastError());
     }
     int flag = 0;
@@ -335,6 +361,7 @@ int wlx_system_destroy(wlx_system *s)
     (void)hipFree(s->d_offsets);
     (void)hipFree(s->d_columns);
     (void)hipFree(s->d_values);
+    (void)hipFree(s->d_values32);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return WLX_OK;

@@ -24,6 +24,8 @@ int device_error(hipError_t error, const char *what);
 
 // ---- wavefront geometry ------------------------------------------------------------------
 constexpr int kWave = 64;
+// LDS of one CU on gfx950 (160 KiB), the most one workgroup can allocate
+constexpr size_t kLdsBytes = 160 * 1024;
 // Sites per lane in the register-resident field layout: site = k*64 + lane, k < npl.
 constexpr int kMaxSitesPerLane = 64; // N <= 4096 on the register path
 
@@ -48,6 +50,8 @@ struct wlx_system {
     uint64_t *d_offsets = nullptr; // CSR: n+1
     uint32_t *d_columns = nullptr; // CSR: nnz
     double *d_values = nullptr;    // CSR: nnz
+    int precision = WLX_COUPLINGS_FP64;
+    float *d_values32 = nullptr;   // CSR with FP32 couplings: the values as floats (nnz)
     float *d_J32 = nullptr;        // FP32 copy of d_J64 for FP32-field engines (made on demand)
     hipStream_t stream = nullptr;
 };

@@ -86,6 +86,7 @@ struct Args {
     const uint64_t *csr_offsets;
     const uint32_t *csr_columns;
     const double *csr_values;
+    const float *csr_values32; // the same values as floats when the system's couplings are FP32
     const float *J32;   // FP32 couplings (FP32-field engines)
     float2 *fields32;   // FP32 fields as (h, c) pairs, value h - c (FP32-field engines)
     uint32_t n, np;
@@ -110,6 +111,9 @@ struct Args {
     uint64_t exchange_interval, max_attempts, check_interval, force_after, min_visits;
     double final_factor, flatness;
     uint32_t inverse_time, nalivaiko, philox, unlimited;
+    // window bins too wide for LDS: the walker works on its bins in HBM (the LDS walker, the
+    // initialisation and the harness operations), with the same arithmetic
+    uint32_t bins_hbm, pad0;
 };
 
 // one wave of a sampling launch: the engine (index into the launch's Args table) and walker.
@@ -175,6 +179,14 @@ __device__ inline Bins bins_view(unsigned char *smem, uint32_t stride)
 {
     return {reinterpret_cast<double *>(smem), reinterpret_cast<uint64_t *>(smem + 8 * size_t(stride)),
             smem + 16 * size_t(stride), smem + 17 * size_t(stride)};
+}
+
+// walker w's bins in place in HBM (Args::bins_hbm). The lanes of one wave see each other's
+// stores to these addresses in program order (one vector L1 per CU), as with LDS.
+__device__ inline Bins bins_hbm_view(const Args &a, uint32_t w)
+{
+    const size_t base = size_t(w) * a.ws_stride;
+    return {a.logg + base, a.hist + base, a.active + base, a.refine + base};
 }
 
 __device__ inline void load_bins(const Args &a, uint32_t w, uint32_t ws, Bins v)
@@ -534,7 +546,7 @@ __device__ inline void finish_walker_state(const Args &a, uint32_t w, WalkerStat
         }
     }
     __syncthreads();
-    store_bins(a, w, ws, v);
+    if (!a.bins_hbm) store_bins(a, w, ws, v);
     if (lane_id() == 0) {
         a.energy[st.slot] = energy;
         a.state[w] = st;
@@ -684,9 +696,10 @@ __global__ __launch_bounds__(kWave) void sample_kernel(const Args *table,
 // fields and spins of n sites, rounded up to 16 sites
 __host__ __device__ inline uint32_t lds_sites(uint32_t n) { return (n + 15u) & ~15u; }
 
-__host__ __device__ inline size_t lds_walker_bytes(uint32_t ws_stride, uint32_t n)
+__host__ __device__ inline size_t lds_walker_bytes(uint32_t ws_stride, uint32_t n,
+                                                   bool bins_hbm = false)
 {
-    return bins_lds_bytes(ws_stride) + size_t(lds_sites(n)) * 9;
+    return (bins_hbm ? 0 : bins_lds_bytes(ws_stride)) + size_t(lds_sites(n)) * 9;
 }
 
 // up to 64 (column, value) entries of a CSR row, one per lane, loaded ahead of their use
@@ -696,6 +709,13 @@ struct RowAhead {
     double value;
 };
 
+// CSR value k: 8 bytes per entry, or 4 when the system's couplings are FP32 (the float is the
+// rounded coupling exactly, so the field update is the same FP64 fma)
+__device__ inline double csr_value(const Args &a, uint64_t k)
+{
+    return a.csr_values32 ? double(a.csr_values32[k]) : a.csr_values[k];
+}
+
 __device__ inline RowAhead load_row_ahead(const Args &a, uint32_t site)
 {
     RowAhead r;
@@ -703,11 +723,11 @@ __device__ inline RowAhead load_row_ahead(const Args &a, uint32_t site)
     r.end = a.csr_offsets[site + 1];
     const uint64_t k = r.begin + lane_id();
     r.column = k < r.end ? a.csr_columns[k] : 0u;
-    r.value = k < r.end ? a.csr_values[k] : 0.0;
+    r.value = k < r.end ? csr_value(a, k) : 0.0;
     return r;
 }
 
-template <bool PHILOX>
+template <bool PHILOX, bool HBM_BINS>
 __device__ inline void lds_walk(const Args &a, uint32_t w, uint64_t request, uint32_t flags,
                                 unsigned char *smem)
 {
@@ -716,10 +736,11 @@ __device__ inline void lds_walk(const Args &a, uint32_t w, uint64_t request, uin
     WalkerState st = a.state[w];
     if (st.escaped) return;
     const uint32_t ws = st.end - st.begin;
-    Bins v = bins_view(smem, a.ws_stride);
-    double *hf = reinterpret_cast<double *>(smem + bins_lds_bytes(a.ws_stride));
+    // window bins in LDS, or in place in HBM for windows too wide for LDS
+    Bins v = HBM_BINS ? bins_hbm_view(a, w) : bins_view(smem, a.ws_stride);
+    double *hf = reinterpret_cast<double *>(smem + (HBM_BINS ? 0 : bins_lds_bytes(a.ws_stride)));
     int8_t *sp = reinterpret_cast<int8_t *>(hf + lds_sites(a.n));
-    load_bins(a, w, ws, v);
+    if (!HBM_BINS) load_bins(a, w, ws, v);
     const size_t base = size_t(st.slot) * a.np;
     for (uint32_t i = lane; i < a.n; i += kWave) {
         hf[i] = a.fields[base + i];
@@ -807,13 +828,13 @@ __device__ inline void lds_walk(const Args &a, uint32_t w, uint64_t request, uin
                     hf[this_row.column] = fma(m, this_row.value, hf[this_row.column]);
                 for (uint64_t k = this_row.begin + kWave + lane; k < this_row.end; k += kWave) {
                     const uint32_t c = a.csr_columns[k];
-                    hf[c] = fma(m, a.csr_values[k], hf[c]);
+                    hf[c] = fma(m, csr_value(a, k), hf[c]);
                 }
             } else if (a.csr_offsets) {
                 const uint64_t row_end = a.csr_offsets[s + 1];
                 for (uint64_t k = a.csr_offsets[s] + lane; k < row_end; k += kWave) {
                     const uint32_t c = a.csr_columns[k];
-                    hf[c] = fma(m, a.csr_values[k], hf[c]);
+                    hf[c] = fma(m, csr_value(a, k), hf[c]);
                 }
             } else {
                 const double *jrow = a.J + size_t(s) * a.np;
@@ -866,7 +887,10 @@ __global__ __launch_bounds__(kWave) void lds_sample_kernel(const Args *table,
     const BatchSlot slot = slots[blockIdx.x];
     const Args a = table[slot.engine];
     extern __shared__ __align__(16) unsigned char smem[];
-    lds_walk<PHILOX>(a, slot.walker, request, flags, smem);
+    if (a.bins_hbm)
+        lds_walk<PHILOX, true>(a, slot.walker, request, flags, smem);
+    else
+        lds_walk<PHILOX, false>(a, slot.walker, request, flags, smem);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2023,8 +2047,9 @@ __global__ __launch_bounds__(kWave) void init_kernel(Args a, const int8_t *initi
     const uint32_t w = blockIdx.x, lane = lane_id();
     WalkerState st = a.state[w];
     const uint32_t ws = st.end - st.begin;
-    Bins v = bins_view(smem, a.ws_stride);
-    double *scratch = reinterpret_cast<double *>(smem + bins_lds_bytes(a.ws_stride));
+    Bins v = a.bins_hbm ? bins_hbm_view(a, w) : bins_view(smem, a.ws_stride);
+    double *scratch =
+        reinterpret_cast<double *>(smem + (a.bins_hbm ? 0 : bins_lds_bytes(a.ws_stride)));
     for (uint32_t b = lane; b < ws; b += kWave) {
         v.lg[b] = 0.0;
         v.H[b] = 0;
@@ -2172,7 +2197,7 @@ __global__ __launch_bounds__(kWave) void init_kernel(Args a, const int8_t *initi
     st.rng[2] = rng.s2;
     st.rng[3] = rng.s3;
     __syncthreads();
-    store_bins(a, w, ws, v);
+    if (!a.bins_hbm) store_bins(a, w, ws, v);
     store_config<NPL>(a, st.slot, neg, h);
     if (lane == 0) {
         a.energy[st.slot] = energy;
@@ -2403,14 +2428,14 @@ __global__ __launch_bounds__(kWave) void walker_op_kernel(Args a, uint32_t w, ui
     extern __shared__ __align__(16) unsigned char smem[];
     WalkerState st = a.state[w];
     const uint32_t ws = st.end - st.begin;
-    Bins v = bins_view(smem, a.ws_stride);
-    load_bins(a, w, ws, v);
+    Bins v = a.bins_hbm ? bins_hbm_view(a, w) : bins_view(smem, a.ws_stride);
+    if (!a.bins_hbm) load_bins(a, w, ws, v);
     __syncthreads();
     const Stats s = statistics(a, st, v, ws);
     const bool ready = ready_for_iteration(a, st, s);
     if (op == 2 && ready) next_iteration(a, st, v, ws);
     __syncthreads();
-    if (op == 2 && ready) store_bins(a, w, ws, v);
+    if (op == 2 && ready && !a.bins_hbm) store_bins(a, w, ws, v);
     if (lane_id() == 0) {
         if (op == 2 && ready) a.state[w] = st;
         out[0] = s.active;
@@ -2555,6 +2580,8 @@ struct wlx_engine {
     hipStream_t stream = nullptr;
     bool initialized = false;
     size_t lds_sample = 0, lds_init = 0, lds_walker = 0;
+    // windows too wide for LDS: bins stay in HBM and only the LDS walker samples
+    bool bins_hbm = false;
     uint64_t kernel_choice = WLX_KERNEL_AUTO;
     bool timing = false;
     uint64_t timing_every = 1, timing_count = 0; // time launches timing_count % every == 0
@@ -2627,6 +2654,8 @@ struct wlx_engine {
         a.csr_offsets = sys->csr ? sys->d_offsets : nullptr; // null: dense rows
         a.csr_columns = sys->d_columns;
         a.csr_values = sys->d_values;
+        a.csr_values32 = sys->csr ? sys->d_values32 : nullptr;
+        a.bins_hbm = bins_hbm ? 1 : 0;
         a.sq_disp = d_sq;
         a.disp_samples = d_cnt;
         a.sampling = d_sampling;
@@ -2656,7 +2685,8 @@ struct wlx_engine {
         const bool philox = cfg.rng == WLX_RNG_PHILOX;
         // the statistics are recorded by the one-flip-at-a-time kernels
         if (collect) return philox ? kernels.sample_ph_seq : kernels.sample_xo;
-        if (kernel_choice == WLX_KERNEL_LDS) return philox ? kernels.lds_ph : kernels.lds_xo;
+        if (kernel_choice == WLX_KERNEL_LDS || bins_hbm)
+            return philox ? kernels.lds_ph : kernels.lds_xo;
         if (kernel_choice == WLX_KERNEL_SEQUENTIAL)
             return philox ? kernels.sample_ph_seq : kernels.sample_xo;
         const bool common =
@@ -2977,23 +3007,29 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
         e->lds_sample = end;
         e->lds_init = end;
     }
-    if (e->lds_init > 160 * 1024)
-        return set_error(WLX_INVALID_ARGUMENT, "Energy windows too wide for LDS residency");
-    if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.sample_xo),
-                                   e->lds_sample))
-        return status;
-    for (SampleFn fn : {e->kernels.sample_xo_fast, e->kernels.sample_ph, e->kernels.sample_ph_fast,
-                        e->kernels.sample_ph_seq})
-        if (int status = set_lds_limit(reinterpret_cast<const void *>(fn), e->lds_sample))
+    if (e->lds_init > kLdsBytes) {
+        // windows too wide for LDS: the bins stay in HBM, where the initialisation, the LDS
+        // walker and the harness operations work on them in place
+        if (e->collect || c->fields == WLX_FIELDS_FP32_KAHAN)
+            return set_error(WLX_INVALID_ARGUMENT, "Energy windows too wide for LDS residency");
+        e->bins_hbm = true;
+        e->lds_sample = 0;
+        e->lds_init = sys->np * sizeof(double); // total_energy's scratch
+    }
+    if (!e->bins_hbm) {
+        for (SampleFn fn : {e->kernels.sample_xo, e->kernels.sample_xo_fast, e->kernels.sample_ph,
+                            e->kernels.sample_ph_fast, e->kernels.sample_ph_seq})
+            if (int status = set_lds_limit(reinterpret_cast<const void *>(fn), e->lds_sample))
+                return status;
+        if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.sample_f32),
+                                       e->lds_sample))
             return status;
-    if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.sample_f32),
-                                   e->lds_sample))
-        return status;
+    }
     if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.init), e->lds_init))
         return status;
-    { // the LDS walker stages the configuration in LDS beside the bins
-        e->lds_walker = lds_walker_bytes(e->ws_stride, uint32_t(sys->n));
-        if (e->lds_walker <= 160 * 1024)
+    { // the LDS walker stages the configuration in LDS beside the bins (or alone)
+        e->lds_walker = lds_walker_bytes(e->ws_stride, uint32_t(sys->n), e->bins_hbm);
+        if (e->lds_walker <= kLdsBytes)
             for (SampleFn fn : {e->kernels.lds_xo, e->kernels.lds_ph})
                 if (int status = set_lds_limit(reinterpret_cast<const void *>(fn), e->lds_walker))
                     return status;
@@ -3087,6 +3123,14 @@ int wlx_engine_walkers(const wlx_engine *e, uint64_t *walkers, uint64_t *spins, 
     if (walkers) *walkers = e->walkers;
     if (spins) *spins = e->sys->n;
     if (bins) *bins = e->bins;
+    return WLX_OK;
+}
+
+int wlx_engine_layout(const wlx_engine *e, uint64_t *lds_bytes, uint64_t *bins_in_hbm)
+{
+    if (!e) return set_error(WLX_INVALID_ARGUMENT, "Null engine");
+    if (lds_bytes) *lds_bytes = e->lds_walker;
+    if (bins_in_hbm) *bins_in_hbm = e->bins_hbm ? 1 : 0;
     return WLX_OK;
 }
 
@@ -3278,8 +3322,11 @@ int wlx_engine_set_kernel(wlx_engine *e, uint64_t kernel)
 {
     if (int status = require_engine(e, false)) return status;
     if (kernel > WLX_KERNEL_LDS) return set_error(WLX_INVALID_ARGUMENT, "Unknown walker kernel");
-    if (kernel == WLX_KERNEL_LDS && e->lds_walker > 160 * 1024)
+    if (kernel == WLX_KERNEL_LDS && e->lds_walker > kLdsBytes)
         return set_error(WLX_INVALID_ARGUMENT, "Fields and window bins exceed the LDS of a CU");
+    if (e->bins_hbm && kernel != WLX_KERNEL_LDS && kernel != WLX_KERNEL_AUTO)
+        return set_error(WLX_INVALID_ARGUMENT,
+                         "Windows too wide for LDS run on the LDS walker only");
     e->kernel_choice = kernel;
     return WLX_OK;
 }
@@ -3324,8 +3371,9 @@ int wlx_engine_autotune(wlx_engine *e, uint64_t epochs, uint64_t *chosen, double
     if (epochs == 0) return set_error(WLX_INVALID_ARGUMENT, "Autotuning needs at least one epoch");
     if (seconds)
         for (int k = 0; k <= int(WLX_KERNEL_LDS); ++k) seconds[k] = -1.0;
-    std::vector<uint64_t> candidates{WLX_KERNEL_LOOKAHEAD, WLX_KERNEL_SEQUENTIAL};
-    if (e->lds_walker <= 160 * 1024) candidates.push_back(WLX_KERNEL_LDS);
+    std::vector<uint64_t> candidates;
+    if (!e->bins_hbm) candidates = {WLX_KERNEL_LOOKAHEAD, WLX_KERNEL_SEQUENTIAL};
+    if (e->lds_walker <= kLdsBytes) candidates.push_back(WLX_KERNEL_LDS);
     uint64_t best = e->kernel_choice == WLX_KERNEL_AUTO ? WLX_KERNEL_LOOKAHEAD : e->kernel_choice;
     double best_s = -1.0;
     // every variant yields the same trajectory, so the trials are part of the run itself
@@ -3952,7 +4000,7 @@ int wlx_batch_create(wlx_engine *const *engines, uint64_t count, wlx_batch **out
         char name[64];
         if (int status = wlx_engine_kernel(engines[0], nullptr, name, sizeof name)) return status;
         b->kernel_name = name;
-    } else if (b->lds > 160 * 1024) {
+    } else if (b->lds > kLdsBytes) {
         return set_error(WLX_INVALID_ARGUMENT, "Fields and window bins exceed the LDS of a CU");
     } else {
         b->sample = b->philox ? engines[0]->kernels.lds_ph : engines[0]->kernels.lds_xo;

@@ -73,11 +73,12 @@ class Backend:
         self.check(getattr(self.lib, f"{self.prefix}_{name}")(*args))
 
     # constructors differ between backends
-    def system_create(self, positions, axes, n, box, periodic, scale, cutoff):
+    def system_create(self, positions, axes, n, box, periodic, scale, cutoff, precision=0):
         out = ctypes.c_void_p()
         self.call("system_create", _ptr(positions, ctypes.c_double), _ptr(axes, ctypes.c_double),
                   ctypes.c_uint64(n), _ptr(box, ctypes.c_double), ctypes.c_int(periodic),
-                  ctypes.c_double(scale), ctypes.c_double(cutoff), ctypes.byref(out))
+                  ctypes.c_double(scale), ctypes.c_double(cutoff), ctypes.c_int(precision),
+                  ctypes.byref(out))
         return out
 
     def system_destroy(self, handle):
@@ -200,10 +201,12 @@ def simple_cubic(nx: int, ny: int, nz: int, spacing: float = 1.0,
 
 # --------------------------------------------------------------------------------------------
 class System:
-    """Coupling system J_ij on the device (kernel a); dense if cutoff <= 0, else CSR."""
+    """Coupling system J_ij on the device (kernel a); dense if cutoff <= 0, else CSR.
+    precision: abi.COUPLINGS_FP64 (the reference's), or abi.COUPLINGS_FP32 (every coupling
+    rounded to binary32 after the FP64 build)."""
 
     def __init__(self, positions, axes, box, periodic: bool = True, coupling_scale: float = 1.0,
-                 cutoff: float = 0.0, backend: Backend | None = None):
+                 cutoff: float = 0.0, backend: Backend | None = None, precision: int = 0):
         self.backend = backend or default_backend()
         self.positions = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
         self.axes = np.ascontiguousarray(axes, dtype=np.float64).reshape(-1, 3)
@@ -211,9 +214,10 @@ class System:
         self.periodic = bool(periodic)
         self.coupling_scale = float(coupling_scale)
         self.cutoff = float(cutoff)
+        self.precision = int(precision)
         self.handle = self.backend.system_create(self.positions, self.axes, len(self.positions),
                                                  self.box, int(self.periodic),
-                                                 self.coupling_scale, self.cutoff)
+                                                 self.coupling_scale, self.cutoff, self.precision)
         n, backend_id, nnz = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         self.backend.call("system_info", self.handle, ctypes.byref(n), ctypes.byref(backend_id),
                           ctypes.byref(nnz))
@@ -223,9 +227,9 @@ class System:
 
     @classmethod
     def simple_cubic(cls, nx, ny, nz, spacing=1.0, axis=(0.0, 0.0, 1.0), periodic=True,
-                     coupling_scale=1.0, cutoff=0.0, backend=None):
+                     coupling_scale=1.0, cutoff=0.0, backend=None, precision=0):
         pos, axes, box, per = simple_cubic(nx, ny, nz, spacing, axis, periodic)
-        return cls(pos, axes, box, per, coupling_scale, cutoff, backend)
+        return cls(pos, axes, box, per, coupling_scale, cutoff, backend, precision)
 
     def dense(self) -> np.ndarray:
         out = np.empty((self.n, self.n))
@@ -313,6 +317,12 @@ class Engine:
         self.backend.call("engine_walkers", self.handle, ctypes.byref(walkers),
                           ctypes.byref(spins), ctypes.byref(bins))
         self.walkers, self.n, self.bins = walkers.value, spins.value, bins.value
+
+    def layout(self) -> tuple[int, bool]:
+        """(LDS bytes of one LDS-walker workgroup, window bins kept in HBM)."""
+        lds, hbm = ctypes.c_uint64(), ctypes.c_uint64()
+        self.backend.call("engine_layout", self.handle, ctypes.byref(lds), ctypes.byref(hbm))
+        return lds.value, bool(hbm.value)
 
     def initialize(self, initial_spins=None, supplied=None):
         if initial_spins is None:

@@ -24,14 +24,14 @@ void check(int status)
     }
 }
 
-Couplings::Couplings(const Geometry &g, double scale, double cutoff)
+Couplings::Couplings(const Geometry &g, double scale, double cutoff, int precision)
     : n_(g.size()), csr_(cutoff > 0.0)
 {
     g.validate();
     const auto p = g.flat_positions(), a = g.flat_axes();
     const double box[3] = {g.box.x, g.box.y, g.box.z};
     check(wlx_system_create(p.data(), a.data(), g.size(), box, g.periodic ? 1 : 0, scale, cutoff,
-                            &system_));
+                            precision, &system_));
 }
 
 Couplings::~Couplings() { wlx_system_destroy(system_); }

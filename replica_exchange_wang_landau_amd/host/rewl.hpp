@@ -16,10 +16,12 @@
 
 namespace rewl::host {
 
-// Couplings (physics.hpp:41-84) built by kernel (a): dense if cutoff <= 0, else CSR.
+// Couplings (physics.hpp:41-84) built by kernel (a): dense if cutoff <= 0, else CSR; precision
+// is an enum wlx_couplings (FP64 as the reference, or rounded to FP32).
 class Couplings {
   public:
-    Couplings(const Geometry &geometry, double coupling_scale, double cutoff);
+    Couplings(const Geometry &geometry, double coupling_scale, double cutoff,
+              int precision = WLX_COUPLINGS_FP64);
     ~Couplings();
     Couplings(const Couplings &) = delete;
     Couplings &operator=(const Couplings &) = delete;

@@ -43,6 +43,13 @@ int engine_rng_from_environment()
     return (v && std::string(v) == "philox") ? WLX_RNG_PHILOX : WLX_RNG_XOSHIRO;
 }
 
+// REWL_COUPLINGS=fp32: couplings rounded to FP32 after the build (FP64 fields and energies)
+int coupling_precision_from_environment()
+{
+    const char *v = std::getenv("REWL_COUPLINGS");
+    return (v && std::string(v) == "fp32") ? WLX_COUPLINGS_FP32 : WLX_COUPLINGS_FP64;
+}
+
 // REWL_FIELDS=fp32: FP32 Kahan fields (needs REWL_RNG=philox); REWL_RESYNC_MCS: its FP64
 // resync cadence in MCS (default 10)
 void engine_fields_from_environment(EngineOptions &options)
@@ -158,7 +165,8 @@ int main(int argc, char **argv)
             std::cerr << "warning: force_accept_after_mcs=" << config.force_accept.mcs
                       << " enables non-standard forced acceptance and may bias the DOS\n";
         auto couplings =
-            std::make_shared<const Couplings>(geometry, config.coupling_scale, config.cutoff);
+            std::make_shared<const Couplings>(geometry, config.coupling_scale, config.cutoff,
+                                              coupling_precision_from_environment());
         if (config.pilot) { // wl_run.cpp:84-104
             const auto r = pilot_energy_range(*couplings, config.seed, config.max_attempts());
             const double padding = std::max(1e-12, 0.05 * (r.maximum - r.minimum));

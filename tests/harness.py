@@ -49,11 +49,11 @@ class OracleBackend(Backend):
 
         raise _ERRORS.get(status, RuntimeError)(self.last_error())
 
-    def system_create(self, positions, axes, n, box, periodic, scale, cutoff):
+    def system_create(self, positions, axes, n, box, periodic, scale, cutoff, precision=0):
         h = self.lib.orc_system_create(_ptr(positions, ctypes.c_double), _ptr(axes, ctypes.c_double),
                                        ctypes.c_uint64(n), _ptr(box, ctypes.c_double),
                                        ctypes.c_int(periodic), ctypes.c_double(scale),
-                                       ctypes.c_double(cutoff))
+                                       ctypes.c_double(cutoff), ctypes.c_int(precision))
         if not h:
             self._raise()
         return ctypes.c_void_p(h)

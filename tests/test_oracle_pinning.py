@@ -740,3 +740,27 @@ def test_philox_walker_converges_to_exact_enumeration(orc, seed):
     d = np.array([lg[b] - np.log(exact[b]) for b in occupied])
     d -= d.mean()
     assert np.abs(d).max() < 0.02 and np.sqrt(np.mean(d * d)) <= 0.05
+
+
+def test_fp32_couplings_are_the_rounded_fp64_build(orc):
+    """precision = FP32 (no reference counterpart): every coupling of the dense and the CSR
+    build is the FP64 coupling rounded to the nearest binary32 value; the matrix stays
+    symmetric and the CSR pattern is unchanged; an unknown precision is refused."""
+    rng = np.random.default_rng(3)
+    n, side = 150, (150 / 0.3) ** (1 / 3)
+    pos = rng.uniform(0.0, side, size=(n, 3))
+    axes = rng.normal(size=(n, 3))
+    axes /= np.linalg.norm(axes, axis=1)[:, None]
+    box = np.array([side] * 3)
+    for cutoff in (0.0, 3.0):
+        s64 = System(pos, axes, box, True, cutoff=cutoff, backend=orc)
+        s32 = System(pos, axes, box, True, cutoff=cutoff, backend=orc, precision=1)
+        d64, d32 = s64.dense(), s32.dense()
+        assert d32.tobytes() == d64.astype(np.float32).astype(np.float64).tobytes()
+        assert np.array_equal(d32, d32.T) and np.count_nonzero(d32 != d64) > n
+        if cutoff:
+            (o64, c64, v64), (o32, c32, v32) = s64.csr(), s32.csr()
+            assert o64.tolist() == o32.tolist() and c64.tolist() == c32.tolist()
+            assert v32.tobytes() == v64.astype(np.float32).astype(np.float64).tobytes()
+    with pytest.raises(InvalidArgument, match="Unknown coupling precision"):
+        System(pos, axes, box, True, backend=orc, precision=2)
