@@ -56,3 +56,57 @@ def test_window_rms_removes_each_window_mean():
         rms = bench.window_rms([(e, k * e)], w)
         assert rms == pytest.approx(k * w / np.sqrt(12.0), rel=1e-3)
     assert bench.window_rms([(e, np.full_like(e, 0.7))], 0.5) == pytest.approx(0.0, abs=1e-12)
+
+
+def test_partition_matches_partition_windows(orc):
+    """workloads.partition restates partition_windows (wang_landau.cpp:40-61) for explicit
+    windows: same windows as the oracle's for many bin counts, window counts and overlaps."""
+    import ctypes
+
+    from replica_exchange_wang_landau_amd import workloads
+    for bins in (2, 7, 57, 100, 1255, 28490, 99999):
+        for count in (1, 2, 3, 5, 8, 16):
+            if count > bins:
+                continue
+            for overlap in (0.0, 0.5, 0.75, 0.9):
+                be = (ctypes.c_uint64 * (2 * count))()
+                if orc.lib.orc_partition_windows(ctypes.c_uint64(bins), ctypes.c_uint64(count),
+                                                 ctypes.c_double(overlap), be) != 0:
+                    continue  # windows that do not overlap are refused
+                want = [(be[2 * k], be[2 * k + 1]) for k in range(count)]
+                assert workloads.partition(bins, count, overlap) == want, (bins, count, overlap)
+
+
+def test_occupied_windows_skip_gaps():
+    """Energies in two lumps 1000 units apart on a 3000-bin grid: the 8 windows cover the two
+    lumps (plus 10 % of the gap on their inner side) and none lies inside the gap; without a
+    gap the windows are partition_windows over the whole grid."""
+    import numpy as np
+
+    from replica_exchange_wang_landau_amd import workloads
+    rng = np.random.default_rng(1)
+    energies = np.concatenate([rng.uniform(-1500, -500, 500), rng.uniform(500, 1500, 300)])
+    windows = workloads.occupied_windows(-1500.0, 1500.0, 8, energies)
+    assert len(windows) == 8
+    gap_lo, gap_hi = int(energies[energies < 0].max() + 1500), int(energies[energies > 0].min() + 1500)
+    for b, e in windows:
+        assert e <= gap_lo + 0.1 * (gap_hi - gap_lo) + 2 or b >= gap_hi - 0.1 * (gap_hi - gap_lo) - 2
+    assert windows[0][0] == 0 and windows[-1][1] == 3000
+    lower = [w for w in windows if w[1] <= 1500]
+    assert len(lower) == 4  # widths ~1100 and ~1100 bins share 8 windows
+    flat = workloads.occupied_windows(-1500.0, 1500.0, 8, rng.uniform(-1500, 1500, 800))
+    assert flat == workloads.partition(3000, 8, 0.75)
+
+
+def test_occupied_windows_ignore_samples_outside_the_grid():
+    """Sample energies beyond the pilot grid do not create intervals outside it; every window
+    lies inside the grid and is at least two bins wide."""
+    import numpy as np
+
+    from replica_exchange_wang_landau_amd import workloads
+    rng = np.random.default_rng(2)
+    energies = np.concatenate([rng.uniform(-900, -400, 400), rng.uniform(450, 700, 50),
+                               rng.uniform(2000, 2600, 100)])  # a lump above the grid
+    windows = workloads.occupied_windows(-1000.0, 1000.0, 12, energies)
+    assert len(windows) == 12
+    assert all(0 <= b and b + 2 <= e <= 2000 for b, e in windows)

@@ -104,32 +104,46 @@ def test_hbm_bins_bit_exact(gpu, orc, rng_mode):
 
 @pytest.fixture(scope="module")
 def c3_setup(gpu):
-    """The 64 systems with their pilot ranges (one launch for all pilots)."""
+    """The 64 systems, their pilot ranges (one launch for all pilots) and configurations
+    (windows over the occupied energy intervals)."""
     systems = workloads.systems(64, 1234, backend=gpu)
     lo, hi = System.pilot_many(systems, 1234, [PILOT_MCS * s.n for s in systems])
-    return systems, lo, hi
+    configs = [workloads.config(system, a, b, seed=1234 + s, max_attempts=system.n * 5,
+                                energies=workloads.sample_energies(system, 2048, s))
+               for s, (system, a, b) in enumerate(zip(systems, lo, hi))]
+    return systems, configs
 
 
-def c3_engines(systems, lo, hi, epochs):
+def c3_engines(systems, configs):
     engines = []
-    for s, (system, a, b) in enumerate(zip(systems, lo, hi)):
-        e = Engine(system, workloads.config(system, a, b, seed=1234 + s,
-                                           max_attempts=system.n * epochs))
+    for system, cfg in zip(systems, configs):
+        e = Engine(system, cfg)
         e.initialize()
         e.set_kernel(abi.KERNEL_LDS)
         engines.append(e)
     return engines
 
 
+def test_occupied_windows_avoid_energy_gaps(c3_setup):
+    """Systems whose close pairs split the density of states get windows over the occupied
+    intervals; the others get the reference's partition."""
+    systems, configs = c3_setup
+    gapped = [c for c in configs if c.explicit_windows is not None]
+    assert 0 < len(gapped) < len(configs)
+    for c in gapped:
+        assert len(c.explicit_windows) == c.windows
+        assert all(0 <= b < e <= c.bins for b, e in c.explicit_windows)
+
+
 def test_c3_batch_as_configured_equals_engines_alone(c3_setup, orc):
     """64 systems, 5 epochs in one launch each (plus graph replays): every walker of every
     engine equals the same engine run alone; the smallest and largest systems equal the
     oracle on the same FP32 couplings. Some windows live in HBM, some in LDS."""
-    systems, lo, hi = c3_setup
+    systems, configs = c3_setup
     sizes = [s.n for s in systems]
     assert len(systems) == 64 and min(sizes) >= 64 and max(sizes) <= 4096
-    batched = c3_engines(systems, lo, hi, epochs=5)
-    alone = c3_engines(systems, lo, hi, epochs=5)
+    batched = c3_engines(systems, configs)
+    alone = c3_engines(systems, configs)
     layouts = [e.layout()[1] for e in batched]
     assert any(layouts) and not all(layouts), "expected both LDS and HBM bins"
     assert sum(e.walkers for e in batched) == sum(8 * workloads.windows_for(n) for n in sizes)
@@ -151,8 +165,7 @@ def test_c3_batch_as_configured_equals_engines_alone(c3_setup, orc):
         system = systems[idx]
         so = System(system.positions, system.axes, system.box, True, cutoff=3.0, backend=orc,
                     precision=abi.COUPLINGS_FP32)
-        eo = Engine(so, workloads.config(system, lo[idx], hi[idx], seed=1234 + idx,
-                                         max_attempts=system.n * 5))
+        eo = Engine(so, configs[idx])
         eo.initialize()
         so_status = eo.run()
         assert so_status.attempted == batched[idx].status().attempted
