@@ -267,69 +267,31 @@ def c4_leg(epochs: int, warmup: int, resyncs: int):
     }
 
 
-C3_MIN_SEPARATION = 0.8
-
-
-def c3_systems(count: int = 64, seed: int = 1234):
-    """C3: `count` dilute dipole systems. N_s uniform in {64..4096} from default_rng(seed);
-    positions uniform in a periodic cube of side (N/0.3)^(1/3) by random sequential addition
-    with a minimum (minimum-image) separation of 0.8, and easy axes uniform on the unit
-    sphere, from default_rng(seed + s); D = 1; CSR couplings within cutoff 3 (z = 33.5).
-    Without the hard core, uniform placement puts some pairs at r ~ 0.01, whose couplings
-    (~1/r^3, up to ~2e3 at N = 1000) spread the energy over ~10^4 bins of width 1."""
-    import numpy as np
-
-    from replica_exchange_wang_landau_amd import System
-
-    sizes = np.random.default_rng(seed).integers(64, 4097, size=count)
-    systems = []
-    for s, n in enumerate(sizes):
-        rng = np.random.default_rng(seed + s)
-        side = (n / 0.3) ** (1 / 3)
-        pos = np.empty((n, 3))
-        placed = 0
-        while placed < n:
-            p = rng.uniform(0.0, side, size=3)
-            d = pos[:placed] - p
-            d -= side * np.round(d / side)
-            if placed and np.min((d * d).sum(axis=1)) < C3_MIN_SEPARATION ** 2:
-                continue
-            pos[placed] = p
-            placed += 1
-        axes = rng.normal(size=(n, 3))
-        axes /= np.sqrt((axes * axes).sum(axis=1))[:, None]
-        systems.append(System(pos, axes, np.array([side, side, side]), True, cutoff=3.0))
-    return systems
-
-
-def c3_leg(epochs: int, warmup: int, graph: int, pilot_mcs: int = 10_000):
-    """C3 batched study: per system the seed-1234 pilot range (pilot_mcs MCS, all systems in
-    one launch) padded by 5 % and snapped outward to bin 1.0; windows = clamp(round(4 + 12
-    (N - 64) / 4032), 4, 16) at overlap 0.75, 8 walkers per window, exchange every MCS, 1/t to
-    ln f = 1e-7, seed 1234 + s, Philox stream, FP64 fields. Every walker of every system runs
-    in one sampling launch per epoch (LDS walker kernel), epochs replayed from HIP graphs of
-    `graph` epochs."""
-    import math
-
-    import numpy as np
-
-    from replica_exchange_wang_landau_amd import Batch, Engine, RunConfig, System, WlParameters, abi
+def c3_leg(epochs: int, warmup: int, graph: int, pilot_mcs: int = 1000,
+           cadence: str = "equalized"):
+    """C3 batched study (replica_exchange_wang_landau_amd.workloads): 64 systems, N uniform
+    in {64..4096}, uniform random placement at density 0.3, axes uniform on the sphere, FP32
+    CSR couplings within cutoff 3; per system the seed-1234 pilot range (pilot_mcs MCS, all
+    systems in one launch) padded by 5 % on bins of 1.0, clamp(round(4 + 12 (N - 64) / 4032),
+    4, 16) windows over the occupied energy intervals at overlap 0.75, 8 walkers per window,
+    1/t to ln f = 1e-7, seed 1234 + s, Philox stream. The exchange cadence is picked for the
+    batch: "equalized" exchanges every round(4096 / N) MCS, so that every walker makes about
+    4096 attempts per epoch and no wave idles while the largest systems finish theirs; "1"
+    exchanges every MCS. Every walker of every
+    system runs in one sampling launch per epoch (LDS walker; window bins in HBM where a
+    window does not fit in LDS), epochs replayed from HIP graphs of `graph` epochs."""
+    from replica_exchange_wang_landau_amd import Batch, Engine, System, abi, workloads
 
     t_setup = time.perf_counter()
-    systems = c3_systems()
+    systems = workloads.systems(64, 1234)
     lo, hi = System.pilot_many(systems, 1234, [pilot_mcs * s.n for s in systems])
     pilot_s = time.perf_counter() - t_setup
     print(f"c3: pilots of {len(systems)} systems in {pilot_s:.1f} s", file=sys.stderr, flush=True)
     engines = []
     for s, (system, a, b) in enumerate(zip(systems, lo, hi)):
-        n = system.n
-        pad = 0.05 * (b - a)
-        windows = min(16, max(4, round(4 + 12 * (n - 64) / 4032)))
-        cfg = RunConfig(emin=float(math.floor(a - pad)), emax=float(math.ceil(b + pad)),
-                        bin_width=1.0, windows=windows, walkers_per_window=8, overlap=0.75,
-                        seed=1234 + s, exchange_interval_attempts=n, max_attempts=0,
-                        rng=abi.RNG_PHILOX,
-                        wl=WlParameters(final_factor=1e-7, check_interval_attempts=100 * n))
+        mcs = max(1, round(workloads.MAX_SPINS / system.n)) if cadence == "equalized" else 1
+        cfg = workloads.config(system, a, b, seed=1234 + s, exchange_mcs=mcs,
+                               energies=workloads.sample_energies(system, 2048, s))
         e = Engine(system, cfg)
         e.initialize()
         e.set_kernel(abi.KERNEL_LDS)
@@ -339,6 +301,11 @@ def c3_leg(epochs: int, warmup: int, graph: int, pilot_mcs: int = 10_000):
     setup_s = time.perf_counter() - t_setup
     batch = Batch(engines)
     batch.set_graph(graph)
+    # the LDS budget (which windows keep their bins in HBM), picked on the live run
+    budget, budget_seconds = batch.autotune(graph)
+    layout = batch.layout()
+    print(f"c3: LDS budget {budget} B ({budget_seconds}), layout {layout}", file=sys.stderr,
+          flush=True)
     batch.run(warmup)
     before = [e.status() for e in engines]
     t0 = time.perf_counter()
@@ -347,17 +314,23 @@ def c3_leg(epochs: int, warmup: int, graph: int, pilot_mcs: int = 10_000):
     after = [e.status() for e in engines]
     attempted = sum(x.attempted - y.attempted for x, y in zip(after, before))
     accepted = sum(x.accepted - y.accepted for x, y in zip(after, before))
-    # algorithmic bytes: an accepted flip reads its CSR row, z (value 8 B + column 4 B)
-    row_bytes = sum((x.accepted - y.accepted) * (e.system.nnz / e.system.n) * 12.0
+    # algorithmic bytes: an accepted flip reads its CSR row, z (FP32 value 4 B + column 4 B)
+    row_bytes = sum((x.accepted - y.accepted) * (e.system.nnz / e.system.n) * 8.0
                     for x, y, e in zip(after, before, engines))
     walkers = sum(e.walkers for e in engines)
     return {
-        "workload": "C3: 64 dilute dipole systems, N uniform in {64..4096} (seed 1234), density "
-                    "0.3 periodic cube (random sequential addition, minimum separation 0.8), "
-                    "axes uniform on the sphere, CSR cutoff 3 (FP64 values), 4-16 windows x 8 "
-                    "walkers, bin 1.0, exchange every MCS, Philox stream",
+        "workload": "C3: 64 dilute dipole systems, N uniform in {64..4096} (seed 1234), "
+                    "uniform random placement at density 0.3 in a periodic cube, axes uniform "
+                    "on the sphere, CSR cutoff 3 with FP32 couplings, 4-16 windows (over the "
+                    "occupied energy intervals) x 8 walkers, bin 1.0, Philox stream",
+        "exchange_cadence": cadence,
         "systems": len(engines), "walkers": walkers,
         "spins": [int(e.system.n) for e in engines],
+        "gapped_systems": sum(e.config.explicit_windows is not None for e in engines),
+        "lds_budget": budget,
+        "autotune_seconds": dict(zip(("160KiB", "64KiB", "32KiB"), budget_seconds)),
+        "launch_groups": layout["groups"], "bins_in_hbm_systems": layout["bins_in_hbm"],
+        "max_lds_bytes": layout["max_lds_bytes"],
         "attempted_flips_per_s": attempted / seconds, "epochs_timed": ran, "engines_done": done,
         "acceptance": accepted / max(1, attempted),
         "ms_per_epoch": seconds * 1e3 / max(1, ran),
@@ -365,7 +338,7 @@ def c3_leg(epochs: int, warmup: int, graph: int, pilot_mcs: int = 10_000):
         "roofline": {"bound": "hbm", "achieved": row_bytes / seconds / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": row_bytes / seconds / 1e9 / HBM_PEAK_GBS,
-                     "note": "CSR row bytes of accepted flips over the whole epoch time"},
+                     "note": "CSR row bytes (8 per entry) of accepted flips over the epoch time"},
         "setup_seconds": {"pilot": pilot_s, "total": setup_s},
     }
 
@@ -721,6 +694,8 @@ def main():
     ap.add_argument("--c3", action="store_true",
                     help="only run the C3 batched study (64 uneven CSR systems in one launch)")
     ap.add_argument("--graph", type=int, default=16, help="epochs per HIP graph (--c3)")
+    ap.add_argument("--c3-cadence", choices=["equalized", "1"], default="equalized",
+                    help="exchange cadence of the C3 batch (--c3)")
     ap.add_argument("--c4", action="store_true",
                     help="only run the C4 FP32-field precision study (resync kernel, drift)")
     args = ap.parse_args()
@@ -728,7 +703,8 @@ def main():
         print(json.dumps(c5_leg(args.c5_curves, 60_000, 120.0)))
         return
     if args.c3:
-        print(json.dumps(c3_leg(args.steps if args.steps != 3000 else 200, 20, args.graph)))
+        print(json.dumps(c3_leg(args.steps if args.steps != 3000 else 200, 20, args.graph,
+                                cadence=args.c3_cadence)))
         return
     if args.c4:
         print(json.dumps(c4_leg(args.steps if args.steps != 3000 else 200, 20, 10)))

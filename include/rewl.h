@@ -319,6 +319,19 @@ int wlx_batch_run(wlx_batch *batch, uint64_t max_epochs, uint64_t *epochs,
 /* Name of the batch's sampling kernel ("lookahead_fast_kernel<8>", "lds_sample_kernel<true>",
  * as wlx_engine_kernel names them). */
 int wlx_batch_kernel(const wlx_batch *batch, char *name, size_t capacity);
+/* The LDS walker's launches are grouped by the LDS one walker needs (classes of 16, 32, 64,
+ * 96 and 160 KiB), one stream, table and graph per group, so that small systems pack many
+ * walkers per CU; groups run their epochs concurrently. Engines whose fields and window bins
+ * need more than `bytes` of LDS keep their bins in HBM in this batch (default: 160 KiB, only
+ * what cannot fit). Results do not depend on it. */
+int wlx_batch_set_lds_budget(wlx_batch *batch, uint64_t bytes);
+/* Launch groups, engines with bins in HBM, and the largest per-walker LDS of any group. */
+int wlx_batch_layout(const wlx_batch *batch, uint64_t *groups, uint64_t *bins_in_hbm,
+                     uint64_t *max_lds_bytes);
+/* Runs `epochs` epochs of the live batch under each LDS budget of {160, 64, 32} KiB (the
+ * trajectories do not depend on it), keeps the fastest, and writes the seconds per budget to
+ * seconds[3] (-1 = not tried). */
+int wlx_batch_autotune(wlx_batch *batch, uint64_t epochs, uint64_t *chosen, double *seconds);
 /* ---- warm-start network (no reference counterpart) ---------------------------------------
  * A 4-layer, 128-wide MLP, 6 features -> 128 -> 128 -> 128 -> 1 with SiLU hidden
  * activations, that maps (E/N, N, density, dilution, slab thickness, axis mode), scaled by the

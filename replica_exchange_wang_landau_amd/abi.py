@@ -173,6 +173,9 @@ EXPORTS = [
     "wlx_batch_set_graph",
     "wlx_batch_run",
     "wlx_batch_kernel",
+    "wlx_batch_set_lds_budget",
+    "wlx_batch_layout",
+    "wlx_batch_autotune",
     "wlx_mlp_params",
     "wlx_mlp_create",
     "wlx_mlp_destroy",

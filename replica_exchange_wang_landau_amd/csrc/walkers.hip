@@ -3905,21 +3905,24 @@ int wlx_engine_kernel_time(wlx_engine *e, double *sample_ms, uint64_t *launches)
 } // extern "C"
 
 // ---- batches --------------------------------------------------------------------------------
-struct wlx_batch {
-    std::vector<wlx_engine *> engines;
+// A batch runs the epochs of several engines together. Replicas of one register layout share
+// that register kernel in one launch. Any other mix runs the LDS walker, whose engines are
+// split into launch groups by the LDS one walker needs (classes of 16, 32, 64, 96 and 160 KiB):
+// a launch's dynamic LDS is that of its largest walker, so a group of small systems packs many
+// walkers per CU instead of inheriting the largest system's allocation. Each group has its own
+// stream, parameter table, slot list (heaviest systems first) and HIP graph; an engine belongs
+// to one group, so the groups' epochs run concurrently and independently. Engines whose
+// window bins would need more LDS than the batch's budget keep their bins in HBM.
+struct BatchGroup {
+    std::vector<uint32_t> members; // indices into wlx_batch::engines
     Args *d_table = nullptr;
     BatchSlot *d_slots = nullptr;
     uint32_t slots = 0;
     size_t lds = 0;
-    bool philox = false;
-    SampleFn sample = nullptr; // lookahead_fast_kernel<NPL> or the LDS walker
+    SampleFn sample = nullptr;
     hipStream_t stream = nullptr;
-    uint64_t graph_epochs = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    std::string kernel_name;
-    std::vector<Globals> globs;
-    double sample_seconds = 0.0;
 
     int enqueue_epochs(uint64_t count)
     {
@@ -3927,7 +3930,7 @@ struct wlx_batch {
             hipLaunchKernelGGL(sample, dim3(slots), dim3(kWave), lds, stream, d_table, d_slots,
                                uint64_t(0), uint32_t(kGate | kCheck));
             REWL_HIP(hipGetLastError());
-            hipLaunchKernelGGL(batch_exchange_kernel, dim3(uint32_t(engines.size())),
+            hipLaunchKernelGGL(batch_exchange_kernel, dim3(uint32_t(members.size())),
                                dim3(kExchangeBlock), 0, stream, d_table);
             REWL_HIP(hipGetLastError());
         }
@@ -3941,23 +3944,141 @@ struct wlx_batch {
         exec = nullptr;
         graph = nullptr;
     }
+
+    void release()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        release_graph();
+        if (d_table) (void)hipFree(d_table);
+        if (d_slots) (void)hipFree(d_slots);
+        if (stream) (void)hipStreamDestroy(stream);
+        d_table = nullptr;
+        d_slots = nullptr;
+        stream = nullptr;
+    }
 };
 
-namespace {
-int batch_poll(wlx_batch *b, uint64_t &done)
-{
-    for (size_t k = 0; k < b->engines.size(); ++k)
-        REWL_HIP(hipMemcpyAsync(&b->globs[k], b->engines[k]->d_glob, sizeof(Globals),
-                                hipMemcpyDeviceToHost, b->stream));
-    REWL_HIP(hipStreamSynchronize(b->stream));
-    done = 0;
-    for (const Globals &g : b->globs) {
-        if (g.escaped) return set_error(WLX_RUNTIME_ERROR, "Walker escaped its energy window");
-        done += g.done ? 1 : 0;
+struct wlx_batch {
+    std::vector<wlx_engine *> engines;
+    std::vector<BatchGroup> groups;
+    std::vector<uint32_t> group_of; // engine -> group
+    bool philox = false;
+    SampleFn common = nullptr; // the register kernel every engine runs alone, if one
+    size_t common_lds = 0;
+    size_t lds_budget = kLdsBytes;
+    uint64_t graph_epochs = 0;
+    std::string kernel_name;
+    std::vector<Globals> globs;
+
+    ~wlx_batch()
+    {
+        for (BatchGroup &g : groups) g.release();
     }
-    return WLX_OK;
-}
-} // namespace
+
+    int capture_graphs()
+    {
+        for (BatchGroup &g : groups) {
+            g.release_graph();
+            if (graph_epochs == 0) continue;
+            REWL_HIP(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+            const int status = g.enqueue_epochs(graph_epochs);
+            hipGraph_t graph = nullptr;
+            const hipError_t end = hipStreamEndCapture(g.stream, &graph);
+            if (status) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return status;
+            }
+            REWL_HIP(end);
+            g.graph = graph;
+            REWL_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+        }
+        return WLX_OK;
+    }
+
+    // (re)builds the launch groups for the current LDS budget
+    int build_groups()
+    {
+        for (BatchGroup &g : groups) g.release();
+        groups.clear();
+        group_of.assign(engines.size(), 0);
+        std::vector<Args> table(engines.size());
+        std::vector<size_t> need(engines.size());
+        static const size_t kClasses[] = {16 * 1024, 32 * 1024, 64 * 1024, 96 * 1024, kLdsBytes};
+        std::map<size_t, uint32_t> by_class;
+        for (size_t k = 0; k < engines.size(); ++k) {
+            wlx_engine *e = engines[k];
+            table[k] = e->args();
+            size_t cls = 0;
+            if (common) {
+                need[k] = common_lds;
+            } else {
+                const bool hbm = e->bins_hbm || e->lds_walker > lds_budget;
+                table[k].bins_hbm = hbm ? 1 : 0;
+                need[k] = lds_walker_bytes(e->ws_stride, uint32_t(e->sys->n), hbm);
+                if (need[k] > kLdsBytes)
+                    return set_error(WLX_INVALID_ARGUMENT,
+                                     "Fields and window bins exceed the LDS of a CU");
+                for (size_t c : kClasses)
+                    if (need[k] <= c) {
+                        cls = c;
+                        break;
+                    }
+            }
+            auto it = by_class.find(cls);
+            if (it == by_class.end()) {
+                it = by_class.emplace(cls, uint32_t(groups.size())).first;
+                groups.emplace_back();
+            }
+            group_of[k] = it->second;
+            groups[it->second].members.push_back(uint32_t(k));
+        }
+        for (BatchGroup &g : groups) {
+            std::vector<Args> sub;
+            std::vector<std::pair<uint64_t, BatchSlot>> order; // heaviest systems first
+            for (uint32_t m : g.members) {
+                const uint32_t local = uint32_t(sub.size());
+                sub.push_back(table[m]);
+                g.lds = std::max(g.lds, need[m]);
+                for (uint32_t w = 0; w < engines[m]->walkers; ++w)
+                    order.push_back({engines[m]->sys->n, BatchSlot{local, w}});
+            }
+            std::stable_sort(order.begin(), order.end(),
+                             [](const auto &x, const auto &y) { return x.first > y.first; });
+            std::vector<BatchSlot> slots;
+            for (const auto &o : order) slots.push_back(o.second);
+            if (slots.size() >= (size_t(1) << 31))
+                return set_error(WLX_INVALID_ARGUMENT, "Too many walkers");
+            g.slots = uint32_t(slots.size());
+            g.sample = common ? common
+                              : (philox ? engines[0]->kernels.lds_ph : engines[0]->kernels.lds_xo);
+            REWL_HIP(hipStreamCreate(&g.stream));
+            if (int st = device_alloc(&g.d_table, sub.size())) return st;
+            if (int st = device_alloc(&g.d_slots, slots.size())) return st;
+            REWL_HIP(hipMemcpyAsync(g.d_table, sub.data(), sub.size() * sizeof(Args),
+                                    hipMemcpyHostToDevice, g.stream));
+            REWL_HIP(hipMemcpyAsync(g.d_slots, slots.data(), slots.size() * sizeof(BatchSlot),
+                                    hipMemcpyHostToDevice, g.stream));
+            REWL_HIP(hipStreamSynchronize(g.stream));
+            if (int status = set_lds_limit(reinterpret_cast<const void *>(g.sample), g.lds))
+                return status;
+        }
+        return capture_graphs();
+    }
+
+    int poll(uint64_t &done)
+    {
+        for (size_t k = 0; k < engines.size(); ++k)
+            REWL_HIP(hipMemcpyAsync(&globs[k], engines[k]->d_glob, sizeof(Globals),
+                                    hipMemcpyDeviceToHost, groups[group_of[k]].stream));
+        for (BatchGroup &g : groups) REWL_HIP(hipStreamSynchronize(g.stream));
+        done = 0;
+        for (const Globals &g : globs) {
+            if (g.escaped) return set_error(WLX_RUNTIME_ERROR, "Walker escaped its energy window");
+            done += g.done ? 1 : 0;
+        }
+        return WLX_OK;
+    }
+};
 
 extern "C" {
 
@@ -3967,8 +4088,6 @@ int wlx_batch_create(wlx_engine *const *engines, uint64_t count, wlx_batch **out
     *out = nullptr;
     if (count > (uint64_t(1) << 20)) return set_error(WLX_INVALID_ARGUMENT, "Too many engines");
     auto b = std::make_unique<wlx_batch>();
-    std::vector<Args> table;
-    std::vector<std::pair<uint64_t, BatchSlot>> order; // heaviest systems first
     // every engine on one register kernel (one family and register layout): that kernel runs
     // the batch; any other mix runs the LDS walker (the trajectories are the same)
     SampleFn common = engines[0] ? engines[0]->sample_fn() : nullptr;
@@ -3987,54 +4106,26 @@ int wlx_batch_create(wlx_engine *const *engines, uint64_t count, wlx_batch **out
                              "Batched engines run FP64 fields without statistics or traces");
         if (e->sample_fn() != common) common = nullptr;
         REWL_HIP(hipStreamSynchronize(e->stream));
-        table.push_back(e->args());
-        b->lds = std::max(b->lds, e->lds_walker);
         common_lds = std::max(common_lds, e->lds_sample);
-        for (uint32_t w = 0; w < e->walkers; ++w)
-            order.push_back({e->sys->n, BatchSlot{uint32_t(k), w}});
         b->engines.push_back(e);
     }
     if (common && common != engines[0]->kernels.lds_ph && common != engines[0]->kernels.lds_xo) {
-        b->sample = common;
-        b->lds = common_lds;
+        b->common = common;
+        b->common_lds = common_lds;
         char name[64];
         if (int status = wlx_engine_kernel(engines[0], nullptr, name, sizeof name)) return status;
         b->kernel_name = name;
-    } else if (b->lds > kLdsBytes) {
-        return set_error(WLX_INVALID_ARGUMENT, "Fields and window bins exceed the LDS of a CU");
     } else {
-        b->sample = b->philox ? engines[0]->kernels.lds_ph : engines[0]->kernels.lds_xo;
         b->kernel_name = b->philox ? "lds_sample_kernel<true>" : "lds_sample_kernel<false>";
     }
-    std::stable_sort(order.begin(), order.end(),
-                     [](const auto &x, const auto &y) { return x.first > y.first; });
-    std::vector<BatchSlot> slots;
-    for (const auto &o : order) slots.push_back(o.second);
-    if (slots.size() >= (size_t(1) << 31)) return set_error(WLX_INVALID_ARGUMENT, "Too many walkers");
-    b->slots = uint32_t(slots.size());
     b->globs.resize(count);
-    REWL_HIP(hipStreamCreate(&b->stream));
-    if (int st = device_alloc(&b->d_table, count)) return st;
-    if (int st = device_alloc(&b->d_slots, slots.size())) return st;
-    REWL_HIP(hipMemcpyAsync(b->d_table, table.data(), count * sizeof(Args), hipMemcpyHostToDevice,
-                            b->stream));
-    REWL_HIP(hipMemcpyAsync(b->d_slots, slots.data(), slots.size() * sizeof(BatchSlot),
-                            hipMemcpyHostToDevice, b->stream));
-    REWL_HIP(hipStreamSynchronize(b->stream));
-    if (int status = set_lds_limit(reinterpret_cast<const void *>(b->sample), b->lds))
-        return status;
+    if (int status = b->build_groups()) return status;
     *out = b.release();
     return WLX_OK;
 }
 
 int wlx_batch_destroy(wlx_batch *b)
 {
-    if (!b) return WLX_OK;
-    if (b->stream) (void)hipStreamSynchronize(b->stream);
-    b->release_graph();
-    if (b->d_table) (void)hipFree(b->d_table);
-    if (b->d_slots) (void)hipFree(b->d_slots);
-    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return WLX_OK;
 }
@@ -4044,20 +4135,32 @@ int wlx_batch_set_graph(wlx_batch *b, uint64_t epochs_per_graph)
     if (!b) return set_error(WLX_INVALID_ARGUMENT, "Null batch");
     if (epochs_per_graph > 4096)
         return set_error(WLX_INVALID_ARGUMENT, "At most 4096 epochs per graph");
-    b->release_graph();
     b->graph_epochs = epochs_per_graph;
-    if (epochs_per_graph == 0) return WLX_OK;
-    REWL_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    const int status = b->enqueue_epochs(epochs_per_graph);
-    hipGraph_t graph = nullptr;
-    const hipError_t end = hipStreamEndCapture(b->stream, &graph);
-    if (status) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return status;
-    }
-    REWL_HIP(end);
-    b->graph = graph;
-    REWL_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return b->capture_graphs();
+}
+
+int wlx_batch_set_lds_budget(wlx_batch *b, uint64_t bytes)
+{
+    if (!b) return set_error(WLX_INVALID_ARGUMENT, "Null batch");
+    if (bytes == 0 || bytes > kLdsBytes)
+        return set_error(WLX_INVALID_ARGUMENT, "The LDS budget must lie in (0, 160 KiB]");
+    for (BatchGroup &g : b->groups) REWL_HIP(hipStreamSynchronize(g.stream));
+    b->lds_budget = bytes;
+    return b->build_groups();
+}
+
+int wlx_batch_layout(const wlx_batch *b, uint64_t *groups, uint64_t *bins_in_hbm,
+                     uint64_t *max_lds_bytes)
+{
+    if (!b) return set_error(WLX_INVALID_ARGUMENT, "Null batch");
+    uint64_t hbm = 0;
+    size_t lds = 0;
+    for (const BatchGroup &g : b->groups) lds = std::max(lds, g.lds);
+    if (!b->common)
+        for (const wlx_engine *e : b->engines) hbm += (e->bins_hbm || e->lds_walker > b->lds_budget);
+    if (groups) *groups = b->groups.size();
+    if (bins_in_hbm) *bins_in_hbm = hbm;
+    if (max_lds_bytes) *max_lds_bytes = lds;
     return WLX_OK;
 }
 
@@ -4072,24 +4175,59 @@ int wlx_batch_run(wlx_batch *b, uint64_t max_epochs, uint64_t *epochs, uint64_t 
 {
     if (!b) return set_error(WLX_INVALID_ARGUMENT, "Null batch");
     uint64_t ran = 0, done = 0;
-    if (int status = batch_poll(b, done)) return status;
+    if (int status = b->poll(done)) return status;
+    // one graph replay, or a growing chunk of plain launches, between polls of the engines'
+    // flags: engines that finish stop on the device, and the call returns soon after the last
     uint64_t chunk = 1;
-    const auto t0 = std::chrono::steady_clock::now();
+    const bool graphs = b->graph_epochs > 0 && !b->groups.empty() && b->groups[0].exec;
     while (ran < max_epochs && done < b->engines.size()) {
-        if (b->exec && max_epochs - ran >= b->graph_epochs) {
-            REWL_HIP(hipGraphLaunch(b->exec, b->stream));
-            ran += b->graph_epochs;
+        uint64_t n;
+        if (graphs && max_epochs - ran >= b->graph_epochs) {
+            n = b->graph_epochs;
+            for (BatchGroup &g : b->groups) REWL_HIP(hipGraphLaunch(g.exec, g.stream));
         } else {
-            const uint64_t n = std::min(max_epochs - ran, chunk);
-            if (int status = b->enqueue_epochs(n)) return status;
-            ran += n;
+            n = std::min(max_epochs - ran, chunk);
+            for (BatchGroup &g : b->groups)
+                if (int status = g.enqueue_epochs(n)) return status;
             chunk = std::min<uint64_t>(chunk * 2, 64);
         }
-        if (int status = batch_poll(b, done)) return status;
+        ran += n;
+        if (int status = b->poll(done)) return status;
     }
-    b->sample_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (epochs) *epochs = ran;
     if (done_engines) *done_engines = done;
+    return WLX_OK;
+}
+
+int wlx_batch_autotune(wlx_batch *b, uint64_t epochs, uint64_t *chosen, double *seconds)
+{
+    if (!b) return set_error(WLX_INVALID_ARGUMENT, "Null batch");
+    if (epochs == 0) return set_error(WLX_INVALID_ARGUMENT, "Autotuning needs at least one epoch");
+    static const uint64_t kBudgets[] = {kLdsBytes, 64 * 1024, 32 * 1024};
+    uint64_t best = b->lds_budget;
+    double best_s = -1.0;
+    for (int k = 0; k < 3; ++k) {
+        if (seconds) seconds[k] = -1.0;
+        if (b->common) continue; // the register kernels hold no bins in LDS to trade
+        if (int status = wlx_batch_set_lds_budget(b, kBudgets[k])) return status;
+        uint64_t ran = 0, done = 0;
+        // the first epochs load the kernels; every variant continues the same trajectories
+        if (int status = wlx_batch_run(b, b->graph_epochs ? b->graph_epochs : 1, &ran, &done))
+            return status;
+        if (done == b->engines.size()) break;
+        const auto t0 = std::chrono::steady_clock::now();
+        if (int status = wlx_batch_run(b, epochs, &ran, &done)) return status;
+        const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (seconds) seconds[k] = s;
+        if (best_s < 0.0 || s < best_s) {
+            best_s = s;
+            best = kBudgets[k];
+        }
+        if (done == b->engines.size()) break;
+    }
+    if (!b->common && best != b->lds_budget)
+        if (int status = wlx_batch_set_lds_budget(b, best)) return status;
+    if (chosen) *chosen = best;
     return WLX_OK;
 }
 

@@ -597,6 +597,25 @@ class Batch:
                           ctypes.byref(epochs), ctypes.byref(done))
         return epochs.value, done.value
 
+    def set_lds_budget(self, nbytes: int):
+        """Engines needing more LDS than `nbytes` keep their window bins in HBM."""
+        self.backend.call("batch_set_lds_budget", self.handle, ctypes.c_uint64(nbytes))
+
+    def layout(self) -> dict:
+        """Launch groups, engines with bins in HBM, largest per-walker LDS."""
+        g, h, l = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        self.backend.call("batch_layout", self.handle, ctypes.byref(g), ctypes.byref(h),
+                          ctypes.byref(l))
+        return {"groups": g.value, "bins_in_hbm": h.value, "max_lds_bytes": l.value}
+
+    def autotune(self, epochs: int) -> tuple[int, list[float]]:
+        """(chosen LDS budget, seconds per budget of 160, 64 and 32 KiB)."""
+        chosen = ctypes.c_uint64()
+        seconds = (ctypes.c_double * 3)()
+        self.backend.call("batch_autotune", self.handle, ctypes.c_uint64(epochs),
+                          ctypes.byref(chosen), seconds)
+        return chosen.value, list(seconds)
+
     def kernel(self) -> str:
         """Name of the sampling kernel the batch launches."""
         name = ctypes.create_string_buffer(64)

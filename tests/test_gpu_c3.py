@@ -171,3 +171,35 @@ def test_c3_batch_as_configured_equals_engines_alone(c3_setup, orc):
         assert so_status.attempted == batched[idx].status().attempted
         for k in range(eo.walkers):
             assert_walkers_equal(batched[idx], eo, k)
+
+
+def test_lds_budget_and_autotune_leave_trajectories_unchanged(c3_setup):
+    """Launch groups by LDS class, an LDS budget that moves more windows' bins to HBM, and the
+    batch autotuner trying budgets on the live run all give the trajectories of plain
+    launches of the engines alone."""
+    import dataclasses
+
+    systems, configs = c3_setup
+    pick = [int(i) for i in np.argsort([s.n for s in systems])[::5]]  # 13 sizes, small to large
+    sub_sys = [systems[i] for i in pick]
+    sub_cfg = [dataclasses.replace(configs[i], max_attempts=systems[i].n * 40) for i in pick]
+    tight, tuned, alone = (c3_engines(sub_sys, sub_cfg) for _ in range(3))
+    bt = Batch(tight)
+    default_layout = bt.layout()
+    bt.set_lds_budget(32 * 1024)
+    layout = bt.layout()
+    assert layout["bins_in_hbm"] > default_layout["bins_in_hbm"]
+    assert layout["max_lds_bytes"] <= 48 * 1024 and layout["groups"] >= 2
+    bu = Batch(tuned)
+    bu.set_graph(2)
+    chosen, seconds = bu.autotune(2)
+    assert chosen in (160 * 1024, 64 * 1024, 32 * 1024) and all(s > 0 for s in seconds)
+    ran = tuned[0].status().epoch
+    assert ran == 3 * (2 + 2)
+    assert bt.run(ran) == (ran, 0)
+    for e in alone:
+        e.run(ran)
+    for x, y, z in zip(tight, tuned, alone):
+        for k in range(x.walkers):
+            assert_walkers_equal(x, z, k)
+            assert_walkers_equal(y, z, k)
