@@ -57,6 +57,7 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "tests"))
 
 HBM_PEAK_GBS = 8000.0
+CLOCK_GHZ = 2.4  # MI355X peak engine clock, for cycles per attempt
 # HIP events bracket every TIMING_EVERY-th sampling launch of the timed region: the mean
 # launch duration for the roofline, without an event pair around every launch of the rate
 TIMING_EVERY = 8
@@ -204,10 +205,17 @@ def cpu_leg(mcs_per_walker: int):
         seconds = time.perf_counter() - t0
         attempted = st.attempted
         kind = "port"
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
     return {"value": attempted / seconds, "unit": "attempted flips/s", "cores": threads,
+            "host_cpus": os.cpu_count(), "usable_cpus": usable,
             "kind": kind,
             "sample": f"run_rewl on the same workload, {mcs_per_walker} MCS per walker "
-                      f"({attempted} attempts incl. walker init) in {seconds:.2f} s"}
+                      f"({attempted} attempts incl. walker init) in {seconds:.2f} s, OpenMP on "
+                      f"{threads} of the host's {os.cpu_count()} CPUs ({usable} usable by "
+                      f"this process; at most 16 threads, the box's share)"}
 
 
 def c4_system():
@@ -775,6 +783,15 @@ def main():
             "kernel": kernel,
             "kernel_ms": g["kernel_ms"],
             "bytes_per_launch": g["per_launch_bytes"],
+            # `achieved` prices the algorithmic row stream (a 2 MiB J that stays in L2); the
+            # counter-measured HBM bytes over the same launch time are the HBM use proper.
+            # What bounds the kernel is each walker's serial decision chain: cycles per attempt
+            # of one walker over the launch (512 attempts per walker per launch)
+            "hbm_measured_gbs": (traffic_bytes(kernel) / (g["kernel_ms"] * 1e-3) / 1e9
+                                 if traffic_bytes(kernel) and g["kernel_ms"] > 0 else None),
+            "row_bytes_note": "algorithmic row bytes, served from L2 (J is 2 MiB)",
+            "cycles_per_attempt": g["kernel_ms"] * 1e-3 * CLOCK_GHZ * 1e9 / 512.0,
+            "clock_ghz": CLOCK_GHZ,
         },
         "cpu_baseline": cpu,
         "time_to_accuracy": tta,
