@@ -189,6 +189,15 @@ int wlx_system_read_csr(const wlx_system *system, uint64_t *offsets, uint64_t *c
  * in the reference's summation order. fields may be NULL. */
 int wlx_system_fields(const wlx_system *system, const int8_t *spins, uint64_t count,
                       double *fields, double *energies);
+/* The same computation as wlx_system_fields without any device allocation. No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops). Staging goes through a fixed 4 MiB device scratch array (one
+ * per GPU, held until the process ends): count * (padded N) * 9 bytes + 8 bytes per
+ * configuration must fit (e.g. up to 900 configurations of N = 512, 110 of N = 4096), larger
+ * requests fail with WLX_INVALID_ARGUMENT. Calls that overlap on one GPU share the scratch
+ * and race; the system must live on the current device. */
+int wlx_system_fields_noalloc(const wlx_system *system, const int8_t *spins, uint64_t count,
+                              double *fields, double *energies);
 /* The wl_run --pilot sweep (wl_run.cpp:84-104): from all spins +1, `attempts` unconditional
  * single-spin flips drawn from Xoshiro256**(seed, 0), tracking the sampled energy range. */
 int wlx_system_pilot(const wlx_system *system, uint64_t seed, uint64_t attempts,

@@ -142,6 +142,7 @@ EXPORTS = [
     "wlx_system_read_dense",
     "wlx_system_read_csr",
     "wlx_system_fields",
+    "wlx_system_fields_noalloc",
     "wlx_system_pilot",
     "wlx_systems_pilot",
     "wlx_device_info",

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 
 namespace rewl {
 
@@ -248,7 +249,78 @@ struct DeviceBuffer {
 
 using namespace rewl;
 
+// ---- scratch of the no-allocation fields entry point -------------------------------------
+// Staging for wlx_system_fields_noalloc: spins (np bytes), fields (np doubles) and the energy
+// of each configuration. Each GPU holds its own copy until the process ends; its device
+// address is taken once per device (hipGetSymbolAddress) and only that address reaches kernels
+// and copies. Calls that overlap on one GPU share it and race.
+constexpr size_t kNoAllocScratchBytes = size_t(4) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_guard;
+unsigned char *g_scratch[kMaxDevices] = {};
+
+// the device address of `scratch` on the current device, looked up once per device
+int scratch_address(int device, unsigned char **out)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return set_error(WLX_DEVICE_ERROR, "Device index out of range for the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_guard);
+    if (!g_scratch[device]) {
+        void *ptr = nullptr;
+        REWL_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch[device] = static_cast<unsigned char *>(ptr);
+    }
+    *out = g_scratch[device];
+    return WLX_OK;
+}
+
+size_t round_up_256(size_t bytes) { return (bytes + 255) / 256 * 256; }
+} // namespace
+
 extern "C" {
+
+int wlx_system_fields_noalloc(const wlx_system *s, const int8_t *spins, uint64_t count,
+                              double *fields, double *energies)
+{
+    if (!s || !spins || !energies) return set_error(WLX_INVALID_ARGUMENT, "Null argument");
+    if (count == 0) return WLX_OK;
+    int device = 0;
+    REWL_HIP(hipGetDevice(&device));
+    if (device != s->device)
+        return set_error(WLX_INVALID_ARGUMENT, "The system lives on another device");
+    const size_t spin_bytes = round_up_256(count * s->np);
+    const size_t field_bytes = round_up_256(count * s->np * sizeof(double));
+    const size_t energy_bytes = count * sizeof(double);
+    if (count > kNoAllocScratchBytes || spin_bytes + field_bytes + energy_bytes > kNoAllocScratchBytes)
+        return set_error(WLX_INVALID_ARGUMENT,
+                         "Configurations exceed the 4 MiB scratch of wlx_system_fields_noalloc "
+                         "(count * padded spins * 9 bytes + 8 bytes per configuration)");
+    for (uint64_t k = 0; k < count * s->n; ++k)
+        if (spins[k] != 1 && spins[k] != -1)
+            return set_error(WLX_INVALID_ARGUMENT, "Spins must be +/-1");
+    unsigned char *base = nullptr;
+    if (int status = scratch_address(device, &base)) return status;
+    int8_t *d_spins = reinterpret_cast<int8_t *>(base);
+    double *d_fields = reinterpret_cast<double *>(base + spin_bytes);
+    double *d_energy = reinterpret_cast<double *>(base + spin_bytes + field_bytes);
+    // spins into rows of np (the padding sites are never read: couplings stop at N)
+    REWL_HIP(hipMemcpy2DAsync(d_spins, s->np, spins, s->n, s->n, count, hipMemcpyHostToDevice,
+                              s->stream));
+    auto *ms = const_cast<wlx_system *>(s);
+    if (int status = system_fields_device(ms, d_spins, count, d_fields, d_energy, s->stream))
+        return status;
+    if (fields)
+        REWL_HIP(hipMemcpy2DAsync(fields, s->n * sizeof(double), d_fields,
+                                  s->np * sizeof(double), s->n * sizeof(double), count,
+                                  hipMemcpyDeviceToHost, s->stream));
+    REWL_HIP(hipMemcpyAsync(energies, d_energy, count * sizeof(double), hipMemcpyDeviceToHost,
+                            s->stream));
+    REWL_HIP(hipStreamSynchronize(s->stream));
+    return WLX_OK;
+}
 
 size_t wlx_last_error(char *buffer, size_t capacity)
 {
