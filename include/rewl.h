@@ -302,10 +302,15 @@ int wlx_engine_warm_start(wlx_engine *engine, const double *log_g, double factor
  * (wang_landau.cpp:244-288). set_kernel forces a family (enum wlx_walker_kernel); kernel
  * reports the family and kernel name the next sampling launch uses; autotune runs `epochs`
  * epochs of the live run with each eligible family (the trajectory does not depend on it),
- * keeps the fastest and writes the seconds per family to seconds[4] (-1 = not tried). */
+ * keeps the fastest and writes the seconds per family to seconds[4] (-1 = not tried). The
+ * Philox lookahead family is tried at widths 16, 8 and 32 (wlx_engine_set_lookahead_width). */
 int wlx_engine_set_kernel(wlx_engine *engine, uint64_t kernel);
 int wlx_engine_kernel(const wlx_engine *engine, uint64_t *kernel, char *name, size_t capacity);
 int wlx_engine_autotune(wlx_engine *engine, uint64_t epochs, uint64_t *chosen, double *seconds);
+/* Lookahead width of the Philox common-configuration kernel: attempts resolved per batch and
+ * rows streamed together after it (4, 8, 16 = default, 32; 32 only for N <= 1024). Every
+ * width gives the same trajectory. No reference counterpart. */
+int wlx_engine_set_lookahead_width(wlx_engine *engine, uint64_t width);
 /* ---- batches of engines (uneven systems in one launch) -------------------------------------
  * No reference counterpart (the reference runs one system per process). A batch runs the
  * epochs of several initialized engines together: per epoch one sampling launch over every

@@ -169,6 +169,7 @@ EXPORTS = [
     "wlx_engine_set_kernel",
     "wlx_engine_kernel",
     "wlx_engine_autotune",
+    "wlx_engine_set_lookahead_width",
     "wlx_batch_create",
     "wlx_batch_destroy",
     "wlx_batch_set_graph",

@@ -1216,12 +1216,12 @@ __device__ inline double batch_factors(uint32_t stage, double factor, uint32_t a
     return double(active) / double(t);
 }
 
-template <int NPL>
+template <int NPL, int KB>
 __device__ __attribute__((always_inline)) inline void fast_walk(const Args &a, uint32_t w,
                                                                 uint64_t request, uint32_t flags,
                                                                 unsigned char *smem)
 {
-    constexpr int KB = kLookaheadBatch;
+    static_assert(KB == 4 || KB == 8 || KB == 16 || KB == 32, "lookahead widths 4, 8, 16, 32");
     const uint32_t lane = lane_id();
     if ((flags & kGate) && a.glob->done) return;
     WalkerState st = a.state[w];
@@ -1396,7 +1396,9 @@ __device__ __attribute__((always_inline)) inline void fast_walk(const Args &a, u
     finish_walker<NPL>(a, w, st, v, ws, neg, h, energy, flags);
 }
 
-template <int NPL>
+// KB: attempts resolved per batch (the lookahead width), which is also the number of
+// accepted rows streamed together after a batch
+template <int NPL, int KB>
 __global__ __launch_bounds__(kWave) void lookahead_fast_kernel(const Args *table,
                                                   const BatchSlot *slots,
                                                   uint64_t request, uint32_t flags)
@@ -1404,7 +1406,7 @@ __global__ __launch_bounds__(kWave) void lookahead_fast_kernel(const Args *table
     const BatchSlot slot = slots[blockIdx.x];
     const Args a = table[slot.engine];
     extern __shared__ __align__(16) unsigned char smem[];
-    fast_walk<NPL>(a, slot.walker, request, flags, smem);
+    fast_walk<NPL, KB>(a, slot.walker, request, flags, smem);
 }
 
 
@@ -2455,8 +2457,13 @@ using InitFn = void (*)(Args, const int8_t *, const uint8_t *, InitParams, InitR
 using FieldsFn = void (*)(Args, const int8_t *, double *, double *);
 using PilotFn = void (*)(Args, uint64_t, uint64_t, double *);
 
+// lookahead widths of the Philox common-configuration kernel, and the index of the default
+constexpr uint32_t kLookaheadWidths[] = {4, 8, 16, 32};
+constexpr int kDefaultWidth = 2;
+
 struct KernelSet {
-    SampleFn sample_xo, sample_xo_fast, sample_ph, sample_ph_fast, sample_ph_seq;
+    SampleFn sample_xo, sample_xo_fast, sample_ph, sample_ph_seq;
+    SampleFn ph_fast_width[4]; // lookahead_fast_kernel<NPL, kLookaheadWidths[k]>
     SampleFn lds_xo, lds_ph; // any N
     F32Fn sample_f32;
     InitFn init;
@@ -2467,7 +2474,9 @@ struct KernelSet {
 template <int NPL> KernelSet kernel_set()
 {
     return {sample_kernel<NPL, false>, lookahead_xo_kernel<NPL>, lookahead_kernel<NPL>,
-            lookahead_fast_kernel<NPL>, sample_kernel<NPL, true>,
+            sample_kernel<NPL, true>,
+            {lookahead_fast_kernel<NPL, 4>, lookahead_fast_kernel<NPL, 8>,
+             lookahead_fast_kernel<NPL, 16>, lookahead_fast_kernel<NPL, 32>},
             lds_sample_kernel<false>, lds_sample_kernel<true>, lookahead_f32_kernel<NPL>,
             init_kernel<NPL>, fields_kernel<NPL>, pilot_kernel<NPL>};
 }
@@ -2583,6 +2592,7 @@ struct wlx_engine {
     // windows too wide for LDS: bins stay in HBM and only the LDS walker samples
     bool bins_hbm = false;
     uint64_t kernel_choice = WLX_KERNEL_AUTO;
+    int width_index = kDefaultWidth; // lookahead width of the Philox common-configuration kernel
     bool timing = false;
     uint64_t timing_every = 1, timing_count = 0; // time launches timing_count % every == 0
     std::vector<hipEvent_t> events;
@@ -2693,7 +2703,7 @@ struct wlx_engine {
             trace_cap == 0 && cfg.wl.force_accept_after_attempts == 0 && !cfg.wl.nalivaiko_mod;
         if (cfg.rng != WLX_RNG_PHILOX) return common ? kernels.sample_xo_fast : kernels.sample_xo;
         if (trace_cap == 0 && cfg.wl.force_accept_after_attempts == 0 && !cfg.wl.nalivaiko_mod)
-            return kernels.sample_ph_fast;
+            return kernels.ph_fast_width[width_index];
         return kernels.sample_ph;
     }
 
@@ -3018,7 +3028,9 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
     }
     if (!e->bins_hbm) {
         for (SampleFn fn : {e->kernels.sample_xo, e->kernels.sample_xo_fast, e->kernels.sample_ph,
-                            e->kernels.sample_ph_fast, e->kernels.sample_ph_seq})
+                            e->kernels.sample_ph_seq, e->kernels.ph_fast_width[0],
+                            e->kernels.ph_fast_width[1], e->kernels.ph_fast_width[2],
+                            e->kernels.ph_fast_width[3]})
             if (int status = set_lds_limit(reinterpret_cast<const void *>(fn), e->lds_sample))
                 return status;
         if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.sample_f32),
@@ -3346,7 +3358,8 @@ int wlx_engine_kernel(const wlx_engine *e, uint64_t *kernel, char *name, size_t 
         base = "lds_sample_kernel";
     } else if (fn == k.sample_xo_fast) {
         base = "lookahead_xo_kernel";
-    } else if (fn == k.sample_ph_fast) {
+    } else if (fn == k.ph_fast_width[0] || fn == k.ph_fast_width[1] || fn == k.ph_fast_width[2] ||
+               fn == k.ph_fast_width[3]) {
         base = "lookahead_fast_kernel";
     } else if (fn == nullptr) { // FP32 fields
         base = "lookahead_f32_kernel";
@@ -3359,10 +3372,30 @@ int wlx_engine_kernel(const wlx_engine *e, uint64_t *kernel, char *name, size_t 
         else if (family == WLX_KERNEL_SEQUENTIAL)
             std::snprintf(name, capacity, "%s<%u,%s>", base, unsigned(e->sys->npl),
                           e->cfg.rng == WLX_RNG_PHILOX ? "true" : "false");
+        else if (fn == k.ph_fast_width[e->width_index] && e->width_index != kDefaultWidth)
+            std::snprintf(name, capacity, "%s<%u,%u>", base, unsigned(e->sys->npl),
+                          kLookaheadWidths[e->width_index]);
         else
             std::snprintf(name, capacity, "%s<%u>", base, unsigned(e->sys->npl));
     }
     return WLX_OK;
+}
+
+int wlx_engine_set_lookahead_width(wlx_engine *e, uint64_t width)
+{
+    if (int status = require_engine(e, false)) return status;
+    // width 32 with more than 16 sites per lane gave wrong histogram counts on the GPU (at 64
+    // sites per lane: 506 registers and 304 bytes of scratch per lane), cause not found: it is
+    // only offered where it was shown exact
+    if (width == 32 && e->sys->npl > 16)
+        return set_error(WLX_INVALID_ARGUMENT,
+                         "Lookahead width 32 is limited to systems of at most 1024 sites");
+    for (int k = 0; k < 4; ++k)
+        if (kLookaheadWidths[k] == width) {
+            e->width_index = k;
+            return WLX_OK;
+        }
+    return set_error(WLX_INVALID_ARGUMENT, "Lookahead width must be 4, 8, 16 or 32");
 }
 
 int wlx_engine_autotune(wlx_engine *e, uint64_t epochs, uint64_t *chosen, double *seconds)
@@ -3371,28 +3404,46 @@ int wlx_engine_autotune(wlx_engine *e, uint64_t epochs, uint64_t *chosen, double
     if (epochs == 0) return set_error(WLX_INVALID_ARGUMENT, "Autotuning needs at least one epoch");
     if (seconds)
         for (int k = 0; k <= int(WLX_KERNEL_LDS); ++k) seconds[k] = -1.0;
-    std::vector<uint64_t> candidates;
-    if (!e->bins_hbm) candidates = {WLX_KERNEL_LOOKAHEAD, WLX_KERNEL_SEQUENTIAL};
-    if (e->lds_walker <= kLdsBytes) candidates.push_back(WLX_KERNEL_LDS);
+    // (family, lookahead width index); the Philox common-configuration lookahead kernel is
+    // tried at each width shown exact for this size
+    std::vector<std::pair<uint64_t, int>> candidates;
+    if (!e->bins_hbm) {
+        const bool widths = e->cfg.rng == WLX_RNG_PHILOX && !e->f32 && !e->collect &&
+                            e->trace_cap == 0 && e->cfg.wl.force_accept_after_attempts == 0 &&
+                            !e->cfg.wl.nalivaiko_mod;
+        if (widths) {
+            for (int k : {kDefaultWidth, 1, 3})
+                if (kLookaheadWidths[k] != 32 || e->sys->npl <= 16)
+                    candidates.push_back({WLX_KERNEL_LOOKAHEAD, k});
+        } else {
+            candidates.push_back({WLX_KERNEL_LOOKAHEAD, e->width_index});
+        }
+        candidates.push_back({WLX_KERNEL_SEQUENTIAL, e->width_index});
+    }
+    if (e->lds_walker <= kLdsBytes) candidates.push_back({WLX_KERNEL_LDS, e->width_index});
     uint64_t best = e->kernel_choice == WLX_KERNEL_AUTO ? WLX_KERNEL_LOOKAHEAD : e->kernel_choice;
+    int best_width = e->width_index;
     double best_s = -1.0;
     // every variant yields the same trajectory, so the trials are part of the run itself
-    for (uint64_t variant : candidates) {
+    for (const auto &[variant, width] : candidates) {
         e->kernel_choice = variant;
+        e->width_index = width;
         wlx_run_status st{};
         if (int status = wlx_engine_run(e, 1, &st)) return status; // first launch: code load
         if (st.done) break;
         const auto t0 = std::chrono::steady_clock::now();
         if (int status = wlx_engine_run(e, epochs, &st)) return status;
         const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        if (seconds) seconds[variant] = s;
+        if (seconds && (seconds[variant] < 0.0 || s < seconds[variant])) seconds[variant] = s;
         if (best_s < 0.0 || s < best_s) {
             best_s = s;
             best = variant;
+            best_width = width;
         }
         if (st.done) break;
     }
     e->kernel_choice = best;
+    e->width_index = best_width;
     if (chosen) *chosen = best;
     return WLX_OK;
 }

@@ -373,6 +373,10 @@ class Engine:
         """Forces a walker kernel family (abi.KERNEL_*); all give the same trajectory."""
         self.backend.call("engine_set_kernel", self.handle, ctypes.c_uint64(kernel))
 
+    def set_lookahead_width(self, width: int):
+        """Attempts per lookahead batch of the Philox common-configuration kernel."""
+        self.backend.call("engine_set_lookahead_width", self.handle, ctypes.c_uint64(width))
+
     def kernel(self) -> tuple[int, str]:
         """(family, kernel name) of the next sampling launch."""
         family = ctypes.c_uint64()
