@@ -1241,7 +1241,7 @@ __device__ __attribute__((always_inline)) inline void fast_walk(const Args &a, u
     if (count > 0 && (start_bin < int64_t(st.begin) || start_bin >= int64_t(st.end))) {
         st.escaped = 1; // wang_landau.cpp:247-248, decided up front as in the general kernel
     } else if (count > 0) {
-        const uint32_t begin = st.begin;
+        const uint32_t begin = st.begin, end_bin = st.end;
         const uint64_t att0 = st.attempted, clk0 = att0 + st.clock_offset;
         uint32_t cur = uint32_t(start_bin);
         double lg_cur = uniform_f64(v.lg[cur - begin]);
@@ -1291,24 +1291,49 @@ __device__ __attribute__((always_inline)) inline void fast_walk(const Args &a, u
             double incz = lane >= freeze ? 0.0 : inc;
             uint32_t m_hi = 0, cur_lane = 0;
             double lr_lane = 0.0;
+            const uint32_t used_s = uniform_u32(used);
+            // EnergyGrid::index (wang_landau.cpp:27-34) folded into the window test: energies
+            // below the grid give negative bins, and above it bins >= count >= end, both outside
+            // [begin, end); only a window that ends at the grid's last bin tests the maximum
+            // (above it: outside; equal to it: the last bin)
+            const bool top = end_bin == uint32_t(a.grid.count);
+            const double count_bins = double(a.grid.count);
+            // +inf unless the window ends at the grid's last bin: the tests below are then false
+            const double top_max = top ? a.grid.maximum : __builtin_huge_val();
+            auto window_bin = [&](double proposed, double fb) __attribute__((always_inline)) {
+                // far outside the grid stays outside after the conversion
+                int32_t idx = int32_t(fmin(fmax(fb, -1.0), count_bins));
+                idx = proposed > top_max ? -1 : idx;
+                idx = proposed == top_max ? int32_t(a.grid.count) - 1 : idx;
+                return uint32_t(idx) - begin;
+            };
             auto decide = [&](uint32_t b) __attribute__((always_inline)) {
-                const uint32_t L = used + b;
+                const uint32_t L = used_s + b;
                 const double hs = readlane_f64(h_run, L);
                 const uint32_t ms_hi = __builtin_amdgcn_readlane(dsig_hi, L);
                 const double ms = hi_double(ms_hi); // -2 sigma
                 const double proposed = energy + ms * hs; // flip_delta, physics.cpp:261-266
-                // EnergyGrid::index, wang_landau.cpp:27-34, and the window test: in VGPRs
-                const bool in = proposed >= a.grid.minimum && proposed <= a.grid.maximum;
-                const double fb = bin_floor(a.grid, proposed);
-                int32_t idx = int32_t(in ? fb : 0.0); // converted only when in range
-                if (proposed == a.grid.maximum) idx = min(idx, int32_t(a.grid.count) - 1);
-                const uint32_t rel = uint32_t(idx) - begin; // idx < end <= count when rel < ws
-                const bool inside = in && rel < ws;
-                const uint32_t rr = inside ? rel : 0u;
-                const double lgn = uniform_f64(v.lg[rr]);
-                const bool seen_n = uniform_u32(v.act[rr]) != 0;
-                const double log_ratio = lg_cur - lgn;
                 const double lu = readlane_f64(p_lu, L);
+                // bin from the reciprocal product; the LDS reads go out before the rare check
+                // that the product's floor is the correctly rounded quotient's (bin_floor)
+                const double x = proposed - a.grid.minimum;
+                const double q = x * a.grid.inverse_width;
+                const double fq = floor(q);
+                uint32_t rel = window_bin(proposed, fq);
+                bool inside = rel < ws;
+                uint32_t rr = inside ? rel : 0u;
+                double lgn = v.lg[rr];
+                bool seen_n = v.act[rr] != 0;
+                const double fr = q - fq; // exact: fq <= q < fq + 1
+                if (fr < a.grid.margin || fr > a.grid.margin_hi) {
+                    rel = window_bin(proposed, floor(x / a.grid.width));
+                    inside = rel < ws;
+                    rr = inside ? rel : 0u;
+                    lgn = v.lg[rr];
+                    seen_n = v.act[rr] != 0;
+                }
+                const uint32_t idx = rel + begin;
+                const double log_ratio = lg_cur - lgn;
                 const bool up = log_ratio >= 0.0;
                 const bool accepted = inside && (up || lu < log_ratio);
                 // log ratio of a drawn decision, +inf when no uniform was drawn (outside)
@@ -1319,7 +1344,7 @@ __device__ __attribute__((always_inline)) inline void fast_walk(const Args &a, u
                 const uint32_t s = __builtin_amdgcn_readlane(p_site, L);
                 dsig_hi ^= (accepted && p_site == s) ? 0x80000000u : 0u;
                 energy = accepted ? proposed : energy;
-                cur = accepted ? uint32_t(idx) : cur;
+                cur = accepted ? idx : cur;
                 lg_cur = accepted ? lgn : lg_cur;
                 cur_seen = accepted ? seen_n : cur_seen;
                 const uint32_t crel = cur - begin;
