@@ -21,23 +21,24 @@ the timed region).
 cpu_baseline runs the reference's own run_rewl (compiled from its sources into oracle/_ref)
 on the same workload with OpenMP on the host cores, bounded to a fixed attempt budget.
 
-time_to_accuracy runs a fresh C2 engine and reports the device time until its stitched DOS
-is within max|dlog g| < 0.01 of a converged C2 reference DOS, after mean-offset alignment
-(tests/golden/c2_reference_dos.json, made by tests/golden/make_c2_reference_dos.py).
-time_to_accuracy.replicas does the same for 8 independent C2 runs (seeds 1234..1241) batched
-into one lookahead-walker launch per epoch, whose stitched DOS are averaged: the 128-walker
-workload fills an eighth of the chip's wave slots, the batch fills them with replicas.
-time_to_accuracy.windows runs the same system, grid and schedule split into 48 windows of 16
-walkers (768 walkers in one launch per epoch), with snapshots every 500 epochs.
-time_to_accuracy.warm trains the warm-start network (256 slab curves, 60 000 steps) and times
-warm-started runs of both layouts.
+time_to_accuracy.plain runs fresh C2 engines (4 seeds) and reports, per seed, the device time
+to the first snapshot after which the stitched DOS stays within max|dlog g| < 0.01 of a
+converged C2 reference DOS (mean offset removed; tests/golden/c2_reference_dos.json, made by
+tests/golden/make_c2_reference_dos.py, whose own error estimate is in
+c2_reference_dos_error.json), and the median over the seeds.
+time_to_accuracy.windows does the same with the system, grid and schedule split into 48
+windows of 16 walkers (768 walkers in one launch per epoch).
+time_to_accuracy.replicas runs 8 independent C2 runs (seeds 1234..1241) batched into one
+lookahead-walker launch per epoch and measures the average of their stitched DOS: the
+128-walker workload fills an eighth of the chip's wave slots, the batch fills them with
+replicas.
 
 `--c4` runs only the C4 precision study instead: a 16x16x16 dense system with FP32 Kahan
 fields, the FP64 matrix-core resync every 10 MCS, and the drift it measures.
 
 `--c5` runs only the learned warm start: converged DOS curves of a slab family (one batch),
 the BF16 network trained on them by the project's kernels, and time to accuracy from flat and
-warm starts on a held-out slab and on C2.
+warm starts on the held-out 12x12x4 slab at dilution 0.2 (tests/golden/c5_reference_dos.json).
 
 `--c3` runs only the batched study: 64 uneven dilute CSR systems (N from 64 to 4096) whose
 walkers all sample in one launch per epoch (the LDS walker kernel), replayed from HIP graphs.
@@ -125,55 +126,97 @@ def gpu_leg(steps: int, warmup: int, rng: int):
 REFERENCE_DOS = ROOT / "tests" / "golden" / "c2_reference_dos.json"
 
 
-def tta_leg(budget_seconds: float, interval_epochs: int, rng: int = 1):
-    """Time to accuracy: a fresh C2 run (seed 1234) against the converged C2 reference DOS
-    (an independent long run, tests/golden/make_c2_reference_dos.py). Every interval_epochs
-    the walkers are read back, summarised per window and stitched (summarize_walkers +
-    stitch_dos). The clock (device time inside Engine.run only, snapshots excluded) stops at
-    the first snapshot that has found every bin the reference holds and lies within
-    max|dlog g| < 0.01 of it after mean-offset alignment."""
-    from replica_exchange_wang_landau_amd import Engine, System, analysis
-
+def reference_dos():
+    """(log g, valid mask, record) of the converged C2 reference, or None."""
     if not REFERENCE_DOS.exists():
         return None
     import numpy as np
 
     ref = json.loads(REFERENCE_DOS.read_text())
-    ref_lg = np.array([float.fromhex(v) for v in ref["log_g"]])
-    ref_valid = np.array(ref["valid"], dtype=np.uint8)
-    engine = Engine(System.simple_cubic(8, 8, 8, 1.0, (0.0, 0.0, 1.0), True), c2_config(rng=rng))
-    engine.initialize()
-    engine.set_graph(GRAPH_EPOCHS)
-    device, reached, best, snaps = 0.0, None, float("inf"), 0
-    attempted = 0
-    while device < budget_seconds:
+    return (np.array([float.fromhex(v) for v in ref["log_g"]]),
+            np.array(ref["valid"], dtype=np.uint8), ref)
+
+
+def stays_below(times, devs, threshold: float):
+    """The first snapshot time from which every later snapshot lies below `threshold`, or
+    None when the last snapshot does not."""
+    reached = None
+    for t, d in zip(times, devs):
+        if d < threshold:
+            reached = t if reached is None else reached
+        else:
+            reached = None
+    return reached
+
+
+def accuracy_series(run_epochs, read_dos, ref, budget_s: float, interval: int, label: str,
+                    threshold: float = 0.01, stay: int = 3):
+    """Device seconds (inside run_epochs only) and max|dlog g| at every snapshot: runs
+    `interval` epochs between snapshots until the deviation has stayed below the threshold
+    for `stay` snapshots, the run is done or the budget is spent."""
+    import numpy as np
+
+    from replica_exchange_wang_landau_amd import analysis
+
+    ref_lg, ref_valid = ref[0], ref[1]
+    device, times, devs, below = 0.0, [], [], 0
+    while device < budget_s:
         t0 = time.perf_counter()
-        st = engine.run(interval_epochs)
+        done = run_epochs(interval)
         device += time.perf_counter() - t0
-        attempted = st.attempted
         try:
-            lg, valid = analysis.engine_dos(engine)
-        except RuntimeError:  # a window without a common valid bin yet
-            continue
-        snaps += 1
+            lg, valid = read_dos()
+        except (RuntimeError, ValueError):
+            continue  # a window without a common valid bin yet
         if np.any((ref_valid != 0) & (valid == 0)):
-            print(f"tta snapshot {snaps}: {device:.1f} s, reference bins not all found yet",
-                  file=sys.stderr, flush=True)
-            continue
-        dev = analysis.max_deviation(lg, valid, ref_lg, ref_valid)
-        best = min(best, dev)
-        print(f"tta snapshot {snaps}: {device:.1f} s, max|dlog g| {dev:.4f}", file=sys.stderr,
-              flush=True)
-        if dev < 0.01:
-            reached = device
+            dev = float("inf")  # reference bins not all found yet
+        else:
+            dev = analysis.max_deviation(lg, valid, ref_lg, ref_valid)
+        times.append(device)
+        devs.append(dev)
+        below = below + 1 if dev < threshold else 0
+        print(f"{label}: {device:.2f} s, max|dlog g| {dev:.4f}", file=sys.stderr, flush=True)
+        if below >= stay or done:
             break
-        if st.done:
-            break
-    return {"threshold": 0.01, "seconds": reached, "attempted": int(attempted),
-            "best_max_dev": best if best != float("inf") else None, "snapshots": snaps,
-            "interval_epochs": interval_epochs, "budget_seconds": budget_seconds,
-            "reference": f"converged plain REWL, seed {ref['seed']}, "
-                         f"{ref['attempted']} attempts (tests/golden/c2_reference_dos.json)"}
+    return times, devs
+
+
+def tta_seeds(make_engine, seeds, budget_s: float, interval: int, label: str,
+              threshold: float = 0.01, ref=None):
+    """Time to accuracy of one run per seed (each engine alone on the device, HIP-graph
+    replays): per seed the first snapshot after which max|dlog g| against `ref` (log g, valid
+    mask; the C2 reference by default) stays below the threshold (stays_below), and the
+    median over the seeds (None where a seed did not get there)."""
+    import statistics
+
+    from replica_exchange_wang_landau_amd import analysis
+
+    ref = reference_dos() if ref is None else ref
+    if ref is None:
+        return None
+    per_seed, attempted, walkers, kernel = [], [], 0, None
+    for seed in seeds:
+        engine = make_engine(seed)
+        engine.set_graph(GRAPH_EPOCHS)
+        walkers, kernel = engine.walkers, engine.kernel()[1]
+
+        def run(n, e=engine):
+            return bool(e.run(n).done)
+
+        times, devs = accuracy_series(run, lambda e=engine: analysis.engine_dos(e), ref,
+                                      budget_s, interval, f"{label} seed {seed}", threshold)
+        reached = stays_below(times, devs, threshold)
+        per_seed.append(reached)
+        attempted.append(int(engine.status().attempted))
+        engine.close()
+    done = [t for t in per_seed if t is not None]
+    median = (statistics.median(per_seed) if len(done) == len(per_seed) else
+              (statistics.median(sorted(done) + [float("inf")] * (len(per_seed) - len(done)))
+               if done else None))
+    return {"seconds_median": median if median != float("inf") else None,
+            "seconds": per_seed, "seeds": list(seeds), "attempted": attempted,
+            "walkers": walkers, "kernel": kernel, "interval_epochs": interval,
+            "budget_seconds": budget_s}
 
 
 def cpu_leg(mcs_per_walker: int):
@@ -355,18 +398,13 @@ def tta_replicas_leg(budget_seconds: float, interval_epochs: int, replicas: int,
                      graph: int = 16):
     """Time to accuracy of `replicas` independent C2 runs (Philox, seeds 1234 + r) run as one
     batch (one lookahead-walker launch over all their walkers per epoch, HIP-graph replays of
-    `graph` epochs): the clock (Batch.run only) stops at the first snapshot where the average
-    of their stitched DOS (analysis.average_dos) holds every reference bin and lies within
-    max|dlog g| < 0.01 of the converged reference."""
+    `graph` epochs), measured on the average of their stitched DOS (analysis.average_dos), with
+    the stays-below criterion of tta_seeds."""
     from replica_exchange_wang_landau_amd import Batch, Engine, System, analysis
 
-    if not REFERENCE_DOS.exists() or replicas < 1:
+    ref = reference_dos()
+    if ref is None or replicas < 1:
         return None
-    import numpy as np
-
-    ref = json.loads(REFERENCE_DOS.read_text())
-    ref_lg = np.array([float.fromhex(v) for v in ref["log_g"]])
-    ref_valid = np.array(ref["valid"], dtype=np.uint8)
     engines = []
     for r in range(replicas):
         e = Engine(System.simple_cubic(8, 8, 8, 1.0, (0.0, 0.0, 1.0), True),
@@ -376,134 +414,26 @@ def tta_replicas_leg(budget_seconds: float, interval_epochs: int, replicas: int,
     batch = Batch(engines)
     batch.set_graph(graph)
     kernel = batch.kernel()
-    device, reached, best, snaps = 0.0, None, float("inf"), 0
-    while device < budget_seconds:
-        t0 = time.perf_counter()
-        batch.run(interval_epochs)
-        device += time.perf_counter() - t0
-        try:
-            lg, valid = analysis.average_dos([analysis.engine_dos(e) for e in engines])
-        except (RuntimeError, ValueError):
-            continue
-        snaps += 1
-        if np.any((ref_valid != 0) & (valid == 0)):
-            print(f"tta replicas snapshot {snaps}: {device:.1f} s, reference bins not all "
-                  "found yet", file=sys.stderr, flush=True)
-            continue
-        dev = analysis.max_deviation(lg, valid, ref_lg, ref_valid)
-        best = min(best, dev)
-        print(f"tta replicas snapshot {snaps}: {device:.1f} s, max|dlog g| {dev:.4f}",
-              file=sys.stderr, flush=True)
-        if dev < 0.01:
-            reached = device
-            break
+
+    def run(n):
+        return batch.run(n)[1] == len(engines)
+
+    times, devs = accuracy_series(
+        run, lambda: analysis.average_dos([analysis.engine_dos(e) for e in engines]), ref,
+        budget_seconds, interval_epochs, "tta replicas")
     attempted = sum(e.status().attempted for e in engines)
+    device = times[-1] if times else 0.0
     batch.close()
-    return {"replicas": replicas, "kernel": kernel, "threshold": 0.01, "seconds": reached,
-            "attempted": int(attempted), "rate": attempted / device if device else None,
-            "best_max_dev": best if best != float("inf") else None, "snapshots": snaps,
+    return {"replicas": replicas, "kernel": kernel, "threshold": 0.01,
+            "seconds": stays_below(times, devs, 0.01), "attempted": int(attempted),
+            "rate": attempted / device if device else None,
             "interval_epochs": interval_epochs, "graph_epochs": graph,
             "budget_seconds": budget_seconds}
 
 
-def tta_windows_leg(budget_seconds: float, interval_epochs: int, windows: int):
-    """Time to accuracy of the C2 system, grid and WL schedule split into `windows` windows of
-    16 walkers (overlap 0.75, seed 1234, Philox) instead of C2's 8: the same clock and
-    threshold as tta_leg, snapshots every interval_epochs."""
-    from replica_exchange_wang_landau_amd import Engine, System
-
-    if not REFERENCE_DOS.exists():
-        return None
-    import numpy as np
-
-    ref = json.loads(REFERENCE_DOS.read_text())
-    ref_lg = np.array([float.fromhex(v) for v in ref["log_g"]])
-    ref_valid = np.array(ref["valid"], dtype=np.uint8)
-    engine = Engine(System.simple_cubic(8, 8, 8, 1.0, (0.0, 0.0, 1.0), True),
-                    c2_config(rng=1, windows=windows))
-    engine.initialize()
-    engine.set_graph(GRAPH_EPOCHS)
-    seconds, attempted, best = accuracy_time(engine, ref_lg, ref_valid, budget_seconds,
-                                             interval_epochs)
-    return {"windows": windows, "walkers": engine.walkers, "kernel": engine.kernel()[1],
-            "threshold": 0.01, "seconds": seconds, "attempted": int(attempted),
-            "best_max_dev": best if best != float("inf") else None,
-            "interval_epochs": interval_epochs, "budget_seconds": budget_seconds}
-
-
-def tta_warm_leg(curves: int, steps: int, budget_seconds: float):
-    """Time to accuracy of warm-started C2 runs (8 windows, snapshots every 4000 epochs; and 48
-    windows, every 500): the network of train_warm_start presets log g and ln f
-    (wlx_engine_warm_start); the clock and threshold are tta_leg's. The flat-start figures of
-    the same layouts are tta_leg's and tta_windows_leg's."""
-    from replica_exchange_wang_landau_amd import Engine, System
-    from replica_exchange_wang_landau_amd import warmstart as ws
-
-    if not REFERENCE_DOS.exists():
-        return None
-    import numpy as np
-
-    ref = json.loads(REFERENCE_DOS.read_text())
-    ref_lg = np.array([float.fromhex(v) for v in ref["log_g"]])
-    ref_valid = np.array(ref["valid"], dtype=np.uint8)
-    mlp, _, training, validation = train_warm_start(curves, steps, "warm")
-    features = (512.0, 1.0, 0.0, 8.0, 0.0)  # N, density, dilution, thickness, axis mode
-    out = {"training": training}
-    for name, windows, interval in (("c2", 8, 4000), ("c2_48_windows", 48, 500)):
-        engine = Engine(System.simple_cubic(8, 8, 8, 1.0, (0.0, 0.0, 1.0), True),
-                        c2_config(rng=1, windows=windows))
-        engine.initialize()
-        # ln f from the error the network leaves inside windows of this layout's width
-        info = engine.read_walker(0).info
-        width = (info.window_end - info.window_begin) * engine.config.bin_width / features[0]
-        rms = window_rms(validation, width)
-        factor = warm_initial_factor(rms, features[0])
-        engine.warm_start(ws.predicted_log_g(mlp, engine, features), factor)
-        seconds, attempted, best = accuracy_time(engine, ref_lg, ref_valid, budget_seconds,
-                                                 interval)
-        out[name] = {"windows": windows, "window_rms_per_spin": rms, "initial_ln_f": factor,
-                     "seconds": seconds, "attempted": int(attempted),
-                     "best_max_dev": best if best != float("inf") else None}
-    return out
-
-
-def accuracy_time(engine, ref_log_g, ref_valid, budget_s: float, interval: int,
-                  threshold: float = 0.01):
-    """Device seconds (Engine.run only) until the stitched DOS holds every reference bin and
-    is within `threshold` of it after mean-offset alignment; (seconds or None, attempts,
-    best deviation)."""
-    import numpy as np
-
-    from replica_exchange_wang_landau_amd import analysis
-
-    device, best, attempted = 0.0, float("inf"), 0
-    while device < budget_s:
-        t0 = time.perf_counter()
-        st = engine.run(interval)
-        device += time.perf_counter() - t0
-        attempted = st.attempted
-        try:
-            lg, valid = analysis.engine_dos(engine)
-        except (RuntimeError, ValueError):
-            print(f"accuracy: {device:.1f} s, windows not yet joined", file=sys.stderr, flush=True)
-            continue
-        if np.any((ref_valid != 0) & (valid == 0)):
-            print(f"accuracy: {device:.1f} s, reference bins not all found", file=sys.stderr,
-                  flush=True)
-        else:
-            dev = analysis.max_deviation(lg, valid, ref_log_g, ref_valid)
-            best = min(best, dev)
-            print(f"accuracy: {device:.1f} s, max|dlog g| {dev:.4f}", file=sys.stderr, flush=True)
-            if dev < threshold:
-                return device, attempted, best
-        if st.done:
-            break
-    return None, attempted, best
-
-
-def train_warm_start(curves: int, steps: int, tag: str = "c5"):
+def train_warm_start(curves: int, steps: int, final: float = 1e-4, tag: str = "c5"):
     """The warm-start network: `curves` slabs (Lx = Ly in 6..12, Lz in 2..6, dilution in
-    [0, 0.4], +z or random in-plane axes; seed 1234) run to ln f = 1e-3 together in one batch,
+    [0, 0.4], +z or random in-plane axes; seed 1234) run to ln f = `final` together in one batch,
     and the 4 x 128 BF16 network trained on (E/N, N, density, dilution, Lz, axis mode) ->
     log g / N by the project's kernels (Adam 1e-3, then 3e-4 and 1e-4 for the last quarter
     each, batch 1024). The last tenth of the curves validates it. Returns (network, per-spin
@@ -515,7 +445,7 @@ def train_warm_start(curves: int, steps: int, tag: str = "c5"):
     t0 = time.perf_counter()
     slabs = ws.slab_family(curves, 1234)
     data = ws.converged_curves(
-        slabs, final_factor=1e-3,
+        slabs, final_factor=final,
         report=lambda d, n: print(f"{tag}: {d}/{n} training curves converged", file=sys.stderr,
                                   flush=True))
     keep = [(s, c) for s, c in zip(slabs, data) if (c[2] != 0).sum() > 4]
@@ -544,119 +474,82 @@ def train_warm_start(curves: int, steps: int, tag: str = "c5"):
     print(f"{tag}: {len(keep)} curves, {len(x)} samples in {curve_s:.0f} s; trained {steps} "
           f"steps in {train_s:.1f} s, loss {losses[0]:.3g} -> {losses[-1]:.3g}", file=sys.stderr,
           flush=True)
-    record = {"curves": split, "validation_curves": len(keep) - split,
+    record = {"curves": split, "validation_curves": len(keep) - split, "final_ln_f": final,
               "validation_rms_per_spin": val_rms, "samples": int(len(x)), "steps": steps,
               "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
               "curve_seconds": curve_s, "train_seconds": train_s}
     return mlp, val_rms, record, validation
 
 
-def window_rms(validation, width_per_spin: float) -> float:
-    """Per-spin RMS of the network's error on the unseen curves after removing its mean over
-    each window: consecutive E/N ranges of `width_per_spin`. Stitching removes each window's
-    offset, so this is the error a warm start leaves inside windows of that width."""
+C5_REFERENCE = ROOT / "tests" / "golden" / "c5_reference_dos.json"
+
+
+def c5_leg(curves: int, final: float, steps: int, budget_s: float, seeds: int,
+           factors=(1e-2, 1e-4), interval: int = 2000):
+    """Learned warm start on the held-out slab (train_warm_start for the network).
+
+    Reference: tests/golden/c5_reference_dos.json (make_c5_reference_dos.py), the average of
+    independent runs of the held-out 12x12x4 slab (dilution 0.2) frozen at ln f <= 1e-6, on
+    the grid of warmstart.slab_config from its recorded pilot range. Time to accuracy as in
+    tta_seeds (first snapshot after which max|dlog g| stays below 0.01, median over `seeds`
+    seeds) from a flat start and from the network's log g at each initial ln f in `factors`.
+    Also reported: the network start's own max|dlog g| to the reference (start_error), the
+    gain that error allows when the run's ln f must fall below f_0.01 anyway,
+    1 / (1 - 0.01 / start_error), and the spread between dilution realizations recorded with
+    the reference (a floor under any network that sees only the slab's parameters)."""
     import numpy as np
 
-    parts = []
-    for e, d in validation:
-        tile = np.floor((e - e.min()) / width_per_spin).astype(np.int64)
-        for k in np.unique(tile):
-            part = d[tile == k]
-            parts.append(part - part.mean())
-    return float(np.sqrt(np.mean(np.concatenate(parts) ** 2)))
-
-
-def warm_initial_factor(val_rms: float, n: float) -> float:
-    """ln f of a warm start: the square of the expected worst log g error of the network
-    (3 x the validation RMS per spin x N), within [1e-4, 1]."""
-    return float(min(1.0, max(1e-4, (3.0 * val_rms * n) ** 2)))
-
-
-def c5_leg(curves: int, steps: int, budget_s: float):
-    """Learned warm start (train_warm_start). Evaluation: time to max|dlog g| < 0.01 from a
-    flat start and from a warm start (network log g; ln f from the network's validation
-    error inside windows of the run's width) on the held-out 12x12x4 slab at dilution 0.2 (reference: a run of this engine to
-    ln f = 1e-5) and on C2 (reference: tests/golden/c2_reference_dos.json), with C2's 8
-    windows and split into 48."""
-    import numpy as np
-
-    from replica_exchange_wang_landau_amd import Engine, System, abi, analysis
+    from replica_exchange_wang_landau_amd import Engine, analysis
     from replica_exchange_wang_landau_amd import warmstart as ws
 
-    mlp, _, training, validation = train_warm_start(curves, steps)
+    if not C5_REFERENCE.exists():
+        return None
+    rec = json.loads(C5_REFERENCE.read_text())
+    ref = (np.array([float.fromhex(v) for v in rec["log_g"]]),
+           np.array(rec["valid"], dtype=np.uint8))
+    mlp, val_rms, training, _ = train_warm_start(curves, steps, final)
+    slab = ws.Slab(*rec["slab"])
+    system = slab.system()
+    if system.n != rec["spins"]:
+        raise RuntimeError("held-out slab differs from the recorded one")
+    lo, hi = rec["pilot"]
 
-    def initial_factor(engine, n):
-        """ln f from the network's error inside windows of the engine's width."""
-        info = engine.read_walker(0).info
-        width = (info.window_end - info.window_begin) * engine.config.bin_width / n
-        return warm_initial_factor(window_rms(validation, width), n)
-
-    def evaluate(make_engine, ref_lg, ref_valid, features, interval):
-        out = {}
-        for mode in ("flat", "warm"):
-            engine = make_engine()
-            if mode == "warm":
-                f0 = initial_factor(engine, features[0])
-                engine.warm_start(ws.predicted_log_g(mlp, engine, features), f0)
-            seconds, attempted, best = accuracy_time(engine, ref_lg, ref_valid, budget_s, interval)
-            out[mode] = {"seconds": seconds, "attempted": int(attempted), "best_max_dev": best}
-            if mode == "warm":
-                out[mode]["initial_ln_f"] = f0
-            print(f"c5: {mode} {out[mode]}", file=sys.stderr, flush=True)
-        if out["flat"]["seconds"] and out["warm"]["seconds"]:
-            out["speedup"] = out["flat"]["seconds"] / out["warm"]["seconds"]
-        return out
-
-    # held-out slab: reference DOS from a long run of this engine
-    held = ws.Slab(12, 4, 0.2, 0, 999_001)
-    hs = held.system()
-    lo, hi = System.pilot_many([hs], 1234, [2000 * hs.n])
-
-    def held_engine(seed=77, final=1e-6):
-        cfg = ws.slab_config(hs, lo[0], hi[0], seed, final)
-        e = Engine(hs, cfg)
+    def engine(seed, factor=None, guess=None):
+        e = Engine(system, ws.slab_config(system, lo, hi, seed, 1e-6))
+        if [e.config.emin, e.config.emax, e.config.bin_width] != rec["grid"]:
+            raise RuntimeError("held-out grid differs from the recorded one")
         e.initialize()
+        if factor is not None:
+            e.warm_start(guess, factor)
         return e
-    ref = held_engine(4242, 1e-5)
-    ref.autotune(2)
-    t2 = time.perf_counter()
-    while True:
-        st = ref.run(20_000)
-        w0 = ref.read_walker(0)
-        print(f"c5: held-out reference epoch {st.epoch}, walker 0 ln f {w0.factor:.3g}",
-              file=sys.stderr, flush=True)
-        if st.done:
-            break
-    ref_s = time.perf_counter() - t2
-    ref_lg, ref_valid = analysis.engine_dos(ref)
-    print(f"c5: held-out reference ({hs.n} spins) in {ref_s:.0f} s", file=sys.stderr, flush=True)
-    held_result = evaluate(held_engine, ref_lg, ref_valid, held.features(hs.n), 500)
 
-    c2_result = c2w_result = None
-    if REFERENCE_DOS.exists():
-        ref = json.loads(REFERENCE_DOS.read_text())
-        c2_lg = np.array([float.fromhex(v) for v in ref["log_g"]])
-        c2_valid = np.array(ref["valid"], dtype=np.uint8)
-        c2_sys = System.simple_cubic(8, 8, 8, 1.0, (0.0, 0.0, 1.0), True)
-        def c2_engine():
-            e = Engine(c2_sys, c2_config(rng=abi.RNG_PHILOX))
-            e.initialize()
-            return e
-        c2_result = evaluate(c2_engine, c2_lg, c2_valid, (512.0, 1.0, 0.0, 8.0, 0.0), 4000)
-
-        def c2_windows_engine():  # the many-window form: 48 windows x 16 walkers
-            e = Engine(c2_sys, c2_config(rng=abi.RNG_PHILOX, windows=48))
-            e.initialize()
-            return e
-        c2w_result = evaluate(c2_windows_engine, c2_lg, c2_valid, (512.0, 1.0, 0.0, 8.0, 0.0),
-                              500)
+    probe = engine(0)
+    guess = ws.predicted_log_g(mlp, probe, slab.features(system.n))
+    probe.close()
+    start = guess - guess[ref[1] != 0].mean()
+    d = (start - ref[0])[ref[1] != 0]
+    d -= d.mean()
+    start_error = float(np.max(np.abs(d)))
+    seed_list = list(range(77, 77 + seeds))
+    flat = tta_seeds(lambda s: engine(s), seed_list, budget_s, interval, "c5 flat", ref=ref)
+    warm = {}
+    for f0 in factors:
+        warm[f"{f0:g}"] = tta_seeds(lambda s, f0=f0: engine(s, f0, guess), seed_list, budget_s,
+                                    interval, f"c5 warm ln f {f0:g}", ref=ref)
+    speedup = {k: flat["seconds_median"] / v["seconds_median"] for k, v in warm.items()
+               if flat["seconds_median"] and v["seconds_median"]}
     return {
-        "workload": "C5: learned warm start (4 x 128 BF16 MLP, FP32 accumulate)",
-        "training": training,
-        "held_out": {"slab": "12x12x4, dilution 0.2, +z axes", "spins": int(hs.n),
-                     "reference_seconds": ref_s, **held_result},
-        "c2": c2_result,
-        "c2_48_windows": c2w_result,
+        "workload": "C5: learned warm start (4 x 128 BF16 MLP, FP32 accumulate) on the "
+                    "held-out 12x12x4 slab, dilution 0.2, +z axes",
+        "training": training, "spins": int(system.n),
+        "reference": {"runs": rec["runs"], "final_ln_f": rec["final_factor"],
+                      "max_dev_between_runs": rec["max_pairwise_max_dev"]},
+        "realization_spread_max_dev": [r["max_dev"] for r in rec["siblings"]],
+        "start_error_max_dev": start_error,
+        "start_error_rms": float(np.sqrt(np.mean(d * d))),
+        "gain_allowed_by_start_error": (1.0 / (1.0 - 0.01 / start_error)
+                                        if start_error > 0.01 else None),
+        "flat": flat, "warm": warm, "speedup_median": speedup,
     }
 
 
@@ -682,9 +575,11 @@ def main():
     ap.add_argument("--rng", choices=["xoshiro", "philox"], default="philox")
     ap.add_argument("--cpu-mcs", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tta-seconds", type=float, default=150.0,
+    ap.add_argument("--tta-seconds", type=float, default=40.0,
                     help="device-time budget of the time-to-accuracy leg (0 skips it)")
-    ap.add_argument("--tta-interval", type=int, default=4000, help="epochs between snapshots")
+    ap.add_argument("--tta-interval", type=int, default=2000, help="epochs between snapshots")
+    ap.add_argument("--tta-seeds", type=int, default=4,
+                    help="independent runs per time-to-accuracy figure (median reported)")
     ap.add_argument("--tta-replicas", type=int, default=8,
                     help="independent C2 replicas batched for the averaged time to accuracy "
                          "(0 skips it)")
@@ -692,13 +587,16 @@ def main():
                     help="device-time budget of the replica leg")
     ap.add_argument("--tta-windows", type=int, default=48,
                     help="windows of the many-window time to accuracy (0 skips it)")
-    ap.add_argument("--tta-warm-curves", type=int, default=256,
-                    help="training curves of the warm-started time to accuracy (0 skips it)")
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
                     help="only run a plain copy of MIB MiB (counter calibration)")
     ap.add_argument("--c5", action="store_true",
                     help="only run the learned warm-start study")
-    ap.add_argument("--c5-curves", type=int, default=256, help="training curves (--c5)")
+    ap.add_argument("--c5-curves", type=int, default=512, help="training curves (--c5)")
+    ap.add_argument("--c5-final", type=float, default=1e-4,
+                    help="ln f the training curves are converged to (--c5)")
+    ap.add_argument("--c5-steps", type=int, default=60_000, help="training steps (--c5)")
+    ap.add_argument("--c5-seconds", type=float, default=60.0,
+                    help="device-time budget per held-out run (--c5)")
     ap.add_argument("--c3", action="store_true",
                     help="only run the C3 batched study (64 uneven CSR systems in one launch)")
     ap.add_argument("--graph", type=int, default=16, help="epochs per HIP graph (--c3)")
@@ -708,7 +606,8 @@ def main():
                     help="only run the C4 FP32-field precision study (resync kernel, drift)")
     args = ap.parse_args()
     if args.c5:
-        print(json.dumps(c5_leg(args.c5_curves, 60_000, 120.0)))
+        print(json.dumps(c5_leg(args.c5_curves, args.c5_final, args.c5_steps, args.c5_seconds,
+                                args.tta_seeds)))
         return
     if args.c3:
         print(json.dumps(c3_leg(args.steps if args.steps != 3000 else 200, 20, args.graph,
@@ -734,20 +633,54 @@ def main():
     print("bench: second-stream leg done", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_leg(args.cpu_mcs)
     print("bench: cpu leg done", file=sys.stderr, flush=True)
-    tta = tta_leg(args.tta_seconds, args.tta_interval, rng) if args.tta_seconds > 0 else None
-    if tta and cpu and tta["seconds"] is not None:
-        # the reference CPU path at its measured rate over the same number of attempts
-        tta["cpu_estimate_seconds"] = tta["attempted"] / cpu["value"]
-    if tta is not None and args.tta_replicas > 0:
-        tta["replicas"] = tta_replicas_leg(args.tta_replicas_seconds, args.tta_interval,
-                                           args.tta_replicas)
-        print("bench: replica leg done", file=sys.stderr, flush=True)
-    if tta is not None and args.tta_windows > 0:
-        tta["windows"] = tta_windows_leg(60.0, 500, args.tta_windows)
-        print("bench: many-window leg done", file=sys.stderr, flush=True)
-    if tta is not None and args.tta_warm_curves > 0:
-        tta["warm"] = tta_warm_leg(args.tta_warm_curves, 60_000, 60.0)
-        print("bench: warm-start leg done", file=sys.stderr, flush=True)
+    tta = None
+    if args.tta_seconds > 0 and reference_dos() is not None:
+        from replica_exchange_wang_landau_amd import Engine, System
+
+        def c2_engine(windows):
+            def make(seed):
+                e = Engine(System.simple_cubic(8, 8, 8, 1.0, (0.0, 0.0, 1.0), True),
+                           c2_config(rng=1, seed=seed, windows=windows))
+                e.initialize()
+                return e
+            return make
+        seeds = list(range(1234, 1234 + args.tta_seeds))
+        ref = reference_dos()[2]
+        tta = {"threshold": 0.01,
+               "criterion": "device seconds to the first snapshot after which max|dlog g| "
+                            "(mean offset removed) stays below the threshold, median over "
+                            f"{len(seeds)} seeds",
+               "reference": f"converged plain REWL to ln f 1e-7, seed {ref['seed']}, "
+                            f"{ref['attempted']} attempts (tests/golden/c2_reference_dos.json)"}
+        error = ROOT / "tests" / "golden" / "c2_reference_dos_error.json"
+        if error.exists():
+            e = json.loads(error.read_text())
+            tta["reference_error"] = {
+                "independent_runs": e["runs"], "their_ln_f": e["final_factor"],
+                "max_dev_between_them": e["max_pairwise_max_dev"],
+                "max_dev_to_reference": e["max_vs_reference"]}
+        tta["plain"] = tta_seeds(c2_engine(8), seeds, args.tta_seconds, args.tta_interval,
+                                 "tta plain")
+        print("bench: plain time-to-accuracy leg done", file=sys.stderr, flush=True)
+        if cpu and tta["plain"]["seconds_median"] is not None:
+            # the reference CPU path at its measured rate over the median run's attempts
+            att = sorted(tta["plain"]["attempted"])[len(seeds) // 2]
+            tta["plain"]["cpu_estimate_seconds"] = att / cpu["value"]
+        if args.tta_windows > 0:
+            tta["windows"] = tta_seeds(c2_engine(args.tta_windows), seeds, 60.0, 500,
+                                       f"tta {args.tta_windows} windows")
+            tta["windows"]["windows"] = args.tta_windows
+            print("bench: many-window leg done", file=sys.stderr, flush=True)
+        if args.tta_replicas > 0:
+            tta["replicas"] = tta_replicas_leg(args.tta_replicas_seconds, args.tta_interval,
+                                               args.tta_replicas)
+            print("bench: replica leg done", file=sys.stderr, flush=True)
+        plain = tta["plain"]["seconds_median"]
+        if plain:
+            tta["speedup_vs_plain"] = {
+                k: plain / v for k, v in (
+                    ("windows", (tta.get("windows") or {}).get("seconds_median")),
+                    ("replicas", (tta.get("replicas") or {}).get("seconds"))) if v}
     line = {
         "metric": "attempted flips/s (all walkers)",
         "value": g["rate"],

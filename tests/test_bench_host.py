@@ -1,5 +1,6 @@
 """Host-side pieces of bench.py that need no GPU: the one-GPU refusal, the counter summary it
-reads the traffic figure from, the warm-start ln f rule and the C2 run configurations."""
+reads the traffic figure from, the time-to-accuracy criterion, the C2 run configurations and the
+batched workload's windows."""
 from __future__ import annotations
 
 import subprocess
@@ -28,10 +29,16 @@ def test_traffic_figures_come_from_the_committed_counter_summary():
     assert bench.traffic_bytes("no_such_kernel<1>") is None
 
 
-@pytest.mark.parametrize("rms,n,expected", [(1e-6, 512, 1e-4), (1e-4, 512, (3e-4 * 512) ** 2),
-                                            (1e-3, 512, 1.0)])
-def test_warm_start_factor_is_the_squared_error_bound_clamped(rms, n, expected):
-    assert bench.warm_initial_factor(rms, n) == pytest.approx(expected, rel=1e-12)
+def test_stays_below_takes_the_first_snapshot_of_the_last_run_below():
+    """Time to accuracy counts from the first snapshot after which every later one is below
+    the threshold: a dip that comes back up does not count, and a run that ends above it has
+    no time."""
+    t = [1.0, 2.0, 3.0, 4.0, 5.0, 6.0]
+    assert bench.stays_below(t, [0.5, 0.009, 0.02, 0.008, 0.007, 0.006], 0.01) == 4.0
+    assert bench.stays_below(t, [0.005] * 6, 0.01) == 1.0
+    assert bench.stays_below(t, [0.5, 0.009, 0.008, 0.007, 0.006, 0.011], 0.01) is None
+    assert bench.stays_below([], [], 0.01) is None
+    assert bench.stays_below(t, [0.5, 0.2, 0.1, 0.05, 0.02, 0.0099], 0.01) == 6.0
 
 
 def test_c2_configurations():
@@ -43,19 +50,6 @@ def test_c2_configurations():
         assert (c.emin, c.emax, c.bin_width) == bench.C2_GRID
         assert c.exchange_interval_attempts == 512 and c.overlap == 0.75
         assert round((c.emax - c.emin) / c.bin_width) == 1255
-
-
-def test_window_rms_removes_each_window_mean():
-    """A linear error d = k e, sampled densely, leaves k w / sqrt(12) inside windows of width w
-    once each window's mean is removed; a constant error leaves nothing."""
-    import numpy as np
-
-    e = np.linspace(0.0, 4.0, 400001)[:-1]
-    k = 2e-3
-    for w in (1.0, 0.5, 0.25):
-        rms = bench.window_rms([(e, k * e)], w)
-        assert rms == pytest.approx(k * w / np.sqrt(12.0), rel=1e-3)
-    assert bench.window_rms([(e, np.full_like(e, 0.7))], 0.5) == pytest.approx(0.0, abs=1e-12)
 
 
 def test_partition_matches_partition_windows(orc):
