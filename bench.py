@@ -118,6 +118,7 @@ def gpu_leg(steps: int, warmup: int, rng: int):
     return {
         "attempted": attempted, "accepted": accepted, "epochs": epochs, "seconds": seconds,
         "rate": attempted / seconds, "kernel_ms": kernel_ms, "launches": launches,
+        "kernel": engine.kernel()[1],
         "achieved_gbs": achieved, "per_launch_bytes": per_launch_bytes,
         "ambiguous": after.ambiguous_decisions,
     }
@@ -624,10 +625,10 @@ def main():
               "(the walkers would shard by energy window across GPUs)", file=sys.stderr)
         sys.exit(2)
     rng = 0 if args.rng == "xoshiro" else 1
-    # N = 512 sites: 8 register slots per lane; without trace, forcing or the Nalivaiko mask
-    # each stream runs its common-configuration lookahead kernel
-    kernel = "lookahead_xo_kernel<8>" if rng == 0 else "lookahead_fast_kernel<8>"
+    # N = 512 sites without trace, forcing or the Nalivaiko mask: the Philox stream runs the
+    # staged lookahead walker (128 walkers, one per CU), the Xoshiro stream its lookahead kernel
     g = gpu_leg(args.steps, args.warmup, rng)
+    kernel = g["kernel"]
     print(f"bench: {args.rng} leg done", file=sys.stderr, flush=True)
     other = gpu_leg(args.steps, args.warmup, 1 - rng)
     print("bench: second-stream leg done", file=sys.stderr, flush=True)
@@ -702,7 +703,7 @@ def main():
             "acceptance": g["accepted"] / max(1, g["attempted"]),
             "ambiguous_log_decisions": g["ambiguous"],
             ("philox" if rng == 0 else "xoshiro"): {
-                "value": other["rate"], "kernel_ms": other["kernel_ms"],
+                "value": other["rate"], "kernel": other["kernel"], "kernel_ms": other["kernel_ms"],
                 "ms_per_step": other["seconds"] * 1e3 / max(1, other["epochs"]),
                 "acceptance": other["accepted"] / max(1, other["attempted"])},
         },

@@ -55,12 +55,18 @@ enum wlx_couplings { WLX_COUPLINGS_FP64 = 0, WLX_COUPLINGS_FP32 = 1 };
  * LOOKAHEAD  batches of proposals resolved together, fields in registers (N <= 4096);
  * SEQUENTIAL one attempt at a time, fields in registers;
  * LDS        one attempt at a time, fields and spins in LDS; a flip scatters the CSR row of a
- *            CSR system or streams the dense row otherwise. Serves any N in one kernel. */
+ *            CSR system or streams the dense row otherwise. Serves any N in one kernel.
+ * STAGED     lookahead with each batch's candidate rows copied into LDS while the previous
+ *            batch decides (Philox stream, no trace, forcing or Nalivaiko mask, 65 to 512
+ *            sites, two 16-row buffers and the window bins within a CU's LDS). AUTO picks it
+ *            for such engines with at most one walker per CU, unless a lookahead width was
+ *            set; batches never use it. */
 enum wlx_walker_kernel {
     WLX_KERNEL_AUTO = 0,
     WLX_KERNEL_LOOKAHEAD = 1,
     WLX_KERNEL_SEQUENTIAL = 2,
-    WLX_KERNEL_LDS = 3
+    WLX_KERNEL_LDS = 3,
+    WLX_KERNEL_STAGED = 4
 };
 
 /* trace flags, one byte per recorded attempt */
@@ -302,7 +308,7 @@ int wlx_engine_warm_start(wlx_engine *engine, const double *log_g, double factor
  * (wang_landau.cpp:244-288). set_kernel forces a family (enum wlx_walker_kernel); kernel
  * reports the family and kernel name the next sampling launch uses; autotune runs `epochs`
  * epochs of the live run with each eligible family (the trajectory does not depend on it),
- * keeps the fastest and writes the seconds per family to seconds[4] (-1 = not tried). The
+ * keeps the fastest and writes the seconds per family to seconds[5] (-1 = not tried). The
  * Philox lookahead family is tried at widths 16, 8 and 32 (wlx_engine_set_lookahead_width). */
 int wlx_engine_set_kernel(wlx_engine *engine, uint64_t kernel);
 int wlx_engine_kernel(const wlx_engine *engine, uint64_t *kernel, char *name, size_t capacity);

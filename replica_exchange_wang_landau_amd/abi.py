@@ -34,6 +34,7 @@ KERNEL_AUTO = 0
 KERNEL_LOOKAHEAD = 1
 KERNEL_SEQUENTIAL = 2
 KERNEL_LDS = 3
+KERNEL_STAGED = 4
 
 TRACE_ACCEPTED = 1
 TRACE_FORCED = 2

@@ -26,6 +26,7 @@
 #include <mutex>
 #include <string>
 #include <memory>
+#include <type_traits>
 
 namespace rewl {
 
@@ -1436,6 +1437,446 @@ __global__ __launch_bounds__(kWave) void lookahead_fast_kernel(const Args *table
 
 
 // ---------------------------------------------------------------------------------------------
+// lookahead sampling with the candidate rows staged in LDS (common configuration, Philox
+// stream, 65 to 512 sites). The decisions are those of fast_walk, in the same order, arranged
+// so that a walker's serial chain holds little besides the arithmetic of the decision and one
+// LDS read (one wave issues one vector instruction per 4 cycles, in order, so every
+// instruction on the decision loop costs time whether or not it is on the dependency chain):
+//   - the 16 rows J[s_q, :] of a batch's proposals are copied from L2 into LDS by
+//     global_load_lds_dwordx4 (1 KiB per instruction) while the previous batch decides, into
+//     one of two buffers, one row per decision; the couplings among the proposals come from
+//     LDS, and the accepted rows are FMA'd into the register fields from LDS;
+//   - the fields and spin bits are mirrored in LDS, so the batch's gather is one LDS read;
+//   - the window's log_g has a +inf entry on either side (relative bins -1 and ws), and the
+//     proposed bin is clamped onto them, so a proposal outside the window gives log ratio -inf
+//     and is rejected by the same comparison log(u) < log ratio that accepts the others (log u
+//     is finite and negative, so that comparison also accepts every log ratio >= 0);
+//   - the bin comes from the reciprocal product, which is exact unless the product lies near
+//     an integer, or at or above the grid maximum's product in the window that ends at the
+//     grid's last bin (rounding is monotonic, so any energy >= maximum lands there);
+//   - a batch is first decided without branches, every decision selecting its outcome, and
+//     with the visited flags and the active count left for after the batch: one atomic OR per
+//     decision's bin then finds the batch's first visits. A decision whose product is not known
+//     exact, or a first visit in the 1/t stage (it changes the 1/t factors of later attempts),
+//     sends the batch back: registers from a snapshot, LDS log_g (from each decision's record
+//     of the value it replaced, in reverse order) and visited flags are restored, and the batch
+//     is decided again with the exact quotient, first visits and the 1/t refresh in line;
+//   - a batch without a repeated proposal site runs a loop without the sign bookkeeping of
+//     repeats, and outside the 1/t stage the log_g increment is one value for the batch;
+//   - each decision records its bin, accepted m, log ratio and previous log_g in LDS; H, the
+//     ambiguity count, the spin flips and the accepted count are applied lane-parallel after
+//     the batch.
+// ---------------------------------------------------------------------------------------------
+constexpr int kStagedBatch = 16;
+
+struct StagedLayout {
+    uint32_t lg, H, act, ref, rows, fields, spins, dups, rec_cm, rec_lr, rec_old;
+    uint32_t total;
+};
+
+// LDS offsets (bytes) of the staged walker: log_g with one entry below the window (lg - 8) and
+// one past it, H, the visited flags (4-byte aligned, with spare bytes on either side), the
+// refine flags, two row buffers of kStagedBatch rows each, the field and spin-bit mirrors, a
+// site table for finding repeated proposals, and the decision records
+__host__ __device__ inline StagedLayout staged_layout(uint32_t stride, uint32_t np)
+{
+    StagedLayout s;
+    s.lg = 8;
+    s.H = 8 * (stride + 2);
+    s.act = s.H + 8 * stride + 4;
+    s.ref = s.act + (stride + 4 + 3) / 4 * 4;
+    s.rows = (s.ref + stride + 15) / 16 * 16;
+    s.fields = s.rows + 2 * kStagedBatch * np * 8;
+    s.spins = s.fields + np * 8;
+    s.dups = s.spins + kWave * 4;
+    s.rec_cm = s.dups + np * 4;
+    s.rec_lr = s.rec_cm + kStagedBatch * 8;
+    s.rec_old = s.rec_lr + kStagedBatch * 8;
+    s.total = s.rec_old + kStagedBatch * 8;
+    return s;
+}
+
+// v_cvt_i32_f64 with the hardware's saturation for values outside the int32 range
+__device__ inline int32_t cvt_i32_sat(double x)
+{
+    int32_t r;
+    asm("v_cvt_i32_f64 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// copies the row at `row` (64 * NPL doubles) into LDS at byte address `dst`: each
+// global_load_lds_dwordx4 moves 64 lanes x 16 B = 1 KiB, lane l's 16 B landing at
+// M0 + offset + 16 l; the instruction offset advances the global and the LDS address together.
+// Counted by vmcnt; stage_wait waits for them. M0 is written here only: the staged walker's
+// own code uses no M0.
+template <int NPL>
+__device__ inline void stage_row(const double *row, uint32_t dst)
+{
+    const uint32_t voff = lane_id() * 16;
+    if constexpr (NPL == 2) {
+        asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\t"
+                     "global_load_lds_dwordx4 %1, %2" :: "s"(dst), "v"(voff), "s"(row));
+    } else if constexpr (NPL == 4) {
+        asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\t"
+                     "global_load_lds_dwordx4 %1, %2\n\t"
+                     "global_load_lds_dwordx4 %1, %2 offset:1024" :: "s"(dst), "v"(voff), "s"(row));
+    } else {
+        static_assert(NPL == 8, "staged rows: 1, 2 or 4 KiB");
+        asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\t"
+                     "global_load_lds_dwordx4 %1, %2\n\t"
+                     "global_load_lds_dwordx4 %1, %2 offset:1024\n\t"
+                     "global_load_lds_dwordx4 %1, %2 offset:2048\n\t"
+                     "global_load_lds_dwordx4 %1, %2 offset:3072" :: "s"(dst), "v"(voff), "s"(row));
+    }
+}
+
+__device__ inline void stage_wait()
+{
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+template <int NPL>
+__device__ __attribute__((always_inline)) inline void staged_walk(const Args &a, uint32_t w,
+                                                                  uint64_t request, uint32_t flags,
+                                                                  unsigned char *smem)
+{
+    static_assert(NPL >= 2 && NPL <= 8, "staged rows: 1, 2 or 4 KiB per row");
+    constexpr int KB = kStagedBatch;
+    constexpr uint32_t kRowBytes = 64 * NPL * 8;
+    constexpr uint32_t kRowDoubles = kRowBytes / 8;
+    constexpr uint32_t kBufBytes = KB * kRowBytes;
+    const uint32_t lane = lane_id();
+    if ((flags & kGate) && a.glob->done) return;
+    WalkerState st = a.state[w];
+    if (st.escaped) return;
+    const uint32_t ws = st.end - st.begin;
+    const StagedLayout lay = staged_layout(a.ws_stride, a.np);
+    Bins v = {reinterpret_cast<double *>(smem + lay.lg), reinterpret_cast<uint64_t *>(smem + lay.H),
+              smem + lay.act, smem + lay.ref};
+    double *fmirror = reinterpret_cast<double *>(smem + lay.fields);
+    uint32_t *smirror = reinterpret_cast<uint32_t *>(smem + lay.spins);
+    uint32_t *duptab = reinterpret_cast<uint32_t *>(smem + lay.dups);
+    uint64_t *rec_cm = reinterpret_cast<uint64_t *>(smem + lay.rec_cm);
+    double *rec_lr = reinterpret_cast<double *>(smem + lay.rec_lr);
+    double *rec_old = reinterpret_cast<double *>(smem + lay.rec_old);
+    const uint32_t rows_lds = uint32_t(reinterpret_cast<uintptr_t>(smem)) + lay.rows;
+    load_bins(a, w, ws, v);
+    if (lane == 0) {
+        v.lg[-1] = __builtin_huge_val();
+        v.lg[ws] = __builtin_huge_val();
+    }
+    uint64_t neg;
+    double h[NPL];
+    load_config<NPL>(a, st.slot, neg, h);
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) fmirror[k * kWave + lane] = h[k];
+    smirror[lane] = uint32_t(neg);
+    double energy = uniform_f64(a.energy[st.slot]);
+    __syncthreads();
+    const uint64_t count = batch_count(a, st, request, flags);
+    const int64_t start_bin = grid_index(a.grid, energy);
+    if (count > 0 && (start_bin < int64_t(st.begin) || start_bin >= int64_t(st.end))) {
+        st.escaped = 1; // wang_landau.cpp:247-248, decided up front as in the general kernel
+    } else if (count > 0) {
+        const uint32_t begin = st.begin;
+        const uint64_t att0 = st.attempted, clk0 = att0 + st.clock_offset;
+        const double *J = a.J;
+        const double gmin = a.grid.minimum, ginv = a.grid.inverse_width;
+        const double margin = a.grid.margin, margin_hi = a.grid.margin_hi;
+        // the product of the grid maximum: at or above it the quotient decides (top window only)
+        const double q_top = st.end == uint32_t(a.grid.count)
+                                 ? (a.grid.maximum - a.grid.minimum) * a.grid.inverse_width
+                                 : __builtin_huge_val();
+        const int32_t lo = int32_t(begin) - 1, hi = int32_t(st.end);
+        double *lgm = v.lg - begin; // by absolute bin
+        uint8_t *actm = v.act - begin;
+        int32_t cur = int32_t(start_bin); // absolute bin
+        double lgc = uniform_f64(lgm[cur]);
+        uint32_t cur_seen = uniform_u32(actm[cur]) != 0 ? 1u : 0u;
+        uint32_t stage = uniform_u32(st.stage), active = uniform_u32(st.active_count);
+        double factor = uniform_f64(st.factor);
+        uint32_t done_total = 0, n_acc = 0, last_acc = 0;
+        uint64_t ambiguous = 0;
+        // proposals of the current block of 64 attempts, and of the next one (drawn one batch
+        // before it starts, so that its first rows can be staged)
+        auto draw = [&](uint64_t first, uint32_t &site, double &lu) {
+            const uint64_t t = first + lane;
+            const Philox4 blk = philox4x64(t >> 1, kPhiloxWalkTag, 0, 0, a.seed, uint64_t(w));
+            const bool odd = t & 1;
+            site = uint32_t(__umul64hi(odd ? blk.v[2] : blk.v[0], uint64_t(a.n)));
+            const double u = double((odd ? blk.v[3] : blk.v[1]) >> 11) * 0x1.0p-53;
+            lu = fmin(log(fmax(u, 0x1.0p-53)), -0x1.0p-1074); // < 0: see the header
+        };
+        uint32_t p_site, n_site = 0;
+        double p_lu, n_lu = 0.0;
+        draw(att0, p_site, p_lu);
+        uint32_t used = 0, buf = 0;
+#pragma unroll
+        for (int q = 0; q < KB; ++q)
+            stage_row<NPL>(J + size_t(__builtin_amdgcn_readlane(p_site, uint32_t(q))) * kRowDoubles,
+                           rows_lds + uint32_t(q) * kRowBytes);
+        uint64_t remaining = count;
+        while (remaining > 0) {
+            const uint32_t K = remaining < uint64_t(KB) ? uint32_t(remaining) : uint32_t(KB);
+            const bool more = remaining > uint64_t(K);
+            const bool block_end = used + KB == kWave;
+            if (more && block_end) draw(att0 + done_total + KB, n_site, n_lu);
+            const uint32_t next_site = block_end ? n_site : p_site;
+            const uint32_t next_first = block_end ? 0u : used + KB;
+            const uint32_t next_rows = rows_lds + (buf ^ 1) * kBufBytes;
+            stage_wait(); // this batch's rows (staged one batch ago) are in LDS
+            // gather: field, spin and couplings J[s_q][s_lane] at each proposal of the batch
+            const double *rows = reinterpret_cast<const double *>(smem + lay.rows + buf * kBufBytes);
+            double hv = fmirror[p_site];
+            const uint32_t bits = smirror[p_site & (kWave - 1)];
+            uint32_t mv_hi = ((bits >> (p_site >> 6)) & 1) ? 0x40000000u : 0xC0000000u; // -2 sigma
+            double jc[KB];
+#pragma unroll
+            for (int q = 0; q < KB; ++q) jc[q] = rows[q * kRowDoubles + p_site];
+            const bool inverse_time = stage == WLX_STAGE_INVERSE_TIME;
+            double inc = batch_factors(stage, factor, active, clk0 + done_total, used);
+            uint32_t freeze = kWave; // lane of the freezing attempt; kWave: none
+            if (inverse_time) {
+                const uint64_t fz = __ballot(lane >= used && inc <= a.final_factor);
+                freeze = fz ? uint32_t(__ffsll((unsigned long long)fz) - 1) : kWave;
+            }
+            double incz = lane >= freeze ? 0.0 : inc;
+            double dv = hi_double(mv_hi) * hv; // flip_delta of each proposal (physics.cpp:261-266)
+            const uint32_t used_s = uniform_u32(used);
+            // the exact decision: the correctly rounded quotient where the product is not known
+            // exact, first visits in line, refreshing the 1/t factors of later attempts
+            auto decide_exact = [&](uint32_t b) __attribute__((always_inline)) {
+                const uint32_t L = used_s + b;
+                const double proposed = energy + readlane_f64(dv, L);
+                const double q = (proposed - gmin) * ginv;
+                const double fq = floor(q);
+                int32_t idx = min(max(cvt_i32_sat(fq), lo), hi);
+                const double fr = q - fq; // exact: fq <= q < fq + 1
+                if (fr < margin || fr > margin_hi || q >= q_top) {
+                    bool in;
+                    const uint32_t g = grid_bin(a.grid, proposed, in);
+                    idx = in ? min(max(int32_t(g), lo), hi) : lo;
+                }
+                const double lgn = lgm[idx];
+                const uint32_t actn = actm[idx];
+                const double lr = lgc - lgn; // -inf outside the window
+                const double luL = readlane_f64(p_lu, L);
+                const uint32_t mL = __builtin_amdgcn_readlane(mv_hi, L);
+                const uint32_t sL = __builtin_amdgcn_readlane(p_site, L);
+                const bool accepted = luL < lr;
+                const uint32_t c_hi = accepted ? mL : 0u;
+                hv = fma(hi_double(c_hi), jc[b], hv);
+                const uint32_t t = p_site == sL ? c_hi : 0u;
+                mv_hi += (t & 0x40000000u) << 1; // a later proposal of the same site: -m
+                dv = hi_double(mv_hi) * hv;
+                energy = accepted ? proposed : energy;
+                cur = accepted ? idx : cur;
+                lgc = accepted ? lgn : lgc;
+                cur_seen = accepted ? (actn != 0 ? 1u : 0u) : cur_seen;
+                if (!cur_seen) { // first visit: update_current_bin (wang_landau.cpp:294-297)
+                    ++active;
+                    cur_seen = 1;
+                    actm[cur] = 1; // every lane writes the same byte
+                    if (inverse_time && L < freeze) { // later attempts see the new count
+                        inc = lane > L ? batch_factors(WLX_STAGE_INVERSE_TIME, factor, active,
+                                                       clk0 + done_total + b + 1, L + 1)
+                                       : inc;
+                        const uint64_t fz = __ballot(lane > L && inc <= a.final_factor);
+                        freeze = fz ? uint32_t(__ffsll((unsigned long long)fz) - 1) : kWave;
+                        incz = lane > L ? (lane >= freeze ? 0.0 : inc) : incz;
+                    }
+                }
+                lgc = lgc + readlane_f64(incz, L);
+                lgm[cur] = lgc; // every lane writes the same value
+                rec_cm[b] = (uint64_t(c_hi) << 32) | uint32_t(cur);
+                rec_lr[b] = lr;
+            };
+            if (K == uint32_t(KB)) {
+                // snapshot for a replay
+                const double energy0 = energy, lgc0 = lgc, hv0 = hv, dv0 = dv;
+                const int32_t cur0 = cur;
+                const uint32_t seen0 = cur_seen, active0 = active, mv0 = mv_hi;
+                // a site proposed twice in the batch (the later proposal's sign depends on the
+                // earlier one's outcome)
+                const bool in_batch = lane >= used && lane < used + KB;
+                if (in_batch) duptab[p_site] = lane;
+                __syncthreads(); // the other lanes' stores, not this lane's own, are read back
+                const bool repeats = __ballot(in_batch && duptab[p_site] != lane) != 0;
+                // per-lane 1/t increments, or one increment for the batch (WL or frozen)
+                const bool per_lane = inverse_time;
+                const double inc_u = uniform_f64(inc);
+                bool redo = false;
+                auto fast_batch = [&](auto repeat_tag, auto per_lane_tag)
+                    __attribute__((always_inline)) {
+                    constexpr bool REPEATS = decltype(repeat_tag)::value;
+                    constexpr bool PER_LANE = decltype(per_lane_tag)::value;
+#pragma unroll
+                    for (int b = 0; b < KB; ++b) { // branch-free decisions
+                        const uint32_t L = used_s + uint32_t(b);
+                        const double proposed = energy + readlane_f64(dv, L);
+                        const double q = (proposed - gmin) * ginv;
+                        const double fq = floor(q);
+                        const int32_t idx = min(max(cvt_i32_sat(fq), lo), hi);
+                        const double lgn = lgm[idx];
+                        const double fr = q - fq;
+                        redo = redo || fr < margin || fr > margin_hi || q >= q_top;
+                        const double luL = readlane_f64(p_lu, L);
+                        const uint32_t mL = __builtin_amdgcn_readlane(mv_hi, L);
+                        const double incL = PER_LANE ? readlane_f64(incz, L) : inc_u;
+                        const double lr = lgc - lgn;
+                        const bool accepted = luL < lr;
+                        const uint32_t c_hi = accepted ? mL : 0u;
+                        hv = fma(hi_double(c_hi), jc[b], hv);
+                        if constexpr (REPEATS) {
+                            const uint32_t sL = __builtin_amdgcn_readlane(p_site, L);
+                            const uint32_t t = p_site == sL ? c_hi : 0u;
+                            mv_hi += (t & 0x40000000u) << 1;
+                        }
+                        dv = hi_double(mv_hi) * hv;
+                        energy = accepted ? proposed : energy;
+                        const double lg_old = accepted ? lgn : lgc;
+                        cur = accepted ? idx : cur;
+                        lgc = lg_old + incL;
+                        lgm[cur] = lgc;
+                        rec_cm[b] = (uint64_t(c_hi) << 32) | uint32_t(cur);
+                        rec_lr[b] = lr;
+                        rec_old[b] = lg_old;
+                        // the next batch's row b, one row per decision (after the last batch of
+                        // a call these are valid rows that nothing reads)
+                        const uint32_t sn = __builtin_amdgcn_readlane(next_site, next_first + uint32_t(b));
+                        stage_row<NPL>(J + size_t(sn) * kRowDoubles, next_rows + uint32_t(b) * kRowBytes);
+                    }
+                };
+                using yes = std::integral_constant<bool, true>;
+                using no = std::integral_constant<bool, false>;
+                if (repeats) {
+                    if (per_lane) fast_batch(yes{}, yes{});
+                    else fast_batch(yes{}, no{});
+                } else {
+                    if (per_lane) fast_batch(no{}, yes{});
+                    else fast_batch(no{}, no{});
+                }
+                // first visits: one atomic OR per decision's bin into its visited flag
+                const uint32_t r = (lane - used) & (KB - 1);
+                const uint32_t bin = uint32_t(rec_cm[r]) - begin;
+                uint32_t *word = reinterpret_cast<uint32_t *>(v.act + (bin & ~3u));
+                const uint32_t shift = 8 * (bin & 3u);
+                bool first = false;
+                if (in_batch)
+                    first = ((__hip_atomic_fetch_or(word, 1u << shift, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_WORKGROUP) >> shift) &
+                             0xFFu) == 0;
+                const uint64_t first_lanes = __ballot(first);
+                active += uint32_t(__popcll(first_lanes));
+                cur_seen = 1;
+                if (redo || (inverse_time && first_lanes)) {
+                    // restore the batch's start: visited flags, log_g in reverse order, registers
+                    if (first)
+                        __hip_atomic_fetch_and(word, ~(0xFFu << shift), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll 1
+                    for (int b = KB - 1; b >= 0; --b)
+                        lgm[int32_t(uint32_t(rec_cm[b]))] = rec_old[b];
+                    energy = energy0;
+                    lgc = lgc0;
+                    hv = hv0;
+                    dv = dv0;
+                    cur = cur0;
+                    cur_seen = seen0;
+                    active = active0;
+                    mv_hi = mv0;
+#pragma unroll
+                    for (int b = 0; b < KB; ++b) decide_exact(uint32_t(b));
+                }
+            } else { // the last batch of a call: nothing is staged after it
+#pragma unroll
+                for (int b = 0; b < KB; ++b)
+                    if (uint32_t(b) < K) decide_exact(uint32_t(b));
+            }
+            if (inverse_time) { // the factor of the last 1/t attempt, and a freeze
+                const uint32_t last_lane = used + K - 1;
+                factor = readlane_f64(inc, min(freeze, last_lane));
+                if (freeze <= last_lane) stage = WLX_STAGE_FROZEN;
+            }
+            // the batch's records, lane-parallel: lane used + b holds decision b
+            const bool mine = lane >= used && lane < used + K;
+            const uint32_t r = (lane - used) & (KB - 1);
+            const uint64_t cm = rec_cm[r];
+            const uint32_t m_hi = mine ? uint32_t(cm >> 32) : 0u;
+            const uint64_t acc_lanes = __ballot(m_hi != 0u);
+            n_acc += uint32_t(__popcll(acc_lanes));
+            if (acc_lanes) last_acc = done_total + (63u - uint32_t(__clzll(acc_lanes))) - used + 1u;
+            if (mine) // H[cur] += 1 of every attempt of the batch
+                __hip_atomic_fetch_add(&v.H[uint32_t(cm) - begin], uint64_t(1), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            { // drawn decisions (finite log ratio < 0) whose log(u) lay within 4 ulp of it
+                const double lr = rec_lr[r];
+                const bool amb = mine && lr < 0.0 && lr > -__builtin_huge_val() &&
+                                 near_threshold(p_lu, lr);
+                ambiguous += __popcll(__ballot(amb));
+            }
+            if (m_hi != 0) // spin flips of the accepted proposals (XOR commutes)
+                __hip_atomic_fetch_xor(&smirror[p_site & (kWave - 1)], 1u << (p_site >> 6),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            // h += m_q J[s_q, :] for the batch's proposals in order, m_q = 0 when rejected (the
+            // product is exact and h + 0 = h: fields are never -0.0)
+            const double *rowl = rows + lane;
+#pragma unroll
+            for (int g = 0; g < KB; g += 4) {
+                double rr[4][NPL];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int k = 0; k < NPL; ++k) rr[j][k] = rowl[(g + j) * kRowDoubles + k * kWave];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const double m = hi_double(__builtin_amdgcn_readlane(m_hi, used_s + uint32_t(g + j)));
+#pragma unroll
+                    for (int k = 0; k < NPL; ++k) h[k] = fma(m, rr[j][k], h[k]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < NPL; ++k) fmirror[k * kWave + lane] = h[k];
+            __syncthreads(); // the next gather reads other lanes' fields and spin bits
+            used = (used + K) & (kWave - 1);
+            if (used == 0) {
+                p_site = n_site;
+                p_lu = n_lu;
+            }
+            buf ^= 1;
+            remaining -= K;
+            done_total += K;
+        }
+        stage_wait();
+        neg = smirror[lane];
+        asm volatile("" ::: "memory");
+        const uint64_t accepted0 = st.accepted, ambiguous0 = st.ambiguous;
+        st = a.state[w];
+        st.attempted = att0 + done_total;
+        st.accepted = accepted0 + n_acc;
+        if (last_acc) st.last_accepted = att0 + last_acc;
+        st.stage = stage;
+        st.factor = factor;
+        st.active_count = active;
+        st.ambiguous = ambiguous0 + ambiguous;
+    }
+    __syncthreads();
+    finish_walker<NPL>(a, w, st, v, ws, neg, h, energy, flags);
+}
+
+template <int NPL>
+__global__ __launch_bounds__(kWave) void lookahead_staged_kernel(const Args *table,
+                                                                 const BatchSlot *slots,
+                                                                 uint64_t request, uint32_t flags)
+{
+    const BatchSlot slot = slots[blockIdx.x];
+    const Args a = table[slot.engine];
+    extern __shared__ __align__(16) unsigned char smem[];
+    staged_walk<NPL>(a, slot.walker, request, flags, smem);
+}
+
+
+// ---------------------------------------------------------------------------------------------
 // lookahead sampling on the reference's Xoshiro256** stream (common configuration: no trace,
 // forcing or Nalivaiko mask). The walker consumes its stream word by word: a site draw (one
 // word, repeated while bounded() rejects it, rng.hpp:29-36), then a uniform only when the
@@ -2490,6 +2931,7 @@ struct KernelSet {
     SampleFn sample_xo, sample_xo_fast, sample_ph, sample_ph_seq;
     SampleFn ph_fast_width[4]; // lookahead_fast_kernel<NPL, kLookaheadWidths[k]>
     SampleFn lds_xo, lds_ph; // any N
+    SampleFn staged; // lookahead_staged_kernel<NPL> (NPL 2, 4, 8), else null
     F32Fn sample_f32;
     InitFn init;
     FieldsFn fields;
@@ -2502,8 +2944,10 @@ template <int NPL> KernelSet kernel_set()
             sample_kernel<NPL, true>,
             {lookahead_fast_kernel<NPL, 4>, lookahead_fast_kernel<NPL, 8>,
              lookahead_fast_kernel<NPL, 16>, lookahead_fast_kernel<NPL, 32>},
-            lds_sample_kernel<false>, lds_sample_kernel<true>, lookahead_f32_kernel<NPL>,
-            init_kernel<NPL>, fields_kernel<NPL>, pilot_kernel<NPL>};
+            lds_sample_kernel<false>, lds_sample_kernel<true>,
+            (NPL >= 2 && NPL <= 8) ? lookahead_staged_kernel<(NPL >= 2 && NPL <= 8 ? NPL : 2)>
+                                   : nullptr,
+            lookahead_f32_kernel<NPL>, init_kernel<NPL>, fields_kernel<NPL>, pilot_kernel<NPL>};
 }
 
 KernelSet kernels_for(uint32_t npl)
@@ -2614,10 +3058,13 @@ struct wlx_engine {
     hipStream_t stream = nullptr;
     bool initialized = false;
     size_t lds_sample = 0, lds_init = 0, lds_walker = 0;
+    size_t lds_staged = 0; // the staged lookahead walker's LDS (0: not available)
+    uint32_t compute_units = 0; // of the engine's device
     // windows too wide for LDS: bins stay in HBM and only the LDS walker samples
     bool bins_hbm = false;
     uint64_t kernel_choice = WLX_KERNEL_AUTO;
     int width_index = kDefaultWidth; // lookahead width of the Philox common-configuration kernel
+    bool width_forced = false;       // set by wlx_engine_set_lookahead_width
     bool timing = false;
     uint64_t timing_every = 1, timing_count = 0; // time launches timing_count % every == 0
     std::vector<hipEvent_t> events;
@@ -2714,7 +3161,18 @@ struct wlx_engine {
         return WLX_OK;
     }
 
-    SampleFn sample_fn() const
+    // the staged lookahead walker serves this engine: Philox common configuration, 2 to 8
+    // sites per lane, its row buffers and window bins within a CU's LDS
+    bool staged_eligible() const
+    {
+        return kernels.staged && lds_staged && !f32 && !collect && !bins_hbm &&
+               cfg.rng == WLX_RNG_PHILOX && trace_cap == 0 &&
+               cfg.wl.force_accept_after_attempts == 0 && !cfg.wl.nalivaiko_mod;
+    }
+
+    // the sampling kernel of the next launch; `batched`: the kernel this engine contributes
+    // to a wlx_batch, where the staged walker (most of a CU's LDS per walker) is not used
+    SampleFn sample_fn(bool batched = false) const
     {
         if (f32) return nullptr; // lookahead_f32_kernel, launched by value (launch_sample)
         const bool philox = cfg.rng == WLX_RNG_PHILOX;
@@ -2727,8 +3185,15 @@ struct wlx_engine {
         const bool common =
             trace_cap == 0 && cfg.wl.force_accept_after_attempts == 0 && !cfg.wl.nalivaiko_mod;
         if (cfg.rng != WLX_RNG_PHILOX) return common ? kernels.sample_xo_fast : kernels.sample_xo;
-        if (trace_cap == 0 && cfg.wl.force_accept_after_attempts == 0 && !cfg.wl.nalivaiko_mod)
+        if (trace_cap == 0 && cfg.wl.force_accept_after_attempts == 0 && !cfg.wl.nalivaiko_mod) {
+            // AUTO: while every walker can have a CU's LDS to itself (the staged walker holds
+            // one workgroup per CU; more walkers queue behind each other)
+            const bool staged = kernel_choice == WLX_KERNEL_STAGED ||
+                                (kernel_choice == WLX_KERNEL_AUTO && !width_forced &&
+                                 walkers <= compute_units);
+            if (staged && !batched && staged_eligible()) return kernels.staged;
             return kernels.ph_fast_width[width_index];
+        }
         return kernels.sample_ph;
     }
 
@@ -2743,9 +3208,10 @@ struct wlx_engine {
         }
         const SampleFn fn = sample_fn();
         const bool lds_fn = fn == kernels.lds_xo || fn == kernels.lds_ph;
+        const size_t lds = lds_fn ? lds_walker : fn == kernels.staged ? lds_staged : lds_sample;
         if (int status = upload_args()) return status;
-        hipLaunchKernelGGL(fn, dim3(walkers), dim3(kWave), lds_fn ? lds_walker : lds_sample, stream,
-                           d_args, d_slots, request, flags);
+        hipLaunchKernelGGL(fn, dim3(walkers), dim3(kWave), lds, stream, d_args, d_slots, request,
+                           flags);
         REWL_HIP(hipGetLastError());
         return WLX_OK;
     }
@@ -3064,6 +3530,21 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
     }
     if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.init), e->lds_init))
         return status;
+    {
+        int device = 0;
+        REWL_HIP(hipGetDevice(&device));
+        int cus = 0;
+        REWL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        e->compute_units = uint32_t(cus);
+    }
+    if (e->kernels.staged && !e->bins_hbm) { // candidate rows staged in LDS
+        const size_t bytes = staged_layout(e->ws_stride, uint32_t(sys->np)).total;
+        if (bytes <= kLdsBytes) {
+            if (int status = set_lds_limit(reinterpret_cast<const void *>(e->kernels.staged), bytes))
+                return status;
+            e->lds_staged = bytes;
+        }
+    }
     { // the LDS walker stages the configuration in LDS beside the bins (or alone)
         e->lds_walker = lds_walker_bytes(e->ws_stride, uint32_t(sys->n), e->bins_hbm);
         if (e->lds_walker <= kLdsBytes)
@@ -3358,7 +3839,11 @@ int wlx_engine_warm_start(wlx_engine *e, const double *log_g, double factor)
 int wlx_engine_set_kernel(wlx_engine *e, uint64_t kernel)
 {
     if (int status = require_engine(e, false)) return status;
-    if (kernel > WLX_KERNEL_LDS) return set_error(WLX_INVALID_ARGUMENT, "Unknown walker kernel");
+    if (kernel > WLX_KERNEL_STAGED) return set_error(WLX_INVALID_ARGUMENT, "Unknown walker kernel");
+    if (kernel == WLX_KERNEL_STAGED && !e->staged_eligible())
+        return set_error(WLX_INVALID_ARGUMENT,
+                         "The staged walker needs the Philox stream in the common configuration, "
+                         "2 to 8 sites per lane and its rows and window bins within LDS");
     if (kernel == WLX_KERNEL_LDS && e->lds_walker > kLdsBytes)
         return set_error(WLX_INVALID_ARGUMENT, "Fields and window bins exceed the LDS of a CU");
     if (e->bins_hbm && kernel != WLX_KERNEL_LDS && kernel != WLX_KERNEL_AUTO)
@@ -3368,10 +3853,10 @@ int wlx_engine_set_kernel(wlx_engine *e, uint64_t kernel)
     return WLX_OK;
 }
 
-int wlx_engine_kernel(const wlx_engine *e, uint64_t *kernel, char *name, size_t capacity)
+namespace {
+// family and name of sampling kernel `fn` of engine e
+void kernel_name(const wlx_engine *e, SampleFn fn, uint64_t *kernel, char *name, size_t capacity)
 {
-    if (!e) return set_error(WLX_INVALID_ARGUMENT, "Null engine");
-    const SampleFn fn = e->sample_fn();
     const KernelSet &k = e->kernels;
     uint64_t family = WLX_KERNEL_LOOKAHEAD;
     const char *base = "lookahead_kernel";
@@ -3386,6 +3871,9 @@ int wlx_engine_kernel(const wlx_engine *e, uint64_t *kernel, char *name, size_t 
     } else if (fn == k.ph_fast_width[0] || fn == k.ph_fast_width[1] || fn == k.ph_fast_width[2] ||
                fn == k.ph_fast_width[3]) {
         base = "lookahead_fast_kernel";
+    } else if (k.staged && fn == k.staged) {
+        family = WLX_KERNEL_STAGED;
+        base = "lookahead_staged_kernel";
     } else if (fn == nullptr) { // FP32 fields
         base = "lookahead_f32_kernel";
     }
@@ -3403,6 +3891,13 @@ int wlx_engine_kernel(const wlx_engine *e, uint64_t *kernel, char *name, size_t 
         else
             std::snprintf(name, capacity, "%s<%u>", base, unsigned(e->sys->npl));
     }
+}
+} // namespace
+
+int wlx_engine_kernel(const wlx_engine *e, uint64_t *kernel, char *name, size_t capacity)
+{
+    if (!e) return set_error(WLX_INVALID_ARGUMENT, "Null engine");
+    kernel_name(e, e->sample_fn(), kernel, name, capacity);
     return WLX_OK;
 }
 
@@ -3418,6 +3913,7 @@ int wlx_engine_set_lookahead_width(wlx_engine *e, uint64_t width)
     for (int k = 0; k < 4; ++k)
         if (kLookaheadWidths[k] == width) {
             e->width_index = k;
+            e->width_forced = true;
             return WLX_OK;
         }
     return set_error(WLX_INVALID_ARGUMENT, "Lookahead width must be 4, 8, 16 or 32");
@@ -3428,7 +3924,7 @@ int wlx_engine_autotune(wlx_engine *e, uint64_t epochs, uint64_t *chosen, double
     if (int status = require_engine(e)) return status;
     if (epochs == 0) return set_error(WLX_INVALID_ARGUMENT, "Autotuning needs at least one epoch");
     if (seconds)
-        for (int k = 0; k <= int(WLX_KERNEL_LDS); ++k) seconds[k] = -1.0;
+        for (int k = 0; k <= int(WLX_KERNEL_STAGED); ++k) seconds[k] = -1.0;
     // (family, lookahead width index); the Philox common-configuration lookahead kernel is
     // tried at each width shown exact for this size
     std::vector<std::pair<uint64_t, int>> candidates;
@@ -3437,6 +3933,7 @@ int wlx_engine_autotune(wlx_engine *e, uint64_t epochs, uint64_t *chosen, double
                             e->trace_cap == 0 && e->cfg.wl.force_accept_after_attempts == 0 &&
                             !e->cfg.wl.nalivaiko_mod;
         if (widths) {
+            if (e->staged_eligible()) candidates.push_back({WLX_KERNEL_STAGED, kDefaultWidth});
             for (int k : {kDefaultWidth, 1, 3})
                 if (kLookaheadWidths[k] != 32 || e->sys->npl <= 16)
                     candidates.push_back({WLX_KERNEL_LOOKAHEAD, k});
@@ -4166,7 +4663,7 @@ int wlx_batch_create(wlx_engine *const *engines, uint64_t count, wlx_batch **out
     auto b = std::make_unique<wlx_batch>();
     // every engine on one register kernel (one family and register layout): that kernel runs
     // the batch; any other mix runs the LDS walker (the trajectories are the same)
-    SampleFn common = engines[0] ? engines[0]->sample_fn() : nullptr;
+    SampleFn common = engines[0] ? engines[0]->sample_fn(true) : nullptr;
     size_t common_lds = 0;
     for (uint64_t k = 0; k < count; ++k) {
         wlx_engine *e = engines[k];
@@ -4180,7 +4677,7 @@ int wlx_batch_create(wlx_engine *const *engines, uint64_t count, wlx_batch **out
         if (e->f32 || e->collect || e->trace_cap)
             return set_error(WLX_INVALID_ARGUMENT,
                              "Batched engines run FP64 fields without statistics or traces");
-        if (e->sample_fn() != common) common = nullptr;
+        if (e->sample_fn(true) != common) common = nullptr;
         REWL_HIP(hipStreamSynchronize(e->stream));
         common_lds = std::max(common_lds, e->lds_sample);
         b->engines.push_back(e);
@@ -4189,7 +4686,7 @@ int wlx_batch_create(wlx_engine *const *engines, uint64_t count, wlx_batch **out
         b->common = common;
         b->common_lds = common_lds;
         char name[64];
-        if (int status = wlx_engine_kernel(engines[0], nullptr, name, sizeof name)) return status;
+        kernel_name(engines[0], common, nullptr, name, sizeof name);
         b->kernel_name = name;
     } else {
         b->kernel_name = b->philox ? "lds_sample_kernel<true>" : "lds_sample_kernel<false>";

@@ -389,7 +389,7 @@ class Engine:
         """Runs `epochs` epochs of the live run with each eligible kernel family and keeps the
         fastest: (chosen family, seconds per family, -1 where not tried)."""
         chosen = ctypes.c_uint64()
-        seconds = (ctypes.c_double * 4)()
+        seconds = (ctypes.c_double * 5)()
         self.backend.call("engine_autotune", self.handle, ctypes.c_uint64(epochs),
                           ctypes.byref(chosen), seconds)
         return chosen.value, list(seconds)
