@@ -126,7 +126,8 @@ struct BatchSlot {
 enum : uint32_t { kGate = 1, kCheck = 2, kTrace = 4 };
 
 // ---- wave helpers -------------------------------------------------------------------------
-__device__ inline uint32_t lane_id() { return threadIdx.x; }
+// lane within the wave (walker kernels run one wave per workgroup, the staged walker two)
+__device__ inline uint32_t lane_id() { return threadIdx.x & (kWave - 1); }
 
 __device__ inline double readlane_f64(double v, uint32_t lane)
 {
@@ -1437,16 +1438,23 @@ __global__ __launch_bounds__(kWave) void lookahead_fast_kernel(const Args *table
 
 
 // ---------------------------------------------------------------------------------------------
-// lookahead sampling with the candidate rows staged in LDS (common configuration, Philox
-// stream, 65 to 512 sites). The decisions are those of fast_walk, in the same order, arranged
-// so that a walker's serial chain holds little besides the arithmetic of the decision and one
-// LDS read (one wave issues one vector instruction per 4 cycles, in order, so every
-// instruction on the decision loop costs time whether or not it is on the dependency chain):
-//   - the 16 rows J[s_q, :] of a batch's proposals are copied from L2 into LDS by
-//     global_load_lds_dwordx4 (1 KiB per instruction) while the previous batch decides, into
-//     one of two buffers, one row per decision; the couplings among the proposals come from
-//     LDS, and the accepted rows are FMA'd into the register fields from LDS;
-//   - the fields and spin bits are mirrored in LDS, so the batch's gather is one LDS read;
+// lookahead sampling with the candidate rows staged in LDS, two waves per walker (common
+// configuration, Philox stream, 65 to 512 sites). The decisions are those of fast_walk, in the
+// same order. One wave issues one vector instruction per 4 cycles, in order, so a walker's
+// speed is set by the instructions its decision loop holds; here the loop holds little besides
+// the arithmetic of each decision and one LDS read, and everything else runs on a second wave
+// of the same workgroup, on another SIMD of the CU:
+//   - wave A (the decider) owns the walker: energy, current bin, log_g, visited flags and spin
+//     bits. Per batch it gathers the field at each proposal site, decides the batch and writes
+//     one record per decision (bin, accepted m, log ratio);
+//   - wave B (the worker) owns the register fields. It applies the previous batch's records
+//     (H, ambiguity count, accepted count), FMA's its accepted rows into the fields and mirrors
+//     them in LDS, draws the Philox proposals one block ahead and copies the next batch's rows
+//     J[s_q, :] from L2 into LDS with global_load_lds_dwordx4 (1 KiB per instruction);
+//   - one workgroup barrier per batch. B's field mirror lags A by one batch, so A's gather adds
+//     the previous batch's accepted rows itself, at the new proposal sites only, read from the
+//     new batch's staged rows (J is symmetric) in acceptance order: the same additions, in the
+//     same order, as the full row update;
 //   - the window's log_g has a +inf entry on either side (relative bins -1 and ws), and the
 //     proposed bin is clamped onto them, so a proposal outside the window gives log ratio -inf
 //     and is rejected by the same comparison log(u) < log ratio that accepts the others (log u
@@ -1462,22 +1470,21 @@ __global__ __launch_bounds__(kWave) void lookahead_fast_kernel(const Args *table
 //     of the value it replaced, in reverse order) and visited flags are restored, and the batch
 //     is decided again with the exact quotient, first visits and the 1/t refresh in line;
 //   - a batch without a repeated proposal site runs a loop without the sign bookkeeping of
-//     repeats, and outside the 1/t stage the log_g increment is one value for the batch;
-//   - each decision records its bin, accepted m, log ratio and previous log_g in LDS; H, the
-//     ambiguity count, the spin flips and the accepted count are applied lane-parallel after
-//     the batch.
+//     repeats, and outside the 1/t stage the log_g increment is one value for the batch.
 // ---------------------------------------------------------------------------------------------
 constexpr int kStagedBatch = 16;
+constexpr int kStagedThreads = 2 * kWave;
 
 struct StagedLayout {
-    uint32_t lg, H, act, ref, rows, fields, spins, dups, rec_cm, rec_lr, rec_old;
+    uint32_t lg, H, act, ref, rows, fields, spins, dups, psite, plu, rec_cm, rec_lr, rec_old, fin;
     uint32_t total;
 };
 
 // LDS offsets (bytes) of the staged walker: log_g with one entry below the window (lg - 8) and
 // one past it, H, the visited flags (4-byte aligned, with spare bytes on either side), the
 // refine flags, two row buffers of kStagedBatch rows each, the field and spin-bit mirrors, a
-// site table for finding repeated proposals, and the decision records
+// site table for finding repeated proposals, two blocks of 64 proposals (sites, log u), two
+// batches of decision records, the log_g each decision replaced, and the waves' final values
 __host__ __device__ inline StagedLayout staged_layout(uint32_t stride, uint32_t np)
 {
     StagedLayout s;
@@ -1489,10 +1496,13 @@ __host__ __device__ inline StagedLayout staged_layout(uint32_t stride, uint32_t 
     s.fields = s.rows + 2 * kStagedBatch * np * 8;
     s.spins = s.fields + np * 8;
     s.dups = s.spins + kWave * 4;
-    s.rec_cm = s.dups + np * 4;
-    s.rec_lr = s.rec_cm + kStagedBatch * 8;
-    s.rec_old = s.rec_lr + kStagedBatch * 8;
-    s.total = s.rec_old + kStagedBatch * 8;
+    s.psite = s.dups + np * 4;
+    s.plu = s.psite + 2 * kWave * 4;
+    s.rec_cm = s.plu + 2 * kWave * 8;
+    s.rec_lr = s.rec_cm + 2 * kStagedBatch * 8;
+    s.rec_old = s.rec_lr + 2 * kStagedBatch * 8;
+    s.fin = s.rec_old + kStagedBatch * 8;
+    s.total = s.fin + 8 * 8;
     return s;
 }
 
@@ -1510,8 +1520,15 @@ __device__ inline int32_t cvt_i32_sat(double x)
 // Counted by vmcnt; stage_wait waits for them. M0 is written here only: the staged walker's
 // own code uses no M0.
 template <int NPL>
-__device__ inline void stage_row(const double *row, uint32_t dst)
+__device__ inline void stage_row(const double *row_any, uint32_t dst_any)
 {
+    // both are wave-uniform; readfirstlane places them in scalar registers for the "s" operands
+    // (the builtin returns int: each half goes through uint32_t, so none is sign-extended)
+    const uint32_t dst = uint32_t(__builtin_amdgcn_readfirstlane(dst_any));
+    const uint64_t ra = reinterpret_cast<uint64_t>(row_any);
+    const uint32_t ra_hi = uint32_t(__builtin_amdgcn_readfirstlane(uint32_t(ra >> 32)));
+    const uint32_t ra_lo = uint32_t(__builtin_amdgcn_readfirstlane(uint32_t(ra)));
+    const double *row = reinterpret_cast<const double *>((uint64_t(ra_hi) << 32) | ra_lo);
     const uint32_t voff = lane_id() * 16;
     if constexpr (NPL == 2) {
         asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\t"
@@ -1546,6 +1563,9 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
     constexpr uint32_t kRowDoubles = kRowBytes / 8;
     constexpr uint32_t kBufBytes = KB * kRowBytes;
     const uint32_t lane = lane_id();
+    // wave A decides, wave B works; the role is uniform in each wave, which readfirstlane tells
+    // the compiler (its code then keeps wave-uniform values scalar)
+    const bool decider = __builtin_amdgcn_readfirstlane(threadIdx.x) < uint32_t(kWave);
     if ((flags & kGate) && a.glob->done) return;
     WalkerState st = a.state[w];
     if (st.escaped) return;
@@ -1556,31 +1576,64 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
     double *fmirror = reinterpret_cast<double *>(smem + lay.fields);
     uint32_t *smirror = reinterpret_cast<uint32_t *>(smem + lay.spins);
     uint32_t *duptab = reinterpret_cast<uint32_t *>(smem + lay.dups);
+    uint32_t *psite = reinterpret_cast<uint32_t *>(smem + lay.psite);
+    double *plu = reinterpret_cast<double *>(smem + lay.plu);
     uint64_t *rec_cm = reinterpret_cast<uint64_t *>(smem + lay.rec_cm);
     double *rec_lr = reinterpret_cast<double *>(smem + lay.rec_lr);
     double *rec_old = reinterpret_cast<double *>(smem + lay.rec_old);
+    uint64_t *fin = reinterpret_cast<uint64_t *>(smem + lay.fin);
     const uint32_t rows_lds = uint32_t(reinterpret_cast<uintptr_t>(smem)) + lay.rows;
+    // both waves load the same bins and configuration (the stores agree)
     load_bins(a, w, ws, v);
-    if (lane == 0) {
-        v.lg[-1] = __builtin_huge_val();
-        v.lg[ws] = __builtin_huge_val();
-    }
     uint64_t neg;
     double h[NPL];
     load_config<NPL>(a, st.slot, neg, h);
+    if (decider) {
+        if (lane == 0) {
+            v.lg[-1] = __builtin_huge_val();
+            v.lg[ws] = __builtin_huge_val();
+        }
+        smirror[lane] = uint32_t(neg);
+    } else {
 #pragma unroll
-    for (int k = 0; k < NPL; ++k) fmirror[k * kWave + lane] = h[k];
-    smirror[lane] = uint32_t(neg);
+        for (int k = 0; k < NPL; ++k) fmirror[k * kWave + lane] = h[k];
+    }
     double energy = uniform_f64(a.energy[st.slot]);
-    __syncthreads();
     const uint64_t count = batch_count(a, st, request, flags);
     const int64_t start_bin = grid_index(a.grid, energy);
+    __syncthreads();
     if (count > 0 && (start_bin < int64_t(st.begin) || start_bin >= int64_t(st.end))) {
         st.escaped = 1; // wang_landau.cpp:247-248, decided up front as in the general kernel
     } else if (count > 0) {
         const uint32_t begin = st.begin;
         const uint64_t att0 = st.attempted, clk0 = att0 + st.clock_offset;
         const double *J = a.J;
+        const uint32_t nb = uint32_t((count + KB - 1) / KB); // batches, the last one partial
+        // proposals of attempts att0 + first + lane (one Philox4x64 block per two attempts)
+        auto draw = [&](uint64_t first, uint32_t &site, double &lu) {
+            const uint64_t t = first + lane;
+            const Philox4 blk = philox4x64(t >> 1, kPhiloxWalkTag, 0, 0, a.seed, uint64_t(w));
+            const bool odd = t & 1;
+            site = uint32_t(__umul64hi(odd ? blk.v[2] : blk.v[0], uint64_t(a.n)));
+            const double u = double((odd ? blk.v[3] : blk.v[1]) >> 11) * 0x1.0p-53;
+            lu = fmin(log(fmax(u, 0x1.0p-53)), -0x1.0p-1074); // < 0: see the header
+        };
+        // ---- wave B: block 0 of the proposals and the first batch's rows ----
+        uint32_t b_site = 0; // B: the proposal sites of the current block
+        uint32_t n_acc = 0, last_acc = 0;
+        uint64_t ambiguous = 0;
+        if (!decider) {
+            double lu;
+            draw(att0, b_site, lu);
+            psite[lane] = b_site;
+            plu[lane] = lu;
+#pragma unroll
+            for (int q = 0; q < KB; ++q)
+                stage_row<NPL>(J + size_t(uint32_t(__builtin_amdgcn_readlane(b_site, uint32_t(q)))) * kRowDoubles,
+                               rows_lds + uint32_t(q) * kRowBytes);
+            stage_wait();
+        }
+        // ---- wave A: the walker ----
         const double gmin = a.grid.minimum, ginv = a.grid.inverse_width;
         const double margin = a.grid.margin, margin_hi = a.grid.margin_hi;
         // the product of the grid maximum: at or above it the quotient decides (top window only)
@@ -1591,283 +1644,327 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
         double *lgm = v.lg - begin; // by absolute bin
         uint8_t *actm = v.act - begin;
         int32_t cur = int32_t(start_bin); // absolute bin
-        double lgc = uniform_f64(lgm[cur]);
-        uint32_t cur_seen = uniform_u32(actm[cur]) != 0 ? 1u : 0u;
+        double lgc = 0.0;
+        uint32_t cur_seen = 0;
+        if (decider) {
+            lgc = uniform_f64(lgm[cur]);
+            cur_seen = uniform_u32(actm[cur]) != 0 ? 1u : 0u;
+        }
         uint32_t stage = uniform_u32(st.stage), active = uniform_u32(st.active_count);
         double factor = uniform_f64(st.factor);
-        uint32_t done_total = 0, n_acc = 0, last_acc = 0;
-        uint64_t ambiguous = 0;
-        // proposals of the current block of 64 attempts, and of the next one (drawn one batch
-        // before it starts, so that its first rows can be staged)
-        auto draw = [&](uint64_t first, uint32_t &site, double &lu) {
-            const uint64_t t = first + lane;
-            const Philox4 blk = philox4x64(t >> 1, kPhiloxWalkTag, 0, 0, a.seed, uint64_t(w));
-            const bool odd = t & 1;
-            site = uint32_t(__umul64hi(odd ? blk.v[2] : blk.v[0], uint64_t(a.n)));
-            const double u = double((odd ? blk.v[3] : blk.v[1]) >> 11) * 0x1.0p-53;
-            lu = fmin(log(fmax(u, 0x1.0p-53)), -0x1.0p-1074); // < 0: see the header
-        };
-        uint32_t p_site, n_site = 0;
-        double p_lu, n_lu = 0.0;
-        draw(att0, p_site, p_lu);
-        uint32_t used = 0, buf = 0;
+        __syncthreads();
+        for (uint32_t i = 0; i <= nb; ++i) {
+            if (decider && i < nb) {
+                // ================= wave A: decide batch i =================
+                const uint32_t used = (i * KB) & (kWave - 1);
+                const uint32_t blk = ((i * KB) / kWave) & 1;
+                const uint32_t done_total = i * KB;
+                const uint64_t left = count - done_total;
+                const uint32_t K = left < uint64_t(KB) ? uint32_t(left) : uint32_t(KB);
+                const uint32_t p_site = psite[blk * kWave + lane];
+                const double p_lu = plu[blk * kWave + lane];
+                const double *rows = reinterpret_cast<const double *>(smem + lay.rows + (i & 1) * kBufBytes);
+                // gather: B's mirror holds the fields after batch i - 2; batch i - 1's accepted
+                // rows are added at this batch's sites, in order, from this batch's rows
+                const uint32_t j = (lane - used) & (KB - 1); // this lane's position in the batch
+                const double *rowj = rows + j * kRowDoubles;
+                double hv = fmirror[p_site];
+                if (i > 0) {
+                    const uint32_t pblk = (((i - 1) * KB) / kWave) & 1;
+                    const uint32_t pused = ((i - 1) * KB) & (kWave - 1);
+                    const uint64_t *pcm = rec_cm + ((i - 1) & 1) * KB;
 #pragma unroll
-        for (int q = 0; q < KB; ++q)
-            stage_row<NPL>(J + size_t(__builtin_amdgcn_readlane(p_site, uint32_t(q))) * kRowDoubles,
-                           rows_lds + uint32_t(q) * kRowBytes);
-        uint64_t remaining = count;
-        while (remaining > 0) {
-            const uint32_t K = remaining < uint64_t(KB) ? uint32_t(remaining) : uint32_t(KB);
-            const bool more = remaining > uint64_t(K);
-            const bool block_end = used + KB == kWave;
-            if (more && block_end) draw(att0 + done_total + KB, n_site, n_lu);
-            const uint32_t next_site = block_end ? n_site : p_site;
-            const uint32_t next_first = block_end ? 0u : used + KB;
-            const uint32_t next_rows = rows_lds + (buf ^ 1) * kBufBytes;
-            stage_wait(); // this batch's rows (staged one batch ago) are in LDS
-            // gather: field, spin and couplings J[s_q][s_lane] at each proposal of the batch
-            const double *rows = reinterpret_cast<const double *>(smem + lay.rows + buf * kBufBytes);
-            double hv = fmirror[p_site];
-            const uint32_t bits = smirror[p_site & (kWave - 1)];
-            uint32_t mv_hi = ((bits >> (p_site >> 6)) & 1) ? 0x40000000u : 0xC0000000u; // -2 sigma
-            double jc[KB];
-#pragma unroll
-            for (int q = 0; q < KB; ++q) jc[q] = rows[q * kRowDoubles + p_site];
-            const bool inverse_time = stage == WLX_STAGE_INVERSE_TIME;
-            double inc = batch_factors(stage, factor, active, clk0 + done_total, used);
-            uint32_t freeze = kWave; // lane of the freezing attempt; kWave: none
-            if (inverse_time) {
-                const uint64_t fz = __ballot(lane >= used && inc <= a.final_factor);
-                freeze = fz ? uint32_t(__ffsll((unsigned long long)fz) - 1) : kWave;
-            }
-            double incz = lane >= freeze ? 0.0 : inc;
-            double dv = hi_double(mv_hi) * hv; // flip_delta of each proposal (physics.cpp:261-266)
-            const uint32_t used_s = uniform_u32(used);
-            // the exact decision: the correctly rounded quotient where the product is not known
-            // exact, first visits in line, refreshing the 1/t factors of later attempts
-            auto decide_exact = [&](uint32_t b) __attribute__((always_inline)) {
-                const uint32_t L = used_s + b;
-                const double proposed = energy + readlane_f64(dv, L);
-                const double q = (proposed - gmin) * ginv;
-                const double fq = floor(q);
-                int32_t idx = min(max(cvt_i32_sat(fq), lo), hi);
-                const double fr = q - fq; // exact: fq <= q < fq + 1
-                if (fr < margin || fr > margin_hi || q >= q_top) {
-                    bool in;
-                    const uint32_t g = grid_bin(a.grid, proposed, in);
-                    idx = in ? min(max(int32_t(g), lo), hi) : lo;
-                }
-                const double lgn = lgm[idx];
-                const uint32_t actn = actm[idx];
-                const double lr = lgc - lgn; // -inf outside the window
-                const double luL = readlane_f64(p_lu, L);
-                const uint32_t mL = __builtin_amdgcn_readlane(mv_hi, L);
-                const uint32_t sL = __builtin_amdgcn_readlane(p_site, L);
-                const bool accepted = luL < lr;
-                const uint32_t c_hi = accepted ? mL : 0u;
-                hv = fma(hi_double(c_hi), jc[b], hv);
-                const uint32_t t = p_site == sL ? c_hi : 0u;
-                mv_hi += (t & 0x40000000u) << 1; // a later proposal of the same site: -m
-                dv = hi_double(mv_hi) * hv;
-                energy = accepted ? proposed : energy;
-                cur = accepted ? idx : cur;
-                lgc = accepted ? lgn : lgc;
-                cur_seen = accepted ? (actn != 0 ? 1u : 0u) : cur_seen;
-                if (!cur_seen) { // first visit: update_current_bin (wang_landau.cpp:294-297)
-                    ++active;
-                    cur_seen = 1;
-                    actm[cur] = 1; // every lane writes the same byte
-                    if (inverse_time && L < freeze) { // later attempts see the new count
-                        inc = lane > L ? batch_factors(WLX_STAGE_INVERSE_TIME, factor, active,
-                                                       clk0 + done_total + b + 1, L + 1)
-                                       : inc;
-                        const uint64_t fz = __ballot(lane > L && inc <= a.final_factor);
-                        freeze = fz ? uint32_t(__ffsll((unsigned long long)fz) - 1) : kWave;
-                        incz = lane > L ? (lane >= freeze ? 0.0 : inc) : incz;
+                    for (int q = 0; q < KB; ++q) {
+                        const uint32_t sa = psite[pblk * kWave + pused + uint32_t(q)];
+                        const uint32_t ma = uint32_t(pcm[q] >> 32);
+                        hv = fma(hi_double(ma), rowj[sa], hv);
                     }
                 }
-                lgc = lgc + readlane_f64(incz, L);
-                lgm[cur] = lgc; // every lane writes the same value
-                rec_cm[b] = (uint64_t(c_hi) << 32) | uint32_t(cur);
-                rec_lr[b] = lr;
-            };
-            if (K == uint32_t(KB)) {
-                // snapshot for a replay
-                const double energy0 = energy, lgc0 = lgc, hv0 = hv, dv0 = dv;
-                const int32_t cur0 = cur;
-                const uint32_t seen0 = cur_seen, active0 = active, mv0 = mv_hi;
-                // a site proposed twice in the batch (the later proposal's sign depends on the
-                // earlier one's outcome)
-                const bool in_batch = lane >= used && lane < used + KB;
-                if (in_batch) duptab[p_site] = lane;
-                __syncthreads(); // the other lanes' stores, not this lane's own, are read back
-                const bool repeats = __ballot(in_batch && duptab[p_site] != lane) != 0;
-                // per-lane 1/t increments, or one increment for the batch (WL or frozen)
-                const bool per_lane = inverse_time;
-                const double inc_u = uniform_f64(inc);
-                bool redo = false;
-                auto fast_batch = [&](auto repeat_tag, auto per_lane_tag)
-                    __attribute__((always_inline)) {
-                    constexpr bool REPEATS = decltype(repeat_tag)::value;
-                    constexpr bool PER_LANE = decltype(per_lane_tag)::value;
+                const uint32_t bits = smirror[p_site & (kWave - 1)];
+                uint32_t mv_hi = ((bits >> (p_site >> 6)) & 1) ? 0x40000000u : 0xC0000000u; // -2 sigma
+                double jc[KB];
 #pragma unroll
-                    for (int b = 0; b < KB; ++b) { // branch-free decisions
-                        const uint32_t L = used_s + uint32_t(b);
-                        const double proposed = energy + readlane_f64(dv, L);
-                        const double q = (proposed - gmin) * ginv;
-                        const double fq = floor(q);
-                        const int32_t idx = min(max(cvt_i32_sat(fq), lo), hi);
-                        const double lgn = lgm[idx];
-                        const double fr = q - fq;
-                        redo = redo || fr < margin || fr > margin_hi || q >= q_top;
-                        const double luL = readlane_f64(p_lu, L);
-                        const uint32_t mL = __builtin_amdgcn_readlane(mv_hi, L);
-                        const double incL = PER_LANE ? readlane_f64(incz, L) : inc_u;
-                        const double lr = lgc - lgn;
-                        const bool accepted = luL < lr;
-                        const uint32_t c_hi = accepted ? mL : 0u;
-                        hv = fma(hi_double(c_hi), jc[b], hv);
-                        if constexpr (REPEATS) {
-                            const uint32_t sL = __builtin_amdgcn_readlane(p_site, L);
-                            const uint32_t t = p_site == sL ? c_hi : 0u;
-                            mv_hi += (t & 0x40000000u) << 1;
+                for (int q = 0; q < KB; ++q) jc[q] = rows[q * kRowDoubles + p_site];
+                const bool inverse_time = stage == WLX_STAGE_INVERSE_TIME;
+                double inc = batch_factors(stage, factor, active, clk0 + done_total, used);
+                uint32_t freeze = kWave; // lane of the freezing attempt; kWave: none
+                if (inverse_time) {
+                    const uint64_t fz = __ballot(lane >= used && inc <= a.final_factor);
+                    freeze = fz ? uint32_t(__ffsll((unsigned long long)fz) - 1) : kWave;
+                }
+                double incz = lane >= freeze ? 0.0 : inc;
+                double dv = hi_double(mv_hi) * hv; // flip_delta of each proposal (physics.cpp:261-266)
+                const uint32_t used_s = uniform_u32(used);
+                uint64_t *rcm = rec_cm + (i & 1) * KB;
+                double *rlr = rec_lr + (i & 1) * KB;
+                // the exact decision: the correctly rounded quotient where the product is not
+                // known exact, first visits in line, refreshing the 1/t factors of later attempts
+                auto decide_exact = [&](uint32_t b) __attribute__((always_inline)) {
+                    const uint32_t L = used_s + b;
+                    const double proposed = energy + readlane_f64(dv, L);
+                    const double q = (proposed - gmin) * ginv;
+                    const double fq = floor(q);
+                    int32_t idx = min(max(cvt_i32_sat(fq), lo), hi);
+                    const double fr = q - fq; // exact: fq <= q < fq + 1
+                    if (fr < margin || fr > margin_hi || q >= q_top) {
+                        bool in;
+                        const uint32_t g = grid_bin(a.grid, proposed, in);
+                        idx = in ? min(max(int32_t(g), lo), hi) : lo;
+                    }
+                    const double lgn = lgm[idx];
+                    const uint32_t actn = actm[idx];
+                    const double lr = lgc - lgn; // -inf outside the window
+                    const double luL = readlane_f64(p_lu, L);
+                    const uint32_t mL = __builtin_amdgcn_readlane(mv_hi, L);
+                    const uint32_t sL = __builtin_amdgcn_readlane(p_site, L);
+                    const bool accepted = luL < lr;
+                    const uint32_t c_hi = accepted ? mL : 0u;
+                    hv = fma(hi_double(c_hi), jc[b], hv);
+                    const uint32_t t = p_site == sL ? c_hi : 0u;
+                    mv_hi += (t & 0x40000000u) << 1; // a later proposal of the same site: -m
+                    dv = hi_double(mv_hi) * hv;
+                    energy = accepted ? proposed : energy;
+                    cur = accepted ? idx : cur;
+                    lgc = accepted ? lgn : lgc;
+                    cur_seen = accepted ? (actn != 0 ? 1u : 0u) : cur_seen;
+                    if (!cur_seen) { // first visit: update_current_bin (wang_landau.cpp:294-297)
+                        ++active;
+                        cur_seen = 1;
+                        actm[cur] = 1; // every lane writes the same byte
+                        if (inverse_time && L < freeze) { // later attempts see the new count
+                            inc = lane > L ? batch_factors(WLX_STAGE_INVERSE_TIME, factor, active,
+                                                           clk0 + done_total + b + 1, L + 1)
+                                           : inc;
+                            const uint64_t fz = __ballot(lane > L && inc <= a.final_factor);
+                            freeze = fz ? uint32_t(__ffsll((unsigned long long)fz) - 1) : kWave;
+                            incz = lane > L ? (lane >= freeze ? 0.0 : inc) : incz;
                         }
-                        dv = hi_double(mv_hi) * hv;
-                        energy = accepted ? proposed : energy;
-                        const double lg_old = accepted ? lgn : lgc;
-                        cur = accepted ? idx : cur;
-                        lgc = lg_old + incL;
-                        lgm[cur] = lgc;
-                        rec_cm[b] = (uint64_t(c_hi) << 32) | uint32_t(cur);
-                        rec_lr[b] = lr;
-                        rec_old[b] = lg_old;
-                        // the next batch's row b, one row per decision (after the last batch of
-                        // a call these are valid rows that nothing reads)
-                        const uint32_t sn = __builtin_amdgcn_readlane(next_site, next_first + uint32_t(b));
-                        stage_row<NPL>(J + size_t(sn) * kRowDoubles, next_rows + uint32_t(b) * kRowBytes);
                     }
+                    lgc = lgc + readlane_f64(incz, L);
+                    lgm[cur] = lgc; // every lane writes the same value
+                    rcm[b] = (uint64_t(c_hi) << 32) | uint32_t(cur);
+                    rlr[b] = lr;
                 };
-                using yes = std::integral_constant<bool, true>;
-                using no = std::integral_constant<bool, false>;
-                if (repeats) {
-                    if (per_lane) fast_batch(yes{}, yes{});
-                    else fast_batch(yes{}, no{});
-                } else {
-                    if (per_lane) fast_batch(no{}, yes{});
-                    else fast_batch(no{}, no{});
-                }
-                // first visits: one atomic OR per decision's bin into its visited flag
-                const uint32_t r = (lane - used) & (KB - 1);
-                const uint32_t bin = uint32_t(rec_cm[r]) - begin;
-                uint32_t *word = reinterpret_cast<uint32_t *>(v.act + (bin & ~3u));
-                const uint32_t shift = 8 * (bin & 3u);
-                bool first = false;
-                if (in_batch)
-                    first = ((__hip_atomic_fetch_or(word, 1u << shift, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_WORKGROUP) >> shift) &
-                             0xFFu) == 0;
-                const uint64_t first_lanes = __ballot(first);
-                active += uint32_t(__popcll(first_lanes));
-                cur_seen = 1;
-                if (redo || (inverse_time && first_lanes)) {
-                    // restore the batch's start: visited flags, log_g in reverse order, registers
-                    if (first)
-                        __hip_atomic_fetch_and(word, ~(0xFFu << shift), __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (K == uint32_t(KB)) {
+                    // snapshot for a replay
+                    const double energy0 = energy, lgc0 = lgc, hv0 = hv, dv0 = dv;
+                    const int32_t cur0 = cur;
+                    const uint32_t seen0 = cur_seen, active0 = active, mv0 = mv_hi;
+                    // a site proposed twice in the batch (the later proposal's sign depends on
+                    // the earlier one's outcome); one wave's LDS accesses complete in order
+                    const bool in_batch = lane >= used && lane < used + KB;
+                    if (in_batch) duptab[p_site] = lane;
+                    asm volatile("" ::: "memory"); // the other lanes' stores are read back
+                    const bool repeats = __ballot(in_batch && duptab[p_site] != lane) != 0;
+                    const double inc_u = uniform_f64(inc); // WL or frozen: one increment
+                    bool redo = false;
+                    auto fast_batch = [&](auto repeat_tag, auto per_lane_tag)
+                        __attribute__((always_inline)) {
+                        constexpr bool REPEATS = decltype(repeat_tag)::value;
+                        constexpr bool PER_LANE = decltype(per_lane_tag)::value;
+#pragma unroll
+                        for (int b = 0; b < KB; ++b) { // branch-free decisions
+                            const uint32_t L = used_s + uint32_t(b);
+                            const double proposed = energy + readlane_f64(dv, L);
+                            const double q = (proposed - gmin) * ginv;
+                            const double fq = floor(q);
+                            const int32_t idx = min(max(cvt_i32_sat(fq), lo), hi);
+                            const double lgn = lgm[idx];
+                            const double fr = q - fq;
+                            redo = redo || fr < margin || fr > margin_hi || q >= q_top;
+                            const double luL = readlane_f64(p_lu, L);
+                            const uint32_t mL = __builtin_amdgcn_readlane(mv_hi, L);
+                            const double incL = PER_LANE ? readlane_f64(incz, L) : inc_u;
+                            const double lr = lgc - lgn;
+                            const bool accepted = luL < lr;
+                            const uint32_t c_hi = accepted ? mL : 0u;
+                            hv = fma(hi_double(c_hi), jc[b], hv);
+                            if constexpr (REPEATS) {
+                                const uint32_t sL = __builtin_amdgcn_readlane(p_site, L);
+                                const uint32_t t = p_site == sL ? c_hi : 0u;
+                                mv_hi += (t & 0x40000000u) << 1;
+                            }
+                            dv = hi_double(mv_hi) * hv;
+                            energy = accepted ? proposed : energy;
+                            const double lg_old = accepted ? lgn : lgc;
+                            cur = accepted ? idx : cur;
+                            lgc = lg_old + incL;
+                            lgm[cur] = lgc;
+                            rcm[b] = (uint64_t(c_hi) << 32) | uint32_t(cur);
+                            rlr[b] = lr;
+                            rec_old[b] = lg_old;
+                        }
+                    };
+                    using yes = std::integral_constant<bool, true>;
+                    using no = std::integral_constant<bool, false>;
+                    if (repeats) {
+                        if (inverse_time) fast_batch(yes{}, yes{});
+                        else fast_batch(yes{}, no{});
+                    } else {
+                        if (inverse_time) fast_batch(no{}, yes{});
+                        else fast_batch(no{}, no{});
+                    }
+                    // first visits: one atomic OR per decision's bin into its visited flag
+                    const uint32_t bin = uint32_t(rcm[j]) - begin;
+                    uint32_t *word = reinterpret_cast<uint32_t *>(v.act + (bin & ~3u));
+                    const uint32_t shift = 8 * (bin & 3u);
+                    bool first = false;
+                    if (in_batch)
+                        first = ((__hip_atomic_fetch_or(word, 1u << shift, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP) >> shift) &
+                                 0xFFu) == 0;
+                    const uint64_t first_lanes = __ballot(first);
+                    active += uint32_t(__popcll(first_lanes));
+                    cur_seen = 1;
+                    if (redo || (inverse_time && first_lanes)) {
+                        // restore the batch's start: visited flags, log_g in reverse order,
+                        // registers; then decide the batch exactly
+                        if (first)
+                            __hip_atomic_fetch_and(word, ~(0xFFu << shift), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll 1
-                    for (int b = KB - 1; b >= 0; --b)
-                        lgm[int32_t(uint32_t(rec_cm[b]))] = rec_old[b];
-                    energy = energy0;
-                    lgc = lgc0;
-                    hv = hv0;
-                    dv = dv0;
-                    cur = cur0;
-                    cur_seen = seen0;
-                    active = active0;
-                    mv_hi = mv0;
+                        for (int b = KB - 1; b >= 0; --b)
+                            lgm[int32_t(uint32_t(rcm[b]))] = rec_old[b];
+                        energy = energy0;
+                        lgc = lgc0;
+                        hv = hv0;
+                        dv = dv0;
+                        cur = cur0;
+                        cur_seen = seen0;
+                        active = active0;
+                        mv_hi = mv0;
 #pragma unroll
-                    for (int b = 0; b < KB; ++b) decide_exact(uint32_t(b));
+                        for (int b = 0; b < KB; ++b) decide_exact(uint32_t(b));
+                    }
+                } else { // the last batch of a call
+#pragma unroll
+                    for (int b = 0; b < KB; ++b)
+                        if (uint32_t(b) < K) decide_exact(uint32_t(b));
                 }
-            } else { // the last batch of a call: nothing is staged after it
+                if (inverse_time) { // the factor of the last 1/t attempt, and a freeze
+                    const uint32_t last_lane = used + K - 1;
+                    factor = readlane_f64(inc, min(freeze, last_lane));
+                    if (freeze <= last_lane) stage = WLX_STAGE_FROZEN;
+                }
+                // spin flips of the accepted proposals (XOR commutes)
+                const bool mine = lane >= used && lane < used + K;
+                const uint32_t m_hi = mine ? uint32_t(rcm[j] >> 32) : 0u;
+                if (m_hi != 0)
+                    __hip_atomic_fetch_xor(&smirror[p_site & (kWave - 1)], 1u << (p_site >> 6),
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else if (!decider) {
+                // ================= wave B: apply batch i - 1, stage batch i + 1 =================
+                if (i > 0) {
+                    const uint32_t bi = i - 1;
+                    const uint32_t used = (bi * KB) & (kWave - 1);
+                    const uint32_t blk = ((bi * KB) / kWave) & 1;
+                    const uint64_t left = count - uint64_t(bi) * KB;
+                    const uint32_t K = left < uint64_t(KB) ? uint32_t(left) : uint32_t(KB);
+                    const bool mine = lane >= used && lane < used + K;
+                    const uint32_t r = (lane - used) & (KB - 1);
+                    const uint64_t cm = rec_cm[(bi & 1) * KB + r];
+                    const uint32_t m_hi = mine ? uint32_t(cm >> 32) : 0u;
+                    const uint64_t acc_lanes = __ballot(m_hi != 0u);
+                    n_acc += uint32_t(__popcll(acc_lanes));
+                    if (acc_lanes)
+                        last_acc = bi * KB + (63u - uint32_t(__clzll(acc_lanes))) - used + 1u;
+                    if (mine) // H[cur] += 1 of every attempt of the batch
+                        __hip_atomic_fetch_add(&v.H[uint32_t(cm) - begin], uint64_t(1),
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    { // drawn decisions (finite log ratio < 0) whose log(u) lay within 4 ulp of it
+                        const double lr = rec_lr[(bi & 1) * KB + r];
+                        const double lu = plu[blk * kWave + lane];
+                        const bool amb = mine && lr < 0.0 && lr > -__builtin_huge_val() &&
+                                         near_threshold(lu, lr);
+                        ambiguous += __popcll(__ballot(amb));
+                    }
+                    // h += m_q J[s_q, :] for the batch's proposals in order, m_q = 0 when
+                    // rejected (the product is exact and h + 0 = h: fields are never -0.0)
+                    const double *rowl =
+                        reinterpret_cast<const double *>(smem + lay.rows + (bi & 1) * kBufBytes) + lane;
+                    const uint32_t used_s = uniform_u32(used);
 #pragma unroll
-                for (int b = 0; b < KB; ++b)
-                    if (uint32_t(b) < K) decide_exact(uint32_t(b));
-            }
-            if (inverse_time) { // the factor of the last 1/t attempt, and a freeze
-                const uint32_t last_lane = used + K - 1;
-                factor = readlane_f64(inc, min(freeze, last_lane));
-                if (freeze <= last_lane) stage = WLX_STAGE_FROZEN;
-            }
-            // the batch's records, lane-parallel: lane used + b holds decision b
-            const bool mine = lane >= used && lane < used + K;
-            const uint32_t r = (lane - used) & (KB - 1);
-            const uint64_t cm = rec_cm[r];
-            const uint32_t m_hi = mine ? uint32_t(cm >> 32) : 0u;
-            const uint64_t acc_lanes = __ballot(m_hi != 0u);
-            n_acc += uint32_t(__popcll(acc_lanes));
-            if (acc_lanes) last_acc = done_total + (63u - uint32_t(__clzll(acc_lanes))) - used + 1u;
-            if (mine) // H[cur] += 1 of every attempt of the batch
-                __hip_atomic_fetch_add(&v.H[uint32_t(cm) - begin], uint64_t(1), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-            { // drawn decisions (finite log ratio < 0) whose log(u) lay within 4 ulp of it
-                const double lr = rec_lr[r];
-                const bool amb = mine && lr < 0.0 && lr > -__builtin_huge_val() &&
-                                 near_threshold(p_lu, lr);
-                ambiguous += __popcll(__ballot(amb));
-            }
-            if (m_hi != 0) // spin flips of the accepted proposals (XOR commutes)
-                __hip_atomic_fetch_xor(&smirror[p_site & (kWave - 1)], 1u << (p_site >> 6),
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            // h += m_q J[s_q, :] for the batch's proposals in order, m_q = 0 when rejected (the
-            // product is exact and h + 0 = h: fields are never -0.0)
-            const double *rowl = rows + lane;
+                    for (int g = 0; g < KB; g += 4) {
+                        double rr[4][NPL];
 #pragma unroll
-            for (int g = 0; g < KB; g += 4) {
-                double rr[4][NPL];
+                        for (int jj = 0; jj < 4; ++jj)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                            for (int k = 0; k < NPL; ++k)
+                                rr[jj][k] = rowl[(g + jj) * kRowDoubles + k * kWave];
 #pragma unroll
-                    for (int k = 0; k < NPL; ++k) rr[j][k] = rowl[(g + j) * kRowDoubles + k * kWave];
+                        for (int jj = 0; jj < 4; ++jj) {
+                            const double m = hi_double(
+                                __builtin_amdgcn_readlane(m_hi, used_s + uint32_t(g + jj)));
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const double m = hi_double(__builtin_amdgcn_readlane(m_hi, used_s + uint32_t(g + j)));
+                            for (int k = 0; k < NPL; ++k) h[k] = fma(m, rr[jj][k], h[k]);
+                        }
+                    }
 #pragma unroll
-                    for (int k = 0; k < NPL; ++k) h[k] = fma(m, rr[j][k], h[k]);
+                    for (int k = 0; k < NPL; ++k) fmirror[k * kWave + lane] = h[k];
+                }
+                if (i + 1 < nb) { // the rows of batch i + 1, and its block of proposals
+                    const uint32_t nused = ((i + 1) * KB) & (kWave - 1);
+                    if (nused == 0) {
+                        double lu;
+                        draw(att0 + uint64_t(i + 1) * KB, b_site, lu);
+                        const uint32_t nblk = (((i + 1) * KB) / kWave) & 1;
+                        psite[nblk * kWave + lane] = b_site;
+                        plu[nblk * kWave + lane] = lu;
+                    }
+                    const uint32_t nrows = rows_lds + ((i + 1) & 1) * kBufBytes;
+                    const uint32_t nused_s = uniform_u32(nused);
+#pragma unroll
+                    for (int q = 0; q < KB; ++q)
+                        stage_row<NPL>(J + size_t(uint32_t(__builtin_amdgcn_readlane(b_site, nused_s + uint32_t(q)))) *
+                                               kRowDoubles,
+                                       nrows + uint32_t(q) * kRowBytes);
+                    stage_wait();
                 }
             }
-#pragma unroll
-            for (int k = 0; k < NPL; ++k) fmirror[k * kWave + lane] = h[k];
-            __syncthreads(); // the next gather reads other lanes' fields and spin bits
-            used = (used + K) & (kWave - 1);
-            if (used == 0) {
-                p_site = n_site;
-                p_lu = n_lu;
-            }
-            buf ^= 1;
-            remaining -= K;
-            done_total += K;
+            __syncthreads();
         }
-        stage_wait();
+        // the waves' results to both waves
+        if (decider && lane == 0) {
+            fin[0] = __double_as_longlong(energy);
+            fin[1] = stage;
+            fin[2] = __double_as_longlong(factor);
+            fin[3] = active;
+        }
+        if (!decider && lane == 0) {
+            fin[4] = n_acc;
+            fin[5] = last_acc;
+            fin[6] = ambiguous;
+        }
+        __syncthreads();
+        energy = __longlong_as_double((long long)fin[0]);
         neg = smirror[lane];
-        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int k = 0; k < NPL; ++k) h[k] = fmirror[k * kWave + lane];
+        const uint64_t done_total = count;
         const uint64_t accepted0 = st.accepted, ambiguous0 = st.ambiguous;
         st = a.state[w];
         st.attempted = att0 + done_total;
-        st.accepted = accepted0 + n_acc;
-        if (last_acc) st.last_accepted = att0 + last_acc;
-        st.stage = stage;
-        st.factor = factor;
-        st.active_count = active;
-        st.ambiguous = ambiguous0 + ambiguous;
+        st.accepted = accepted0 + fin[4];
+        if (fin[5]) st.last_accepted = att0 + fin[5];
+        st.stage = uint32_t(fin[1]);
+        st.factor = __longlong_as_double((long long)fin[2]);
+        st.active_count = uint32_t(fin[3]);
+        st.ambiguous = ambiguous0 + fin[6];
     }
     __syncthreads();
-    finish_walker<NPL>(a, w, st, v, ws, neg, h, energy, flags);
+    if (decider) {
+        finish_walker<NPL>(a, w, st, v, ws, neg, h, energy, flags);
+    } else { // wave B meets finish_walker_state's barriers (same state, same branch)
+        if ((flags & kCheck) && !st.escaped && st.stage == WLX_STAGE_WANG_LANDAU) __syncthreads();
+        __syncthreads();
+    }
 }
 
 template <int NPL>
-__global__ __launch_bounds__(kWave) void lookahead_staged_kernel(const Args *table,
-                                                                 const BatchSlot *slots,
-                                                                 uint64_t request, uint32_t flags)
+__global__ __launch_bounds__(kStagedThreads) void lookahead_staged_kernel(const Args *table,
+                                                                          const BatchSlot *slots,
+                                                                          uint64_t request,
+                                                                          uint32_t flags)
 {
     const BatchSlot slot = slots[blockIdx.x];
     const Args a = table[slot.engine];
@@ -3210,7 +3307,8 @@ struct wlx_engine {
         const bool lds_fn = fn == kernels.lds_xo || fn == kernels.lds_ph;
         const size_t lds = lds_fn ? lds_walker : fn == kernels.staged ? lds_staged : lds_sample;
         if (int status = upload_args()) return status;
-        hipLaunchKernelGGL(fn, dim3(walkers), dim3(kWave), lds, stream, d_args, d_slots, request,
+        const uint32_t threads = fn == kernels.staged ? kStagedThreads : kWave;
+        hipLaunchKernelGGL(fn, dim3(walkers), dim3(threads), lds, stream, d_args, d_slots, request,
                            flags);
         REWL_HIP(hipGetLastError());
         return WLX_OK;
