@@ -1654,37 +1654,41 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
         double factor = uniform_f64(st.factor);
         __syncthreads();
         for (uint32_t i = 0; i <= nb; ++i) {
+            // ---- wave A: gather batch i, before B may overwrite batch i - 1's rows ----
+            const uint32_t used = (i * KB) & (kWave - 1);
+            const uint32_t blk = ((i * KB) / kWave) & 1;
+            const uint32_t done_total = i * KB;
+            const uint32_t j = (lane - used) & (KB - 1); // this lane's position in the batch
+            uint32_t p_site = 0, mv_hi = 0;
+            double p_lu = 0.0, hv = 0.0;
+            double jc[KB];
             if (decider && i < nb) {
-                // ================= wave A: decide batch i =================
-                const uint32_t used = (i * KB) & (kWave - 1);
-                const uint32_t blk = ((i * KB) / kWave) & 1;
-                const uint32_t done_total = i * KB;
-                const uint64_t left = count - done_total;
-                const uint32_t K = left < uint64_t(KB) ? uint32_t(left) : uint32_t(KB);
-                const uint32_t p_site = psite[blk * kWave + lane];
-                const double p_lu = plu[blk * kWave + lane];
-                const double *rows = reinterpret_cast<const double *>(smem + lay.rows + (i & 1) * kBufBytes);
-                // gather: B's mirror holds the fields after batch i - 2; batch i - 1's accepted
-                // rows are added at this batch's sites, in order, from this batch's rows
-                const uint32_t j = (lane - used) & (KB - 1); // this lane's position in the batch
-                const double *rowj = rows + j * kRowDoubles;
-                double hv = fmirror[p_site];
+                p_site = psite[blk * kWave + lane];
+                p_lu = plu[blk * kWave + lane];
+                const double *rows =
+                    reinterpret_cast<const double *>(smem + lay.rows + (i & 1) * kBufBytes);
+                // B's mirror holds the fields after batch i - 2; batch i - 1's rows are added at
+                // this batch's sites, in order (m = 0 for a rejection), from batch i - 1's staged
+                // rows: J[s_a][s_j] is row a of that batch at column s_j
+                hv = fmirror[p_site];
                 if (i > 0) {
-                    const uint32_t pblk = (((i - 1) * KB) / kWave) & 1;
-                    const uint32_t pused = ((i - 1) * KB) & (kWave - 1);
+                    const double *prow = reinterpret_cast<const double *>(
+                                             smem + lay.rows + ((i - 1) & 1) * kBufBytes) + p_site;
                     const uint64_t *pcm = rec_cm + ((i - 1) & 1) * KB;
 #pragma unroll
-                    for (int q = 0; q < KB; ++q) {
-                        const uint32_t sa = psite[pblk * kWave + pused + uint32_t(q)];
-                        const uint32_t ma = uint32_t(pcm[q] >> 32);
-                        hv = fma(hi_double(ma), rowj[sa], hv);
-                    }
+                    for (int q = 0; q < KB; ++q)
+                        hv = fma(hi_double(uint32_t(pcm[q] >> 32)), prow[q * kRowDoubles], hv);
                 }
                 const uint32_t bits = smirror[p_site & (kWave - 1)];
-                uint32_t mv_hi = ((bits >> (p_site >> 6)) & 1) ? 0x40000000u : 0xC0000000u; // -2 sigma
-                double jc[KB];
+                mv_hi = ((bits >> (p_site >> 6)) & 1) ? 0x40000000u : 0xC0000000u; // -2 sigma
 #pragma unroll
                 for (int q = 0; q < KB; ++q) jc[q] = rows[q * kRowDoubles + p_site];
+            }
+            __syncthreads(); // B applies and overwrites batch i - 1's rows after this
+            if (decider && i < nb) {
+                // ================= wave A: decide batch i =================
+                const uint64_t left = count - done_total;
+                const uint32_t K = left < uint64_t(KB) ? uint32_t(left) : uint32_t(KB);
                 const bool inverse_time = stage == WLX_STAGE_INVERSE_TIME;
                 double inc = batch_factors(stage, factor, active, clk0 + done_total, used);
                 uint32_t freeze = kWave; // lane of the freezing attempt; kWave: none
