@@ -1472,6 +1472,29 @@ __global__ __launch_bounds__(kWave) void lookahead_fast_kernel(const Args *table
 //   - a batch without a repeated proposal site runs a loop without the sign bookkeeping of
 //     repeats, and outside the 1/t stage the log_g increment is one value for the batch.
 // ---------------------------------------------------------------------------------------------
+// f(std::integral_constant<int, k>) for k = 0 .. N - 1, in order (a compile-time index for
+// the DPP broadcasts below)
+template <int N, int K = 0, class F> __device__ __attribute__((always_inline)) inline void static_for(F &&f)
+{
+    if constexpr (K < N) {
+        f(std::integral_constant<int, K>{});
+        static_for<N, K + 1>(f);
+    }
+}
+
+// lane B of the value's 16-lane row, in every lane of that row (DPP row_newbcast)
+template <int B> __device__ inline uint32_t row_bcast_u32(uint32_t v)
+{
+    return uint32_t(__builtin_amdgcn_mov_dpp(int(v), 0x150 + B, 0xF, 0xF, true));
+}
+template <int B> __device__ inline double row_bcast_f64(double v)
+{
+    const uint64_t bits = __double_as_longlong(v);
+    const uint32_t lo = row_bcast_u32<B>(uint32_t(bits));
+    const uint32_t hi = row_bcast_u32<B>(uint32_t(bits >> 32));
+    return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
+}
+
 constexpr int kStagedBatch = 16;
 constexpr int kStagedThreads = 2 * kWave;
 
@@ -1655,16 +1678,19 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
         __syncthreads();
         for (uint32_t i = 0; i <= nb; ++i) {
             // ---- wave A: gather batch i, before B may overwrite batch i - 1's rows ----
+            // A holds the batch replicated in the four 16-lane rows of the wave: lane l carries
+            // proposal l % 16, so lane b of every row is decision b, and a DPP row broadcast
+            // (row_newbcast) gives every lane its value without a scalar round trip
             const uint32_t used = (i * KB) & (kWave - 1);
             const uint32_t blk = ((i * KB) / kWave) & 1;
             const uint32_t done_total = i * KB;
-            const uint32_t j = (lane - used) & (KB - 1); // this lane's position in the batch
+            const uint32_t jl = lane & (KB - 1); // this lane's proposal in the batch
             uint32_t p_site = 0, mv_hi = 0;
             double p_lu = 0.0, hv = 0.0;
             double jc[KB];
             if (decider && i < nb) {
-                p_site = psite[blk * kWave + lane];
-                p_lu = plu[blk * kWave + lane];
+                p_site = psite[blk * kWave + used + jl];
+                p_lu = plu[blk * kWave + used + jl];
                 const double *rows =
                     reinterpret_cast<const double *>(smem + lay.rows + (i & 1) * kBufBytes);
                 // B's mirror holds the fields after batch i - 2; batch i - 1's rows are added at
@@ -1690,22 +1716,30 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                 const uint64_t left = count - done_total;
                 const uint32_t K = left < uint64_t(KB) ? uint32_t(left) : uint32_t(KB);
                 const bool inverse_time = stage == WLX_STAGE_INVERSE_TIME;
-                double inc = batch_factors(stage, factor, active, clk0 + done_total, used);
-                uint32_t freeze = kWave; // lane of the freezing attempt; kWave: none
+                // log_g increment of attempt done_total + jl (update_inverse_time_factor,
+                // wang_landau.cpp:309-315): the WL factor, the 1/t factor active / attempted
+                // (unchanged while no bin is active), or 0 once frozen; `from` is the first
+                // proposal whose factor is (re)computed
+                auto factor_at = [&](uint32_t stage_now, uint32_t active_now, uint32_t from) {
+                    if (stage_now == WLX_STAGE_FROZEN) return 0.0;
+                    if (stage_now == WLX_STAGE_WANG_LANDAU || active_now == 0) return factor;
+                    const uint64_t t = clk0 + done_total + 1 + uint64_t(jl >= from ? jl : from);
+                    return double(active_now) / double(t);
+                };
+                double inc = factor_at(stage, active, 0);
+                uint32_t freeze = KB; // proposal of the freezing attempt; KB: none
                 if (inverse_time) {
-                    const uint64_t fz = __ballot(lane >= used && inc <= a.final_factor);
-                    freeze = fz ? uint32_t(__ffsll((unsigned long long)fz) - 1) : kWave;
+                    const uint64_t fz = __ballot(lane < uint32_t(KB) && inc <= a.final_factor);
+                    freeze = fz ? uint32_t(__ffsll((unsigned long long)fz) - 1) : uint32_t(KB);
                 }
-                double incz = lane >= freeze ? 0.0 : inc;
+                double incz = jl >= freeze ? 0.0 : inc;
                 double dv = hi_double(mv_hi) * hv; // flip_delta of each proposal (physics.cpp:261-266)
-                const uint32_t used_s = uniform_u32(used);
                 uint64_t *rcm = rec_cm + (i & 1) * KB;
                 double *rlr = rec_lr + (i & 1) * KB;
                 // the exact decision: the correctly rounded quotient where the product is not
                 // known exact, first visits in line, refreshing the 1/t factors of later attempts
                 auto decide_exact = [&](uint32_t b) __attribute__((always_inline)) {
-                    const uint32_t L = used_s + b;
-                    const double proposed = energy + readlane_f64(dv, L);
+                    const double proposed = energy + readlane_f64(dv, b);
                     const double q = (proposed - gmin) * ginv;
                     const double fq = floor(q);
                     int32_t idx = min(max(cvt_i32_sat(fq), lo), hi);
@@ -1718,9 +1752,9 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                     const double lgn = lgm[idx];
                     const uint32_t actn = actm[idx];
                     const double lr = lgc - lgn; // -inf outside the window
-                    const double luL = readlane_f64(p_lu, L);
-                    const uint32_t mL = __builtin_amdgcn_readlane(mv_hi, L);
-                    const uint32_t sL = __builtin_amdgcn_readlane(p_site, L);
+                    const double luL = readlane_f64(p_lu, b);
+                    const uint32_t mL = __builtin_amdgcn_readlane(mv_hi, b);
+                    const uint32_t sL = __builtin_amdgcn_readlane(p_site, b);
                     const bool accepted = luL < lr;
                     const uint32_t c_hi = accepted ? mL : 0u;
                     hv = fma(hi_double(c_hi), jc[b], hv);
@@ -1735,16 +1769,15 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                         ++active;
                         cur_seen = 1;
                         actm[cur] = 1; // every lane writes the same byte
-                        if (inverse_time && L < freeze) { // later attempts see the new count
-                            inc = lane > L ? batch_factors(WLX_STAGE_INVERSE_TIME, factor, active,
-                                                           clk0 + done_total + b + 1, L + 1)
-                                           : inc;
-                            const uint64_t fz = __ballot(lane > L && inc <= a.final_factor);
-                            freeze = fz ? uint32_t(__ffsll((unsigned long long)fz) - 1) : kWave;
-                            incz = lane > L ? (lane >= freeze ? 0.0 : inc) : incz;
+                        if (inverse_time && b < freeze) { // later attempts see the new count
+                            inc = jl > b ? factor_at(WLX_STAGE_INVERSE_TIME, active, b + 1) : inc;
+                            const uint64_t fz =
+                                __ballot(lane < uint32_t(KB) && lane > b && inc <= a.final_factor);
+                            freeze = fz ? uint32_t(__ffsll((unsigned long long)fz) - 1) : uint32_t(KB);
+                            incz = jl > b ? (jl >= freeze ? 0.0 : inc) : incz;
                         }
                     }
-                    lgc = lgc + readlane_f64(incz, L);
+                    lgc = lgc + readlane_f64(incz, b);
                     lgm[cur] = lgc; // every lane writes the same value
                     rcm[b] = (uint64_t(c_hi) << 32) | uint32_t(cur);
                     rlr[b] = lr;
@@ -1756,35 +1789,34 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                     const uint32_t seen0 = cur_seen, active0 = active, mv0 = mv_hi;
                     // a site proposed twice in the batch (the later proposal's sign depends on
                     // the earlier one's outcome); one wave's LDS accesses complete in order
-                    const bool in_batch = lane >= used && lane < used + KB;
-                    if (in_batch) duptab[p_site] = lane;
+                    const bool row0 = lane < uint32_t(KB);
+                    if (row0) duptab[p_site] = lane;
                     asm volatile("" ::: "memory"); // the other lanes' stores are read back
-                    const bool repeats = __ballot(in_batch && duptab[p_site] != lane) != 0;
+                    const bool repeats = __ballot(row0 && duptab[p_site] != lane) != 0;
                     const double inc_u = uniform_f64(inc); // WL or frozen: one increment
                     bool redo = false;
                     auto fast_batch = [&](auto repeat_tag, auto per_lane_tag)
                         __attribute__((always_inline)) {
                         constexpr bool REPEATS = decltype(repeat_tag)::value;
                         constexpr bool PER_LANE = decltype(per_lane_tag)::value;
-#pragma unroll
-                        for (int b = 0; b < KB; ++b) { // branch-free decisions
-                            const uint32_t L = used_s + uint32_t(b);
-                            const double proposed = energy + readlane_f64(dv, L);
+                        static_for<KB>([&](auto bc) __attribute__((always_inline)) {
+                            constexpr int b = decltype(bc)::value; // branch-free decision b
+                            const double proposed = energy + row_bcast_f64<b>(dv);
                             const double q = (proposed - gmin) * ginv;
                             const double fq = floor(q);
                             const int32_t idx = min(max(cvt_i32_sat(fq), lo), hi);
                             const double lgn = lgm[idx];
                             const double fr = q - fq;
                             redo = redo || fr < margin || fr > margin_hi || q >= q_top;
-                            const double luL = readlane_f64(p_lu, L);
-                            const uint32_t mL = __builtin_amdgcn_readlane(mv_hi, L);
-                            const double incL = PER_LANE ? readlane_f64(incz, L) : inc_u;
+                            const double luL = row_bcast_f64<b>(p_lu);
+                            const uint32_t mL = row_bcast_u32<b>(mv_hi);
+                            const double incL = PER_LANE ? row_bcast_f64<b>(incz) : inc_u;
                             const double lr = lgc - lgn;
                             const bool accepted = luL < lr;
                             const uint32_t c_hi = accepted ? mL : 0u;
                             hv = fma(hi_double(c_hi), jc[b], hv);
                             if constexpr (REPEATS) {
-                                const uint32_t sL = __builtin_amdgcn_readlane(p_site, L);
+                                const uint32_t sL = row_bcast_u32<b>(p_site);
                                 const uint32_t t = p_site == sL ? c_hi : 0u;
                                 mv_hi += (t & 0x40000000u) << 1;
                             }
@@ -1797,7 +1829,7 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                             rcm[b] = (uint64_t(c_hi) << 32) | uint32_t(cur);
                             rlr[b] = lr;
                             rec_old[b] = lg_old;
-                        }
+                        });
                     };
                     using yes = std::integral_constant<bool, true>;
                     using no = std::integral_constant<bool, false>;
@@ -1809,11 +1841,11 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                         else fast_batch(no{}, no{});
                     }
                     // first visits: one atomic OR per decision's bin into its visited flag
-                    const uint32_t bin = uint32_t(rcm[j]) - begin;
+                    const uint32_t bin = uint32_t(rcm[jl]) - begin;
                     uint32_t *word = reinterpret_cast<uint32_t *>(v.act + (bin & ~3u));
                     const uint32_t shift = 8 * (bin & 3u);
                     bool first = false;
-                    if (in_batch)
+                    if (row0)
                         first = ((__hip_atomic_fetch_or(word, 1u << shift, __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_WORKGROUP) >> shift) &
                                  0xFFu) == 0;
@@ -1846,13 +1878,12 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                         if (uint32_t(b) < K) decide_exact(uint32_t(b));
                 }
                 if (inverse_time) { // the factor of the last 1/t attempt, and a freeze
-                    const uint32_t last_lane = used + K - 1;
-                    factor = readlane_f64(inc, min(freeze, last_lane));
-                    if (freeze <= last_lane) stage = WLX_STAGE_FROZEN;
+                    factor = readlane_f64(inc, min(freeze, K - 1));
+                    if (freeze <= K - 1) stage = WLX_STAGE_FROZEN;
                 }
                 // spin flips of the accepted proposals (XOR commutes)
-                const bool mine = lane >= used && lane < used + K;
-                const uint32_t m_hi = mine ? uint32_t(rcm[j] >> 32) : 0u;
+                const bool mine = lane < K;
+                const uint32_t m_hi = mine ? uint32_t(rcm[jl] >> 32) : 0u;
                 if (m_hi != 0)
                     __hip_atomic_fetch_xor(&smirror[p_site & (kWave - 1)], 1u << (p_site >> 6),
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
