@@ -66,19 +66,22 @@ TIMING_EVERY = 8
 GRAPH_EPOCHS = 16
 C2_GRID = (-273.0, 354.5, 0.5)
 # HBM bytes per launch of the sampling kernels on this workload, from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE passes scaled by a copy-kernel calibration; the summary (with its commands) is
-# committed here and is read, not re-measured, by this script
-TRAFFIC_SUMMARY = ROOT / "profiles" / "lookahead_fast_kernel_c2_8x8x8.md"
+# WRITE_SIZE passes scaled by a copy-kernel calibration; the summaries (with their commands) are
+# committed under profiles/ and are read, not re-measured, by this script
+TRAFFIC_SUMMARIES = ("lookahead_staged_kernel_c2_8x8x8.md", "lookahead_fast_kernel_c2_8x8x8.md")
 
 
 def traffic_bytes(kernel: str):
-    """HBM bytes per launch recorded for `kernel` in the committed counter summary, or None."""
-    if not TRAFFIC_SUMMARY.exists():
-        return None
-    for line in TRAFFIC_SUMMARY.read_text().splitlines():
-        words = line.split()
-        if len(words) == 3 and words[0] == "traffic_bytes_per_launch" and words[2] == kernel:
-            return float(words[1])
+    """HBM bytes per launch recorded for `kernel` in the committed counter summaries (the first
+    that names it), or None."""
+    for name in TRAFFIC_SUMMARIES:
+        path = ROOT / "profiles" / name
+        if not path.exists():
+            continue
+        for line in path.read_text().splitlines():
+            words = line.split()
+            if len(words) == 3 and words[0] == "traffic_bytes_per_launch" and words[2] == kernel:
+                return float(words[1])
     return None
 
 

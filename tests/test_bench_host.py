@@ -22,8 +22,9 @@ def test_more_than_one_gpu_is_refused_with_a_message():
     assert "one GPU only" in r.stderr and r.stdout == ""
 
 
-def test_traffic_figures_come_from_the_committed_counter_summary():
-    for kernel in ("lookahead_fast_kernel<8>", "lookahead_xo_kernel<8>"):
+def test_traffic_figures_come_from_the_committed_counter_summaries():
+    for kernel in ("lookahead_staged_kernel<8>", "lookahead_fast_kernel<8>",
+                   "lookahead_xo_kernel<8>"):
         value = bench.traffic_bytes(kernel)
         assert value is not None and 1e6 < value < 1e9
     assert bench.traffic_bytes("no_such_kernel<1>") is None
