@@ -1460,8 +1460,9 @@ __global__ __launch_bounds__(kWave) void lookahead_fast_kernel(const Args *table
 //     and is rejected by the same comparison log(u) < log ratio that accepts the others (log u
 //     is finite and negative, so that comparison also accepts every log ratio >= 0);
 //   - the bin comes from the reciprocal product, which is exact unless the product lies near
-//     an integer, or at or above the grid maximum's product in the window that ends at the
-//     grid's last bin (rounding is monotonic, so any energy >= maximum lands there);
+//     an integer, or, in the window that ends at the grid's last bin, lies between the grid
+//     maximum's product and max(that product, bin count) (rounding is monotonic, so the
+//     maximum itself lands there, and a product above the band floors past the last bin);
 //   - a batch is first decided without branches, every decision selecting its outcome, and
 //     with the visited flags and the active count left for after the batch: one atomic OR per
 //     decision's bin then finds the batch's first visits. A decision whose product is not known
@@ -1570,6 +1571,14 @@ __device__ inline void stage_row(const double *row_any, uint32_t dst_any)
     }
 }
 
+// the median of x, lo and hi: x clamped to [lo, hi] for lo <= hi, in one instruction
+__device__ inline int32_t med3_i32(int32_t x, int32_t lo, int32_t hi)
+{
+    int32_t r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(lo), "v"(hi));
+    return r;
+}
+
 __device__ inline void stage_wait()
 {
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -1663,6 +1672,17 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
         const double q_top = st.end == uint32_t(a.grid.count)
                                  ? (a.grid.maximum - a.grid.minimum) * a.grid.inverse_width
                                  : __builtin_huge_val();
+        // products in [q_top, max(q_top, count)] are the grid maximum itself (the last bin) or
+        // energies above it whose product's floor is still inside the grid: the quotient decides
+        // them. Above that band the floor is at least count and the proposal lands on the +inf
+        // entry, rejected as an energy outside the grid. The fast path tests the distance to the
+        // band's centre against its half width plus `margin` (at least 4 ulp of count, more
+        // than the rounding of the centre and of the distance); outside the top window the centre
+        // is +inf and no distance is within it.
+        const double band_hi = fmax(q_top, double(a.grid.count));
+        const double band_c = st.end == uint32_t(a.grid.count) ? 0.5 * (q_top + band_hi)
+                                                                : __builtin_huge_val();
+        const double band_r = 0.5 * (band_hi - q_top) + margin;
         const int32_t lo = int32_t(begin) - 1, hi = int32_t(st.end);
         double *lgm = v.lg - begin; // by absolute bin
         uint8_t *actm = v.act - begin;
@@ -1794,7 +1814,11 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                     asm volatile("" ::: "memory"); // the other lanes' stores are read back
                     const bool repeats = __ballot(row0 && duptab[p_site] != lane) != 0;
                     const double inc_u = uniform_f64(inc); // WL or frozen: one increment
-                    bool redo = false;
+                    // the batch's smallest and largest fractional parts and the smallest
+                    // distance to the top window's band:
+                    // a lane-wise minimum and maximum per decision (a comparison per decision
+                    // would hand its result to the scalar unit, which waits for it)
+                    double fr_min = 1.0, fr_max = 0.0, band_d = __builtin_huge_val();
                     auto fast_batch = [&](auto repeat_tag, auto per_lane_tag)
                         __attribute__((always_inline)) {
                         constexpr bool REPEATS = decltype(repeat_tag)::value;
@@ -1804,10 +1828,12 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                             const double proposed = energy + row_bcast_f64<b>(dv);
                             const double q = (proposed - gmin) * ginv;
                             const double fq = floor(q);
-                            const int32_t idx = min(max(cvt_i32_sat(fq), lo), hi);
+                            const int32_t idx = med3_i32(cvt_i32_sat(fq), lo, hi);
                             const double lgn = lgm[idx];
                             const double fr = q - fq;
-                            redo = redo || fr < margin || fr > margin_hi || q >= q_top;
+                            fr_min = fmin(fr_min, fr);
+                            fr_max = fmax(fr_max, fr);
+                            band_d = fmin(band_d, fabs(q - band_c));
                             const double luL = row_bcast_f64<b>(p_lu);
                             const uint32_t mL = row_bcast_u32<b>(mv_hi);
                             const double incL = PER_LANE ? row_bcast_f64<b>(incz) : inc_u;
@@ -1852,6 +1878,7 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                     const uint64_t first_lanes = __ballot(first);
                     active += uint32_t(__popcll(first_lanes));
                     cur_seen = 1;
+                    const bool redo = fr_min < margin || fr_max > margin_hi || band_d <= band_r;
                     if (redo || (inverse_time && first_lanes)) {
                         // restore the batch's start: visited flags, log_g in reverse order,
                         // registers; then decide the batch exactly
