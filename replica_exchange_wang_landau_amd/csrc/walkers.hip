@@ -1823,13 +1823,24 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                         __attribute__((always_inline)) {
                         constexpr bool REPEATS = decltype(repeat_tag)::value;
                         constexpr bool PER_LANE = decltype(per_lane_tag)::value;
+                        // the records of decision b - 1 are written while decision b's LDS
+                        // read is in flight, with the work that does not need its result (the
+                        // compiler would otherwise leave the read's latency uncovered)
+                        uint64_t cm_prev = 0;
+                        double lr_prev = 0.0, old_prev = 0.0;
                         static_for<KB>([&](auto bc) __attribute__((always_inline)) {
                             constexpr int b = decltype(bc)::value; // branch-free decision b
                             const double proposed = energy + row_bcast_f64<b>(dv);
                             const double q = (proposed - gmin) * ginv;
                             const double fq = floor(q);
                             const int32_t idx = med3_i32(cvt_i32_sat(fq), lo, hi);
-                            const double lgn = lgm[idx];
+                            double lgn = lgm[idx];
+                            asm volatile("" ::: "memory"); // the read is issued first
+                            if constexpr (b > 0) {
+                                rcm[b - 1] = cm_prev;
+                                rlr[b - 1] = lr_prev;
+                                rec_old[b - 1] = old_prev;
+                            }
                             const double fr = q - fq;
                             fr_min = fmin(fr_min, fr);
                             fr_max = fmax(fr_max, fr);
@@ -1837,12 +1848,15 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                             const double luL = row_bcast_f64<b>(p_lu);
                             const uint32_t mL = row_bcast_u32<b>(mv_hi);
                             const double incL = PER_LANE ? row_bcast_f64<b>(incz) : inc_u;
+                            uint32_t sL = 0;
+                            if constexpr (REPEATS) sL = row_bcast_u32<b>(p_site);
+                            asm volatile("" : "+v"(lgn) : "v"(fr_min), "v"(fr_max), "v"(band_d), "v"(luL),
+                                         "v"(mL), "v"(incL), "v"(sL) : "memory");
                             const double lr = lgc - lgn;
                             const bool accepted = luL < lr;
                             const uint32_t c_hi = accepted ? mL : 0u;
                             hv = fma(hi_double(c_hi), jc[b], hv);
                             if constexpr (REPEATS) {
-                                const uint32_t sL = row_bcast_u32<b>(p_site);
                                 const uint32_t t = p_site == sL ? c_hi : 0u;
                                 mv_hi += (t & 0x40000000u) << 1;
                             }
@@ -1852,10 +1866,13 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                             cur = accepted ? idx : cur;
                             lgc = lg_old + incL;
                             lgm[cur] = lgc;
-                            rcm[b] = (uint64_t(c_hi) << 32) | uint32_t(cur);
-                            rlr[b] = lr;
-                            rec_old[b] = lg_old;
+                            cm_prev = (uint64_t(c_hi) << 32) | uint32_t(cur);
+                            lr_prev = lr;
+                            old_prev = lg_old;
                         });
+                        rcm[KB - 1] = cm_prev;
+                        rlr[KB - 1] = lr_prev;
+                        rec_old[KB - 1] = old_prev;
                     };
                     using yes = std::integral_constant<bool, true>;
                     using no = std::integral_constant<bool, false>;
