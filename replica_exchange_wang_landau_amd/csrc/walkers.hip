@@ -1615,8 +1615,60 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
     double *rec_old = reinterpret_cast<double *>(smem + lay.rec_old);
     uint64_t *fin = reinterpret_cast<uint64_t *>(smem + lay.fin);
     const uint32_t rows_lds = uint32_t(reinterpret_cast<uintptr_t>(smem)) + lay.rows;
-    // both waves load the same bins and configuration (the stores agree)
-    load_bins(a, w, ws, v);
+    const double *J = a.J;
+    const uint64_t count = batch_count(a, st, request, flags);
+    // proposals of attempts first + lane (one Philox4x64 block per two attempts)
+    auto draw = [&](uint64_t first, uint32_t &site, double &lu) {
+        const uint64_t t = first + lane;
+        const Philox4 blk = philox4x64(t >> 1, kPhiloxWalkTag, 0, 0, a.seed, uint64_t(w));
+        const bool odd = t & 1;
+        site = uint32_t(__umul64hi(odd ? blk.v[2] : blk.v[0], uint64_t(a.n)));
+        const double u = double((odd ? blk.v[3] : blk.v[1]) >> 11) * 0x1.0p-53;
+        lu = fmin(log(fmax(u, 0x1.0p-53)), -0x1.0p-1074); // < 0: see the header
+    };
+    // wave B requests block 0 of the proposals and the first batch's rows before the walker's
+    // bins and configuration are loaded, so that their latencies overlap
+    uint32_t b_site = 0; // B: the proposal sites of the current block
+    if (!decider && count > 0) {
+        double lu;
+        draw(st.attempted, b_site, lu);
+        psite[lane] = b_site;
+        plu[lane] = lu;
+#pragma unroll
+        for (int q = 0; q < KB; ++q)
+            stage_row<NPL>(J + size_t(uint32_t(__builtin_amdgcn_readlane(b_site, uint32_t(q)))) * kRowDoubles,
+                           rows_lds + uint32_t(q) * kRowBytes);
+    }
+    // the window's bins by both waves, four bins per thread with every load of a round issued
+    // before its LDS stores; the configuration by both (the stores agree)
+    {
+        const size_t base = size_t(w) * a.ws_stride;
+        for (uint32_t b0 = threadIdx.x; b0 < ws; b0 += 4 * kStagedThreads) {
+            double lg4[4] = {0.0, 0.0, 0.0, 0.0};
+            uint64_t h4[4] = {0, 0, 0, 0};
+            uint8_t act4[4] = {0, 0, 0, 0}, ref4[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t b = b0 + uint32_t(j) * kStagedThreads;
+                if (b < ws) {
+                    lg4[j] = a.logg[base + b];
+                    h4[j] = a.hist[base + b];
+                    act4[j] = a.active[base + b];
+                    ref4[j] = a.refine[base + b];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t b = b0 + uint32_t(j) * kStagedThreads;
+                if (b < ws) {
+                    v.lg[b] = lg4[j];
+                    v.H[b] = h4[j];
+                    v.act[b] = act4[j];
+                    v.ref[b] = ref4[j];
+                }
+            }
+        }
+    }
     uint64_t neg;
     double h[NPL];
     load_config<NPL>(a, st.slot, neg, h);
@@ -1631,40 +1683,17 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
         for (int k = 0; k < NPL; ++k) fmirror[k * kWave + lane] = h[k];
     }
     double energy = uniform_f64(a.energy[st.slot]);
-    const uint64_t count = batch_count(a, st, request, flags);
     const int64_t start_bin = grid_index(a.grid, energy);
+    stage_wait(); // B: the first batch's rows are in LDS
     __syncthreads();
     if (count > 0 && (start_bin < int64_t(st.begin) || start_bin >= int64_t(st.end))) {
         st.escaped = 1; // wang_landau.cpp:247-248, decided up front as in the general kernel
     } else if (count > 0) {
         const uint32_t begin = st.begin;
         const uint64_t att0 = st.attempted, clk0 = att0 + st.clock_offset;
-        const double *J = a.J;
         const uint32_t nb = uint32_t((count + KB - 1) / KB); // batches, the last one partial
-        // proposals of attempts att0 + first + lane (one Philox4x64 block per two attempts)
-        auto draw = [&](uint64_t first, uint32_t &site, double &lu) {
-            const uint64_t t = first + lane;
-            const Philox4 blk = philox4x64(t >> 1, kPhiloxWalkTag, 0, 0, a.seed, uint64_t(w));
-            const bool odd = t & 1;
-            site = uint32_t(__umul64hi(odd ? blk.v[2] : blk.v[0], uint64_t(a.n)));
-            const double u = double((odd ? blk.v[3] : blk.v[1]) >> 11) * 0x1.0p-53;
-            lu = fmin(log(fmax(u, 0x1.0p-53)), -0x1.0p-1074); // < 0: see the header
-        };
-        // ---- wave B: block 0 of the proposals and the first batch's rows ----
-        uint32_t b_site = 0; // B: the proposal sites of the current block
-        uint32_t n_acc = 0, last_acc = 0;
-        uint64_t ambiguous = 0;
-        if (!decider) {
-            double lu;
-            draw(att0, b_site, lu);
-            psite[lane] = b_site;
-            plu[lane] = lu;
-#pragma unroll
-            for (int q = 0; q < KB; ++q)
-                stage_row<NPL>(J + size_t(uint32_t(__builtin_amdgcn_readlane(b_site, uint32_t(q)))) * kRowDoubles,
-                               rows_lds + uint32_t(q) * kRowBytes);
-            stage_wait();
-        }
+        uint32_t n_acc = 0, last_acc = 0; // B
+        uint64_t ambiguous = 0;           // B
         // ---- wave A: the walker ----
         const double gmin = a.grid.minimum, ginv = a.grid.inverse_width;
         const double margin = a.grid.margin, margin_hi = a.grid.margin_hi;
@@ -1695,7 +1724,6 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
         }
         uint32_t stage = uniform_u32(st.stage), active = uniform_u32(st.active_count);
         double factor = uniform_f64(st.factor);
-        __syncthreads();
         for (uint32_t i = 0; i <= nb; ++i) {
             // ---- wave A: gather batch i, before B may overwrite batch i - 1's rows ----
             // A holds the batch replicated in the four 16-lane rows of the wave: lane l carries
