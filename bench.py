@@ -282,14 +282,17 @@ def c4_system():
 def c4_leg(epochs: int, warmup: int, resyncs: int):
     """C4 precision study: dense J (FP32 copy streamed by the walkers, FP64 kept for the
     resync), FP32 Kahan fields, FP64 H = J.S resync every 10 MCS, 32 windows x 8 walkers,
-    bin 2.0 over the seed-1234 10^4-MCS pilot range padded by 5 %, Philox stream."""
+    bin 2.0 over the seed-1234 10^3-MCS pilot range padded by 5 %, Philox stream. (A pilot of
+    10^4 MCS streams 41 M rows of the 128 MiB J on one wave and takes minutes.)"""
     import math
 
     from replica_exchange_wang_landau_amd import Engine, RunConfig, WlParameters, abi
 
     system = c4_system()
     n = system.n
-    lo, hi = system.pilot(1234, 10_000 * n)
+    print(f"c4: N={n}, pilot of 1000 MCS", file=sys.stderr, flush=True)
+    lo, hi = system.pilot(1234, 1_000 * n)
+    print(f"c4: pilot range [{lo:.1f}, {hi:.1f}]", file=sys.stderr, flush=True)
     pad = 0.05 * (hi - lo)
     emin, emax = 2.0 * math.floor((lo - pad) / 2.0), 2.0 * math.ceil((hi + pad) / 2.0)
     cfg = RunConfig(emin=emin, emax=emax, bin_width=2.0, windows=32, walkers_per_window=8,
@@ -298,11 +301,13 @@ def c4_leg(epochs: int, warmup: int, resyncs: int):
                     wl=WlParameters(final_factor=1e-7, check_interval_attempts=100 * n))
     engine = Engine(system, cfg)
     engine.initialize()
+    print(f"c4: {engine.walkers} walkers initialised", file=sys.stderr, flush=True)
     engine.run(warmup)
     before = engine.status()
     t0 = time.perf_counter()
     after = engine.run(epochs)
     seconds = time.perf_counter() - t0
+    print(f"c4: {epochs} epochs in {seconds:.2f} s", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
     for _ in range(resyncs):
         engine.resync()
@@ -311,8 +316,8 @@ def c4_leg(epochs: int, warmup: int, resyncs: int):
     flops = 2.0 * n * n * engine.walkers
     return {
         "workload": f"C4: simple-cubic 16x16x16 periodic, random +-x/y/z axes (seed 1234), "
-                    f"N={n}, grid [{emin},{emax}] bin 2.0, 32 windows x 8 walkers, FP32 Kahan "
-                    f"fields, FP64 resync every 10 MCS",
+                    f"N={n}, grid [{emin},{emax}] bin 2.0 (10^3-MCS pilot), 32 windows x 8 "
+                    f"walkers, FP32 Kahan fields, FP64 resync every 10 MCS",
         "attempted_flips_per_s": (after.attempted - before.attempted) / seconds,
         "epochs_timed": epochs,
         "resync": {"ms": resync_s * 1e3, "flop": flops, "achieved_tflops": flops / resync_s / 1e12,
@@ -357,7 +362,7 @@ def c3_leg(epochs: int, warmup: int, graph: int, pilot_mcs: int = 1000,
     batch = Batch(engines)
     batch.set_graph(graph)
     # the LDS budget (which windows keep their bins in HBM), picked on the live run
-    budget, budget_seconds = batch.autotune(graph)
+    budget, budget_seconds = batch.autotune(graph if graph > 0 else 16)  # plain launches at 0
     layout = batch.layout()
     print(f"c3: LDS budget {budget} B ({budget_seconds}), layout {layout}", file=sys.stderr,
           flush=True)
