@@ -1500,7 +1500,7 @@ constexpr int kStagedBatch = 16;
 constexpr int kStagedThreads = 2 * kWave;
 
 struct StagedLayout {
-    uint32_t lg, H, act, ref, rows, fields, spins, dups, psite, plu, rec_cm, rec_lr, rec_old, fin;
+    uint32_t lg, H, act, ref, rows, fields, spins, dups, psite, plu, rec_cm, rec_lr, rec_old, fin, rflag;
     uint32_t total;
 };
 
@@ -1508,7 +1508,8 @@ struct StagedLayout {
 // one past it, H, the visited flags (4-byte aligned, with spare bytes on either side), the
 // refine flags, two row buffers of kStagedBatch rows each, the field and spin-bit mirrors, a
 // site table for finding repeated proposals, two blocks of 64 proposals (sites, log u), two
-// batches of decision records, the log_g each decision replaced, and the waves' final values
+// batches of decision records, the log_g each decision replaced, the waves' final values, and
+// two batches' repeated-site flags
 __host__ __device__ inline StagedLayout staged_layout(uint32_t stride, uint32_t np)
 {
     StagedLayout s;
@@ -1526,7 +1527,8 @@ __host__ __device__ inline StagedLayout staged_layout(uint32_t stride, uint32_t 
     s.rec_lr = s.rec_cm + 2 * kStagedBatch * 8;
     s.rec_old = s.rec_lr + 2 * kStagedBatch * 8;
     s.fin = s.rec_old + kStagedBatch * 8;
-    s.total = s.fin + 8 * 8;
+    s.rflag = s.fin + 8 * 8;
+    s.total = s.rflag + 2 * 4;
     return s;
 }
 
@@ -1614,6 +1616,7 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
     double *rec_lr = reinterpret_cast<double *>(smem + lay.rec_lr);
     double *rec_old = reinterpret_cast<double *>(smem + lay.rec_old);
     uint64_t *fin = reinterpret_cast<uint64_t *>(smem + lay.fin);
+    uint32_t *rflag = reinterpret_cast<uint32_t *>(smem + lay.rflag);
     const uint32_t rows_lds = uint32_t(reinterpret_cast<uintptr_t>(smem)) + lay.rows;
     const double *J = a.J;
     const uint64_t count = batch_count(a, st, request, flags);
@@ -1625,6 +1628,16 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
         site = uint32_t(__umul64hi(odd ? blk.v[2] : blk.v[0], uint64_t(a.n)));
         const double u = double((odd ? blk.v[3] : blk.v[1]) >> 11) * 0x1.0p-53;
         lu = fmin(log(fmax(u, 0x1.0p-53)), -0x1.0p-1074); // < 0: see the header
+    };
+    // wave B: whether the batch at lanes [used, used + KB) of `sites` proposes a site twice (a
+    // later proposal's sign then depends on the earlier one's outcome), into rflag[slot]; one
+    // wave's LDS accesses complete in order, so its read sees its own writes
+    auto flag_repeats = [&](uint32_t sites, uint32_t used, uint32_t slot) {
+        const bool in = lane >= used && lane < used + uint32_t(KB);
+        if (in) duptab[sites] = lane;
+        asm volatile("" ::: "memory"); // the other lanes' stores are read back
+        const bool dup = __ballot(in && duptab[sites] != lane) != 0;
+        if (lane == 0) rflag[slot] = dup ? 1u : 0u;
     };
     // wave B requests block 0 of the proposals and the first batch's rows before the walker's
     // bins and configuration are loaded, so that their latencies overlap
@@ -1638,6 +1651,7 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
         for (int q = 0; q < KB; ++q)
             stage_row<NPL>(J + size_t(uint32_t(__builtin_amdgcn_readlane(b_site, uint32_t(q)))) * kRowDoubles,
                            rows_lds + uint32_t(q) * kRowBytes);
+        flag_repeats(b_site, 0, 0);
     }
     // the window's bins by both waves, four bins per thread with every load of a round issued
     // before its LDS stores; the configuration by both (the stores agree)
@@ -1724,6 +1738,7 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
         }
         uint32_t stage = uniform_u32(st.stage), active = uniform_u32(st.active_count);
         double factor = uniform_f64(st.factor);
+        uint32_t m_prev = 0; // A: the previous batch's accepted signs (lane l: proposal l % 16)
         for (uint32_t i = 0; i <= nb; ++i) {
             // ---- wave A: gather batch i, before B may overwrite batch i - 1's rows ----
             // A holds the batch replicated in the four 16-lane rows of the wave: lane l carries
@@ -1733,7 +1748,7 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
             const uint32_t blk = ((i * KB) / kWave) & 1;
             const uint32_t done_total = i * KB;
             const uint32_t jl = lane & (KB - 1); // this lane's proposal in the batch
-            uint32_t p_site = 0, mv_hi = 0;
+            uint32_t p_site = 0, mv_hi = 0, rep_flag = 0;
             double p_lu = 0.0, hv = 0.0;
             double jc[KB];
             if (decider && i < nb) {
@@ -1748,13 +1763,15 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                 if (i > 0) {
                     const double *prow = reinterpret_cast<const double *>(
                                              smem + lay.rows + ((i - 1) & 1) * kBufBytes) + p_site;
-                    const uint64_t *pcm = rec_cm + ((i - 1) & 1) * KB;
-#pragma unroll
-                    for (int q = 0; q < KB; ++q)
-                        hv = fma(hi_double(uint32_t(pcm[q] >> 32)), prow[q * kRowDoubles], hv);
+                    // the accepted signs of batch i - 1, kept by this wave in every 16-lane row
+                    static_for<KB>([&](auto qc) __attribute__((always_inline)) {
+                        constexpr int q = decltype(qc)::value;
+                        hv = fma(hi_double(row_bcast_u32<q>(m_prev)), prow[q * kRowDoubles], hv);
+                    });
                 }
                 const uint32_t bits = smirror[p_site & (kWave - 1)];
                 mv_hi = ((bits >> (p_site >> 6)) & 1) ? 0x40000000u : 0xC0000000u; // -2 sigma
+                rep_flag = rflag[i & 1];
 #pragma unroll
                 for (int q = 0; q < KB; ++q) jc[q] = rows[q * kRowDoubles + p_site];
             }
@@ -1835,12 +1852,10 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                     const double energy0 = energy, lgc0 = lgc, hv0 = hv, dv0 = dv;
                     const int32_t cur0 = cur;
                     const uint32_t seen0 = cur_seen, active0 = active, mv0 = mv_hi;
-                    // a site proposed twice in the batch (the later proposal's sign depends on
-                    // the earlier one's outcome); one wave's LDS accesses complete in order
+                    // a site proposed twice in the batch (found by wave B when it staged the
+                    // batch's rows)
                     const bool row0 = lane < uint32_t(KB);
-                    if (row0) duptab[p_site] = lane;
-                    asm volatile("" ::: "memory"); // the other lanes' stores are read back
-                    const bool repeats = __ballot(row0 && duptab[p_site] != lane) != 0;
+                    const bool repeats = uniform_u32(rep_flag) != 0;
                     const double inc_u = uniform_f64(inc); // WL or frozen: one increment
                     // the batch's smallest and largest fractional parts and the smallest
                     // distance to the top window's band:
@@ -1953,9 +1968,11 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                     factor = readlane_f64(inc, min(freeze, K - 1));
                     if (freeze <= K - 1) stage = WLX_STAGE_FROZEN;
                 }
-                // spin flips of the accepted proposals (XOR commutes)
-                const bool mine = lane < K;
-                const uint32_t m_hi = mine ? uint32_t(rcm[jl] >> 32) : 0u;
+                // spin flips of the accepted proposals (XOR commutes); every row keeps the
+                // batch's signs for the next gather, and row 0 flips
+                const uint32_t m_row = jl < K ? uint32_t(rcm[jl] >> 32) : 0u;
+                m_prev = m_row;
+                const uint32_t m_hi = lane < K ? m_row : 0u;
                 if (m_hi != 0)
                     __hip_atomic_fetch_xor(&smirror[p_site & (kWave - 1)], 1u << (p_site >> 6),
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -2025,6 +2042,7 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                         stage_row<NPL>(J + size_t(uint32_t(__builtin_amdgcn_readlane(b_site, nused_s + uint32_t(q)))) *
                                                kRowDoubles,
                                        nrows + uint32_t(q) * kRowBytes);
+                    flag_repeats(b_site, nused_s, (i + 1) & 1);
                     stage_wait();
                 }
             }
