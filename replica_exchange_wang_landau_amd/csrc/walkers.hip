@@ -1984,45 +1984,44 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                     const uint32_t blk = ((bi * KB) / kWave) & 1;
                     const uint64_t left = count - uint64_t(bi) * KB;
                     const uint32_t K = left < uint64_t(KB) ? uint32_t(left) : uint32_t(KB);
-                    const bool mine = lane >= used && lane < used + K;
-                    const uint32_t r = (lane - used) & (KB - 1);
+                    // lane l holds record l % 16 in every 16-lane row (the signs are then
+                    // broadcast by DPP); row 0's first K lanes count the batch once
+                    const uint32_t r = lane & (KB - 1);
+                    const bool mine = lane < K;
                     const uint64_t cm = rec_cm[(bi & 1) * KB + r];
-                    const uint32_t m_hi = mine ? uint32_t(cm >> 32) : 0u;
+                    const double lr = rec_lr[(bi & 1) * KB + r];
+                    const double lu = plu[blk * kWave + used + r];
+                    const uint32_t m_row = r < K ? uint32_t(cm >> 32) : 0u;
+                    const uint32_t m_hi = mine ? m_row : 0u;
                     const uint64_t acc_lanes = __ballot(m_hi != 0u);
                     n_acc += uint32_t(__popcll(acc_lanes));
-                    if (acc_lanes)
-                        last_acc = bi * KB + (63u - uint32_t(__clzll(acc_lanes))) - used + 1u;
+                    if (acc_lanes) last_acc = bi * KB + (63u - uint32_t(__clzll(acc_lanes))) + 1u;
                     if (mine) // H[cur] += 1 of every attempt of the batch
                         __hip_atomic_fetch_add(&v.H[uint32_t(cm) - begin], uint64_t(1),
                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    { // drawn decisions (finite log ratio < 0) whose log(u) lay within 4 ulp of it
-                        const double lr = rec_lr[(bi & 1) * KB + r];
-                        const double lu = plu[blk * kWave + lane];
-                        const bool amb = mine && lr < 0.0 && lr > -__builtin_huge_val() &&
-                                         near_threshold(lu, lr);
-                        ambiguous += __popcll(__ballot(amb));
-                    }
+                    // drawn decisions (finite log ratio < 0) whose log(u) lay within 4 ulp of it
+                    const bool amb = mine && lr < 0.0 && lr > -__builtin_huge_val() &&
+                                     near_threshold(lu, lr);
+                    ambiguous += __popcll(__ballot(amb));
                     // h += m_q J[s_q, :] for the batch's proposals in order, m_q = 0 when
                     // rejected (the product is exact and h + 0 = h: fields are never -0.0)
                     const double *rowl =
                         reinterpret_cast<const double *>(smem + lay.rows + (bi & 1) * kBufBytes) + lane;
-                    const uint32_t used_s = uniform_u32(used);
-#pragma unroll
-                    for (int g = 0; g < KB; g += 4) {
+                    static_for<KB / 4>([&](auto gc) __attribute__((always_inline)) {
+                        constexpr int g = 4 * decltype(gc)::value;
                         double rr[4][NPL];
 #pragma unroll
                         for (int jj = 0; jj < 4; ++jj)
 #pragma unroll
                             for (int k = 0; k < NPL; ++k)
                                 rr[jj][k] = rowl[(g + jj) * kRowDoubles + k * kWave];
-#pragma unroll
-                        for (int jj = 0; jj < 4; ++jj) {
-                            const double m = hi_double(
-                                __builtin_amdgcn_readlane(m_hi, used_s + uint32_t(g + jj)));
+                        static_for<4>([&](auto jc_) __attribute__((always_inline)) {
+                            constexpr int jj = decltype(jc_)::value;
+                            const double m = hi_double(row_bcast_u32<g + jj>(m_row));
 #pragma unroll
                             for (int k = 0; k < NPL; ++k) h[k] = fma(m, rr[jj][k], h[k]);
-                        }
-                    }
+                        });
+                    });
 #pragma unroll
                     for (int k = 0; k < NPL; ++k) fmirror[k * kWave + lane] = h[k];
                 }
