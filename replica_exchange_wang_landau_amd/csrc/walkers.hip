@@ -1760,20 +1760,28 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                 // this batch's sites, in order (m = 0 for a rejection), from batch i - 1's staged
                 // rows: J[s_a][s_j] is row a of that batch at column s_j
                 hv = fmirror[p_site];
+                double pr[KB]; // batch i - 1's rows at this site
+                const double *prow = reinterpret_cast<const double *>(
+                                         smem + lay.rows + ((i - 1) & 1) * kBufBytes) + p_site;
                 if (i > 0) {
-                    const double *prow = reinterpret_cast<const double *>(
-                                             smem + lay.rows + ((i - 1) & 1) * kBufBytes) + p_site;
-                    // the accepted signs of batch i - 1, kept by this wave in every 16-lane row
-                    static_for<KB>([&](auto qc) __attribute__((always_inline)) {
-                        constexpr int q = decltype(qc)::value;
-                        hv = fma(hi_double(row_bcast_u32<q>(m_prev)), prow[q * kRowDoubles], hv);
-                    });
+#pragma unroll
+                    for (int q = 0; q < KB; ++q) pr[q] = prow[q * kRowDoubles];
                 }
                 const uint32_t bits = smirror[p_site & (kWave - 1)];
-                mv_hi = ((bits >> (p_site >> 6)) & 1) ? 0x40000000u : 0xC0000000u; // -2 sigma
                 rep_flag = rflag[i & 1];
 #pragma unroll
                 for (int q = 0; q < KB; ++q) jc[q] = rows[q * kRowDoubles + p_site];
+                // every read above is issued before the additions below wait for the first
+                // (the scheduler would otherwise keep two reads in flight)
+                __builtin_amdgcn_sched_barrier(0);
+                if (i > 0) {
+                    // the accepted signs of batch i - 1, kept by this wave in every 16-lane row
+                    static_for<KB>([&](auto qc) __attribute__((always_inline)) {
+                        constexpr int q = decltype(qc)::value;
+                        hv = fma(hi_double(row_bcast_u32<q>(m_prev)), pr[q], hv);
+                    });
+                }
+                mv_hi = ((bits >> (p_site >> 6)) & 1) ? 0x40000000u : 0xC0000000u; // -2 sigma
             }
             __syncthreads(); // B applies and overwrites batch i - 1's rows after this
             if (decider && i < nb) {
