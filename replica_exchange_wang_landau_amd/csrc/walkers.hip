@@ -1862,7 +1862,6 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                     const uint32_t seen0 = cur_seen, active0 = active, mv0 = mv_hi;
                     // a site proposed twice in the batch (found by wave B when it staged the
                     // batch's rows)
-                    const bool row0 = lane < uint32_t(KB);
                     const bool repeats = uniform_u32(rep_flag) != 0;
                     const double inc_u = uniform_f64(inc); // WL or frozen: one increment
                     // the batch's smallest and largest fractional parts and the smallest
@@ -1934,15 +1933,15 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
                         if (inverse_time) fast_batch(no{}, yes{});
                         else fast_batch(no{}, no{});
                     }
-                    // first visits: one atomic OR per decision's bin into its visited flag
+                    // first visits: an atomic OR per decision's bin into its visited flag, from
+                    // all four copies of the batch (exactly one lane finds each newly visited
+                    // bin unset, so the count and the restore below are those of one copy)
                     const uint32_t bin = uint32_t(rcm[jl]) - begin;
                     uint32_t *word = reinterpret_cast<uint32_t *>(v.act + (bin & ~3u));
                     const uint32_t shift = 8 * (bin & 3u);
-                    bool first = false;
-                    if (row0)
-                        first = ((__hip_atomic_fetch_or(word, 1u << shift, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_WORKGROUP) >> shift) &
-                                 0xFFu) == 0;
+                    const bool first = ((__hip_atomic_fetch_or(word, 1u << shift, __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_WORKGROUP) >>
+                                         shift) & 0xFFu) == 0;
                     const uint64_t first_lanes = __ballot(first);
                     active += uint32_t(__popcll(first_lanes));
                     cur_seen = 1;
