@@ -1,6 +1,8 @@
 // Warm-start network: a 4-layer, 128-wide MLP (6 features -> 128 -> 128 -> 128 -> 1, SiLU
 // hidden activations) trained and evaluated by hand-written kernels. Matrix products take BF16
-// operands on the matrix cores (v_mfma_f32_16x16x16_bf16) and accumulate in FP32; biases,
+// operands on the matrix cores and accumulate in FP32: v_mfma_f32_16x16x32_bf16 (gfx950's
+// 32-deep BF16 step) where the reduction runs over 128 features, v_mfma_f32_16x16x16_bf16
+// where it runs over the 16 padded inputs or the 16 samples of a tile; biases,
 // pre-activations, the output layer, the loss and Adam run in FP32 on FP32 master weights.
 //
 // One wave processes a tile of 16 samples through all layers with its activations in LDS.
@@ -34,6 +36,8 @@ constexpr size_t kOffB3 = kOffW3 + kWidth;
 constexpr size_t kParams = kOffB3 + 1;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8s __attribute__((ext_vector_type(8))); // BF16 bit patterns
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // FP32 -> BF16, round to nearest even (inputs are finite)
@@ -63,6 +67,15 @@ __device__ inline f32x4 mma(bf16x4 a, bf16x4 b, f32x4 c)
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// v_mfma_f32_16x16x32_bf16 on gfx950: lane l holds A[l % 16][8 (l / 16) + t] and
+// B[8 (l / 16) + t][l % 16] for t < 8, and D[4 (l / 16) + r][l % 16] in register r (measured
+// the same way)
+__device__ inline f32x4 mma32(bf16x8s a, bf16x8s b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
 struct Weights { // BF16 copies used by the matrix products
     const uint16_t *w0, *w1, *w2, *w3;
     const float *p; // FP32 master (biases, b3)
@@ -86,15 +99,28 @@ __device__ void hidden_layer(const uint16_t *act_in, const uint16_t *w, const fl
 #pragma unroll 1
     for (int jt = 0; jt < kWidth / 16; ++jt) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        if constexpr (K % 32 == 0) {
 #pragma unroll
-        for (int k0 = 0; k0 < K; k0 += 16) {
-            bf16x4 a, bb;
+            for (int k0 = 0; k0 < K; k0 += 32) {
+                bf16x8s a, bb;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                a[t] = short(act_in[i * K + k0 + 4 * g + t]);
-                bb[t] = short(w[(jt * 16 + i) * K + k0 + 4 * g + t]);
+                for (int t = 0; t < 8; ++t) {
+                    a[t] = short(act_in[i * K + k0 + 8 * g + t]);
+                    bb[t] = short(w[(jt * 16 + i) * K + k0 + 8 * g + t]);
+                }
+                acc = mma32(a, bb, acc);
             }
-            acc = mma(a, bb, acc);
+        } else {
+#pragma unroll
+            for (int k0 = 0; k0 < K; k0 += 16) {
+                bf16x4 a, bb;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    a[t] = short(act_in[i * K + k0 + 4 * g + t]);
+                    bb[t] = short(w[(jt * 16 + i) * K + k0 + 4 * g + t]);
+                }
+                acc = mma(a, bb, acc);
+            }
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -177,14 +203,14 @@ __device__ void back_through(const uint16_t *w, const float *z_prev, TileLds &s)
     for (int kt = 0; kt < kWidth / 16; ++kt) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int j0 = 0; j0 < kWidth; j0 += 16) {
-            bf16x4 a, b;
+        for (int j0 = 0; j0 < kWidth; j0 += 32) {
+            bf16x8s a, b;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                a[t] = short(s.dz[i * kWidth + j0 + 4 * g + t]);          // A[p][j]
-                b[t] = short(w[(j0 + 4 * g + t) * kWidth + kt * 16 + i]); // B[j][k]
+            for (int t = 0; t < 8; ++t) {
+                a[t] = short(s.dz[i * kWidth + j0 + 8 * g + t]);          // A[p][j]
+                b[t] = short(w[(j0 + 8 * g + t) * kWidth + kt * 16 + i]); // B[j][k]
             }
-            acc = mma(a, b, acc);
+            acc = mma32(a, b, acc);
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
