@@ -1440,21 +1440,27 @@ __global__ __launch_bounds__(kWave) void lookahead_fast_kernel(const Args *table
 // ---------------------------------------------------------------------------------------------
 // lookahead sampling with the candidate rows staged in LDS, two waves per walker (common
 // configuration, Philox stream, 65 to 512 sites). The decisions are those of fast_walk, in the
-// same order. One wave issues one vector instruction per 4 cycles, in order, so a walker's
-// speed is set by the instructions its decision loop holds; here the loop holds little besides
-// the arithmetic of each decision and one LDS read, and everything else runs on a second wave
-// of the same workgroup, on another SIMD of the CU:
+// same order. A wave issues about one vector instruction every 4.6 cycles, in order, whether or
+// not it depends on the previous one, so a walker's speed is set by the instructions its
+// decision loop holds; here the loop holds little besides the arithmetic of each decision and
+// one LDS read, and everything else runs on a second wave of the same workgroup, on another
+// SIMD of the CU:
 //   - wave A (the decider) owns the walker: energy, current bin, log_g, visited flags and spin
-//     bits. Per batch it gathers the field at each proposal site, decides the batch and writes
-//     one record per decision (bin, accepted m, log ratio);
+//     bits. Per batch of 16 attempts it gathers the field at each proposal site, decides the
+//     batch and writes one record per decision (bin, accepted m, log ratio, replaced log_g);
 //   - wave B (the worker) owns the register fields. It applies the previous batch's records
 //     (H, ambiguity count, accepted count), FMA's its accepted rows into the fields and mirrors
-//     them in LDS, draws the Philox proposals one block ahead and copies the next batch's rows
-//     J[s_q, :] from L2 into LDS with global_load_lds_dwordx4 (1 KiB per instruction);
-//   - one workgroup barrier per batch. B's field mirror lags A by one batch, so A's gather adds
-//     the previous batch's accepted rows itself, at the new proposal sites only, read from the
-//     new batch's staged rows (J is symmetric) in acceptance order: the same additions, in the
-//     same order, as the full row update;
+//     them in LDS, draws the Philox proposals one block ahead, copies the next batch's rows
+//     J[s_q, :] from L2 into LDS with global_load_lds_dwordx4 (1 KiB per instruction) and
+//     flags a batch that proposes a site twice;
+//   - two workgroup barriers per batch. B's field mirror lags A by one batch, so A's gather adds
+//     the previous batch's accepted rows itself, at the new proposal sites only: J[s_a][s_j] is
+//     row a of the previous batch's staged rows at column s_j, added in acceptance order, the
+//     same additions in the same order as the full row update. The barrier after A's gather
+//     holds B's overwrite of those rows until A has read them;
+//   - A holds the batch replicated in the four 16-lane rows of its wave (lane l: proposal
+//     l % 16), so a DPP row broadcast gives every lane decision b's values without a scalar
+//     round trip;
 //   - the window's log_g has a +inf entry on either side (relative bins -1 and ws), and the
 //     proposed bin is clamped onto them, so a proposal outside the window gives log ratio -inf
 //     and is rejected by the same comparison log(u) < log ratio that accepts the others (log u
@@ -1711,7 +1717,8 @@ __device__ __attribute__((always_inline)) inline void staged_walk(const Args &a,
         // ---- wave A: the walker ----
         const double gmin = a.grid.minimum, ginv = a.grid.inverse_width;
         const double margin = a.grid.margin, margin_hi = a.grid.margin_hi;
-        // the product of the grid maximum: at or above it the quotient decides (top window only)
+        // the product of the grid maximum (top window only, +inf elsewhere): at or above it the
+        // exact path takes the quotient
         const double q_top = st.end == uint32_t(a.grid.count)
                                  ? (a.grid.maximum - a.grid.minimum) * a.grid.inverse_width
                                  : __builtin_huge_val();
