@@ -3027,6 +3027,13 @@ constexpr int kExchangeBlock = 256;
 
 __device__ inline void exchange_epoch(const Args &a, uint32_t gated, uint64_t epoch_arg)
 {
+    // termination flags (parallel.cpp:391-393, 527-530): a swap moves configurations, not
+    // stages or attempt counts, so they are read first, alongside the gate and epoch
+    uint64_t not_frozen = 0, below_limit = 0;
+    for (uint32_t k = threadIdx.x; k < a.walkers; k += blockDim.x) {
+        not_frozen += a.state[k].stage != WLX_STAGE_FROZEN;
+        below_limit += a.state[k].attempted < a.max_attempts;
+    }
     if (gated && a.glob->done) return;
     const uint64_t epoch = gated ? a.glob->epoch : epoch_arg;
     const uint32_t parity = uint32_t(epoch & 1);
@@ -3074,12 +3081,6 @@ __device__ inline void exchange_epoch(const Args &a, uint32_t gated, uint64_t ep
                 exchange_statistics(a, R, r, r.slot, el);
             }
         }
-    }
-    // termination flags (parallel.cpp:391-393, 527-530)
-    uint64_t not_frozen = 0, below_limit = 0;
-    for (uint32_t k = threadIdx.x; k < a.walkers; k += blockDim.x) {
-        not_frozen += a.state[k].stage != WLX_STAGE_FROZEN;
-        below_limit += a.state[k].attempted < a.max_attempts;
     }
     uint64_t vals[4] = {taken, ambiguous, not_frozen, below_limit};
 #pragma unroll
