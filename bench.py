@@ -440,12 +440,16 @@ def tta_replicas_leg(budget_seconds: float, interval_epochs: int, replicas: int,
             "budget_seconds": budget_seconds}
 
 
-def train_warm_start(curves: int, steps: int, final: float = 1e-4, tag: str = "c5"):
+def train_warm_start(curves: int, steps: int, final: float = 1e-4, tag: str = "c5",
+                     gaussian: bool = False):
     """The warm-start network: `curves` slabs (Lx = Ly in 6..12, Lz in 2..6, dilution in
     [0, 0.4], +z or random in-plane axes; seed 1234) run to ln f = `final` together in one batch,
     and the 4 x 128 BF16 network trained on (E/N, N, density, dilution, Lz, axis mode) ->
     log g / N by the project's kernels (Adam 1e-3, then 3e-4 and 1e-4 for the last quarter
-    each, batch 1024). The last tenth of the curves validates it. Returns (network, per-spin
+    each, batch 1024). With `gaussian` the target is log g / N minus each realization's
+    Gaussian part -(E/N)^2 / (2 s), s = sum_{i<j} J_ij^2 / N (warmstart.energy_spread), so the
+    network learns the non-Gaussian shape. The last tenth of the curves validates it. Returns
+    (network, per-spin
     validation RMS after removing each curve's mean offset, training record)."""
     import numpy as np
 
@@ -459,8 +463,8 @@ def train_warm_start(curves: int, steps: int, final: float = 1e-4, tag: str = "c
                                   flush=True))
     keep = [(s, c) for s, c in zip(slabs, data) if (c[2] != 0).sum() > 4]
     split = len(keep) * 9 // 10
-    x, y = ws.training_set([s for s, _ in keep[:split]], [c for _, c in keep[:split]])
-    vx, vy = ws.training_set([s for s, _ in keep[split:]], [c for _, c in keep[split:]])
+    x, y = ws.training_set([s for s, _ in keep[:split]], [c for _, c in keep[:split]], gaussian)
+    vx, vy = ws.training_set([s for s, _ in keep[split:]], [c for _, c in keep[split:]], gaussian)
     curve_s = time.perf_counter() - t0
     mlp = ws.Mlp(1234)
     t1 = time.perf_counter()
@@ -486,6 +490,7 @@ def train_warm_start(curves: int, steps: int, final: float = 1e-4, tag: str = "c
     record = {"curves": split, "validation_curves": len(keep) - split, "final_ln_f": final,
               "validation_rms_per_spin": val_rms, "samples": int(len(x)), "steps": steps,
               "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+              "target": "log g / N minus the Gaussian part" if gaussian else "log g / N",
               "curve_seconds": curve_s, "train_seconds": train_s}
     return mlp, val_rms, record, validation
 
@@ -494,7 +499,7 @@ C5_REFERENCE = ROOT / "tests" / "golden" / "c5_reference_dos.json"
 
 
 def c5_leg(curves: int, final: float, steps: int, budget_s: float, seeds: int,
-           factors=(1e-2, 1e-4), interval: int = 2000):
+           factors=(1e-2, 1e-4), interval: int = 2000, gaussian: bool = False):
     """Learned warm start on the held-out slab (train_warm_start for the network).
 
     Reference: tests/golden/c5_reference_dos.json (make_c5_reference_dos.py), the average of
@@ -516,7 +521,7 @@ def c5_leg(curves: int, final: float, steps: int, budget_s: float, seeds: int,
     rec = json.loads(C5_REFERENCE.read_text())
     ref = (np.array([float.fromhex(v) for v in rec["log_g"]]),
            np.array(rec["valid"], dtype=np.uint8))
-    mlp, val_rms, training, _ = train_warm_start(curves, steps, final)
+    mlp, val_rms, training, _ = train_warm_start(curves, steps, final, gaussian=gaussian)
     slab = ws.Slab(*rec["slab"])
     system = slab.system()
     if system.n != rec["spins"]:
@@ -533,7 +538,8 @@ def c5_leg(curves: int, final: float, steps: int, budget_s: float, seeds: int,
         return e
 
     probe = engine(0)
-    guess = ws.predicted_log_g(mlp, probe, slab.features(system.n))
+    guess = ws.predicted_log_g(mlp, probe, slab.features(system.n),
+                               ws.energy_spread(system) if gaussian else None)
     probe.close()
     start = guess - guess[ref[1] != 0].mean()
     d = (start - ref[0])[ref[1] != 0]
@@ -604,6 +610,8 @@ def main():
     ap.add_argument("--c5-final", type=float, default=1e-4,
                     help="ln f the training curves are converged to (--c5)")
     ap.add_argument("--c5-steps", type=int, default=60_000, help="training steps (--c5)")
+    ap.add_argument("--c5-gaussian", action="store_true",
+                    help="train the network on the non-Gaussian part of log g / N (--c5)")
     ap.add_argument("--c5-seconds", type=float, default=60.0,
                     help="device-time budget per held-out run (--c5)")
     ap.add_argument("--c3", action="store_true",
@@ -616,7 +624,7 @@ def main():
     args = ap.parse_args()
     if args.c5:
         print(json.dumps(c5_leg(args.c5_curves, args.c5_final, args.c5_steps, args.c5_seconds,
-                                args.tta_seeds)))
+                                args.tta_seeds, gaussian=args.c5_gaussian)))
         return
     if args.c3:
         print(json.dumps(c3_leg(args.steps if args.steps != 3000 else 200, 20, args.graph,

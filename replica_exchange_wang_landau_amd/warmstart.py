@@ -134,6 +134,24 @@ def network_input(e_per_spin, n, density, dilution, thickness, axis_mode) -> np.
     return np.stack(cols, -1).astype(np.float32)
 
 
+def energy_spread(system: System) -> float:
+    """s = sigma^2 / N with sigma^2 = sum over pairs i < j of J_ij^2: the variance of E over
+    uniformly random spins (the infinite-temperature ensemble, where the mean energy is 0 because
+    J_ii = 0). Near the band centre log g(E) / N ~ ln 2 - (E/N)^2 / (2 s)."""
+    if system.backend_name == "csr":
+        _, _, values = system.csr()
+        total = float(np.sum(values.astype(np.float64) ** 2))
+    else:
+        total = float(np.sum(system.dense().astype(np.float64) ** 2))
+    return total / 2.0 / system.n  # each pair appears twice in the symmetric matrix
+
+
+def gaussian_log_g(e_per_spin, spread: float) -> np.ndarray:
+    """The Gaussian part of log g / N with its peak at 0: -(E/N)^2 / (2 s)."""
+    e = np.asarray(e_per_spin, dtype=np.float64)
+    return -(e * e) / (2.0 * spread)
+
+
 def slab_family(count: int, seed: int = 1234) -> list[Slab]:
     """Lx = Ly in {6..12}, Lz in {2..6}, dilution uniform in [0, 0.4], axis mode 0 or 1."""
     rng = np.random.default_rng(seed)
@@ -187,23 +205,33 @@ def converged_curves(slabs, final_factor: float = 1e-3, pilot_mcs: int = 2000,
     return out
 
 
-def training_set(slabs, curves):
-    """Network rows and targets (log g - max log g) / N over every valid bin of every curve."""
+def training_set(slabs, curves, gaussian: bool = False):
+    """Network rows and targets (log g - max log g) / N over every valid bin of every curve;
+    with `gaussian`, the target is what remains after each realization's own Gaussian part
+    (gaussian_log_g with its energy_spread), so the network learns the non-Gaussian shape."""
     xs, ys = [], []
     for slab, (centers, log_g, valid, system) in zip(slabs, curves):
         ok = valid != 0
         n, density, dilution, thickness, axis_mode = slab.features(system.n)
-        xs.append(network_input(centers[ok] / n, n, density, dilution, thickness, axis_mode))
-        ys.append((log_g[ok] / n).astype(np.float32))
+        e = centers[ok] / n
+        xs.append(network_input(e, n, density, dilution, thickness, axis_mode))
+        y = log_g[ok] / n
+        if gaussian:
+            y = y - gaussian_log_g(e, energy_spread(system))
+        ys.append(y.astype(np.float32))
     return np.concatenate(xs), np.concatenate(ys)
 
 
-def predicted_log_g(mlp: Mlp, engine: Engine, features) -> np.ndarray:
-    """N * network(E/N) at every bin centre of the engine's grid."""
+def predicted_log_g(mlp: Mlp, engine: Engine, features, spread: float | None = None) -> np.ndarray:
+    """N * network(E/N) at every bin centre of the engine's grid, plus N times the Gaussian
+    part with energy spread `spread` for a network trained on the residual."""
     n, density, dilution, thickness, axis_mode = features
     centers = engine.config.emin + (np.arange(engine.bins) + 0.5) * engine.config.bin_width
     x = network_input(centers / n, n, density, dilution, thickness, axis_mode)
-    return n * mlp.predict(x).astype(np.float64)
+    out = mlp.predict(x).astype(np.float64)
+    if spread is not None:
+        out = out + gaussian_log_g(centers / n, spread)
+    return n * out
 
 
 def warm_start(engine: Engine, log_g, factor: float) -> None:
