@@ -40,6 +40,10 @@ fields, the FP64 matrix-core resync every 10 MCS, and the drift it measures.
 the BF16 network trained on them by the project's kernels, and time to accuracy from flat and
 warm starts on the held-out 12x12x4 slab at dilution 0.2 (tests/golden/c5_reference_dos.json).
 
+`--c2-warm` runs only the warm-start requirement study on C2: time to accuracy from the
+converged reference log g plus smooth errors of known size, at several initial ln f, against
+the flat start on the same seeds.
+
 `--c3` runs only the batched study: 64 uneven dilute CSR systems (N from 64 to 4096) whose
 walkers all sample in one launch per epoch (the LDS walker kernel), replayed from HIP graphs.
 """
@@ -568,6 +572,67 @@ def c5_leg(curves: int, final: float, steps: int, budget_s: float, seeds: int,
     }
 
 
+def c2_warm_leg(errors, factors, seeds: int, budget_s: float, interval: int):
+    """How close a warm start must be to pay on C2 (8x8x8 periodic dense, 8 windows x 16
+    walkers, Philox stream). Each start is the converged C2 reference log g plus a smooth error
+    of known amplitude, eps * sin(6 pi x) with x running from 0 to 1 over the reference's valid
+    bins (three periods, so every window sees a slope); each is run at each initial ln f in
+    `factors`. Time to accuracy as in tta_seeds, against the same reference, next to the flat
+    start on the same seeds. start_error is the start's own max|dlog g| to the reference."""
+    import numpy as np
+
+    from replica_exchange_wang_landau_amd import Engine, System, analysis
+
+    ref = reference_dos()
+    if ref is None:
+        return None
+    ref_lg, ref_valid = ref[0], ref[1]
+    idx = np.flatnonzero(ref_valid != 0)
+    x = (idx - idx[0]) / max(1, idx[-1] - idx[0])
+    shape = np.zeros_like(ref_lg)
+    shape[idx] = np.sin(6.0 * np.pi * x)
+    seed_list = list(range(1234, 1234 + seeds))
+
+    def engine(seed, start=None, factor=None):
+        e = Engine(System.simple_cubic(8, 8, 8, 1.0, (0.0, 0.0, 1.0), True),
+                   c2_config(rng=1, seed=seed))
+        e.initialize()
+        if start is not None:
+            e.warm_start(start, factor)
+        return e
+
+    flat = tta_seeds(lambda s: engine(s), seed_list, 40.0, interval, "c2 flat")
+    print("c2 warm: flat leg done", file=sys.stderr, flush=True)
+    rows = []
+    for eps in errors:
+        start = np.where(ref_valid != 0, ref_lg + eps * shape, 0.0)
+        row = {"eps": eps,
+               "start_error": analysis.max_deviation(start, ref_valid, ref_lg, ref_valid),
+               "factors": {}}
+        for f0 in factors:
+            r = tta_seeds(lambda s, st=start, f=f0: engine(s, st, f), seed_list, budget_s,
+                          interval, f"c2 warm eps {eps:g} ln f {f0:g}")
+            med = r["seconds_median"]
+            row["factors"][f"{f0:g}"] = {
+                "seconds": r["seconds"], "seconds_median": med,
+                "speedup_median": (flat["seconds_median"] / med
+                                   if med and flat["seconds_median"] else None)}
+        best = [v["speedup_median"] for v in row["factors"].values() if v["speedup_median"]]
+        row["best_speedup"] = max(best) if best else None
+        rows.append(row)
+        print(f"c2 warm: eps {eps:g} done, best speedup {row['best_speedup']}",
+              file=sys.stderr, flush=True)
+    paying = [r["start_error"] for r in rows if (r["best_speedup"] or 0.0) >= 5.0]
+    return {
+        "workload": "C2 warm-start requirement: 8x8x8 periodic dense, 8 windows x 16 walkers, "
+                    "Philox stream, started from the converged reference log g plus "
+                    "eps * sin(6 pi x) over its valid bins",
+        "flat": flat, "rows": rows, "budget_seconds_per_warm_run": budget_s,
+        "interval_epochs": interval,
+        "largest_start_error_paying_5x": max(paying) if paying else None,
+    }
+
+
 def calibrate_copy(mib: int):
     """A plain device copy of known size (read mib MiB, write mib MiB), five times: the
     reference point that scales FETCH_SIZE / WRITE_SIZE counter readings to bytes."""
@@ -614,6 +679,15 @@ def main():
                     help="train the network on the non-Gaussian part of log g / N (--c5)")
     ap.add_argument("--c5-seconds", type=float, default=60.0,
                     help="device-time budget per held-out run (--c5)")
+    ap.add_argument("--c2-warm", action="store_true",
+                    help="only run the C2 warm-start requirement study (reference log g plus "
+                         "errors of known size)")
+    ap.add_argument("--c2-warm-errors", type=float, nargs="+",
+                    default=[0.005, 0.01, 0.03, 0.1, 0.3, 1.0],
+                    help="start errors (amplitudes) of the warm-start study (--c2-warm)")
+    ap.add_argument("--c2-warm-factors", type=float, nargs="+",
+                    default=[1e-3, 1e-4, 1e-5, 1e-6],
+                    help="initial ln f of the warm-start study (--c2-warm)")
     ap.add_argument("--c3", action="store_true",
                     help="only run the C3 batched study (64 uneven CSR systems in one launch)")
     ap.add_argument("--graph", type=int, default=16, help="epochs per HIP graph (--c3)")
@@ -625,6 +699,10 @@ def main():
     if args.c5:
         print(json.dumps(c5_leg(args.c5_curves, args.c5_final, args.c5_steps, args.c5_seconds,
                                 args.tta_seeds, gaussian=args.c5_gaussian)))
+        return
+    if args.c2_warm:
+        print(json.dumps(c2_warm_leg(args.c2_warm_errors, args.c2_warm_factors,
+                                     args.tta_seeds, 2.5, 256)))
         return
     if args.c3:
         print(json.dumps(c3_leg(args.steps if args.steps != 3000 else 200, 20, args.graph,
