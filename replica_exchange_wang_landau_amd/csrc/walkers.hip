@@ -1494,12 +1494,12 @@ template <int B> __device__ inline uint32_t row_bcast_u32(uint32_t v)
 {
     return uint32_t(__builtin_amdgcn_mov_dpp(int(v), 0x150 + B, 0xF, 0xF, true));
 }
+// one v_mov_b64_dpp: gfx950's 64-bit DPP move takes row_newbcast, so a double moves in one
+// instruction instead of two 32-bit moves
 template <int B> __device__ inline double row_bcast_f64(double v)
 {
-    const uint64_t bits = __double_as_longlong(v);
-    const uint32_t lo = row_bcast_u32<B>(uint32_t(bits));
-    const uint32_t hi = row_bcast_u32<B>(uint32_t(bits >> 32));
-    return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
+    const long long bits = __double_as_longlong(v);
+    return __longlong_as_double(__builtin_amdgcn_update_dpp(bits, bits, 0x150 + B, 0xF, 0xF, true));
 }
 
 constexpr int kStagedBatch = 16;
@@ -1579,12 +1579,12 @@ __device__ inline void stage_row(const double *row_any, uint32_t dst_any)
     }
 }
 
-// the median of x, lo and hi: x clamped to [lo, hi] for lo <= hi, in one instruction
+// the median of x, lo and hi: x clamped to [lo, hi] for lo <= hi. The compiler turns this
+// form of the median into one v_med3_i32 (and, unlike inline assembly, needs no wait state
+// after it)
 __device__ inline int32_t med3_i32(int32_t x, int32_t lo, int32_t hi)
 {
-    int32_t r;
-    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(lo), "v"(hi));
-    return r;
+    return max(min(x, lo), min(max(x, lo), hi));
 }
 
 __device__ inline void stage_wait()
