@@ -695,6 +695,10 @@ def main():
                     help="exchange cadence of the C3 batch (--c3)")
     ap.add_argument("--c4", action="store_true",
                     help="only run the C4 FP32-field precision study (resync kernel, drift)")
+    ap.add_argument("--line-c4-epochs", type=int, default=40,
+                    help="timed C4 epochs in the default line's compact c4 object (0: leave out)")
+    ap.add_argument("--line-c3-epochs", type=int, default=32,
+                    help="timed C3 epochs in the default line's compact c3 object (0: leave out)")
     args = ap.parse_args()
     if args.c5:
         print(json.dumps(c5_leg(args.c5_curves, args.c5_final, args.c5_steps, args.c5_seconds,
@@ -824,6 +828,24 @@ def main():
         "cpu_baseline": cpu,
         "time_to_accuracy": tta,
     }
+    # the other GPU-scale configurations, compact, so that the default line measures them too
+    if args.line_c4_epochs > 0:
+        c4 = c4_leg(args.line_c4_epochs, 5, 10)
+        line["c4"] = {
+            "workload": c4["workload"], "attempted_flips_per_s": c4["attempted_flips_per_s"],
+            "epochs_timed": c4["epochs_timed"],
+            "resync_ms": c4["resync"]["ms"], "resync_tflops": c4["resync"]["achieved_tflops"],
+            "resync_frac_fp64_mfma_peak": c4["resync"]["frac"],
+            "drift": c4["drift"]}
+        print("bench: c4 leg done", file=sys.stderr, flush=True)
+    if args.line_c3_epochs > 0:
+        graph = 16 if args.line_c3_epochs % 16 == 0 else 0
+        c3 = c3_leg(args.line_c3_epochs, 16, graph)
+        line["c3"] = {k: c3[k] for k in (
+            "workload", "exchange_cadence", "systems", "walkers", "attempted_flips_per_s",
+            "epochs_timed", "ms_per_epoch", "graph_epochs", "lds_budget", "acceptance")}
+        line["c3"]["row_bytes_frac_hbm_peak"] = c3["roofline"]["frac"]
+        print("bench: c3 leg done", file=sys.stderr, flush=True)
     print(json.dumps(line))
 
 
