@@ -323,6 +323,10 @@ def c4_leg(epochs: int, warmup: int, resyncs: int):
                     f"N={n}, grid [{emin},{emax}] bin 2.0 (10^3-MCS pilot), 32 windows x 8 "
                     f"walkers, FP32 Kahan fields, FP64 resync every 10 MCS",
         "attempted_flips_per_s": (after.attempted - before.attempted) / seconds,
+        "acceptance": (after.accepted - before.accepted) / max(1, after.attempted - before.attempted),
+        # algorithmic bytes: every accepted flip streams its FP32 row of J (N x 4 B); the 64 MiB
+        # FP32 J exceeds the 4 MiB L2 of an XCD, so the rows come from MALL or HBM
+        "row_bytes_gbs": (after.accepted - before.accepted) * n * 4.0 / seconds / 1e9,
         "epochs_timed": epochs,
         "resync": {"ms": resync_s * 1e3, "flop": flops, "achieved_tflops": flops / resync_s / 1e12,
                    "peak_tflops": 78.6, "frac": flops / resync_s / 1e12 / 78.6,
@@ -833,6 +837,8 @@ def main():
         c4 = c4_leg(args.line_c4_epochs, 5, 10)
         line["c4"] = {
             "workload": c4["workload"], "attempted_flips_per_s": c4["attempted_flips_per_s"],
+            "acceptance": c4["acceptance"], "row_bytes_gbs": c4["row_bytes_gbs"],
+            "row_bytes_frac_hbm_peak": c4["row_bytes_gbs"] / HBM_PEAK_GBS,
             "epochs_timed": c4["epochs_timed"],
             "resync_ms": c4["resync"]["ms"], "resync_tflops": c4["resync"]["achieved_tflops"],
             "resync_frac_fp64_mfma_peak": c4["resync"]["frac"],

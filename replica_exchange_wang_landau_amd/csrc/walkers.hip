@@ -2340,6 +2340,27 @@ __global__ __launch_bounds__(kWave) void lookahead_xo_kernel(const Args *table,
 // the result equals a one-flip-at-a-time FP32 Kahan walker on the same stream. The drift
 // against FP64 is removed by the periodic resync below.
 // ---------------------------------------------------------------------------------------------
+// FP32 walker layout: lane l holds in slot k the site f32_site(l, k). Each lane owns runs of
+// G = min(NPL, 4) consecutive sites, one run in every 64·G sites, so that a row's values and a
+// configuration's pairs load G per lane at a time (one dwordx4 per lane per 256 sites once
+// NPL >= 4). Only the register placement changes: HBM keeps fields, spins and rows in site order.
+template <int NPL> constexpr uint32_t f32_group() { return NPL < 4 ? uint32_t(NPL) : 4u; }
+template <int NPL> __device__ inline uint32_t f32_site(uint32_t lane, uint32_t k)
+{
+    constexpr uint32_t G = f32_group<NPL>();
+    return (k / G) * (kWave * G) + lane * G + (k % G);
+}
+template <int NPL> __device__ inline uint32_t f32_lane(uint32_t site)
+{
+    constexpr uint32_t G = f32_group<NPL>();
+    return (site % (kWave * G)) / G;
+}
+template <int NPL> __device__ inline uint32_t f32_slot(uint32_t site)
+{
+    constexpr uint32_t G = f32_group<NPL>();
+    return (site / (kWave * G)) * G + site % G;
+}
+
 template <int NPL> __device__ inline float select_slot_f32(const float (&h)[NPL], uint32_t k)
 {
     float level[NPL];
@@ -2357,9 +2378,10 @@ template <int NPL> __device__ inline float select_slot_f32(const float (&h)[NPL]
 template <int NPL> __device__ inline float gather_f32(const float (&h)[NPL], uint32_t site)
 {
     float hv[NPL];
+    const int src = int(f32_lane<NPL>(site));
 #pragma unroll
-    for (int k = 0; k < NPL; ++k) hv[k] = __shfl(h[k], int(site & (kWave - 1)));
-    return select_slot_f32<NPL>(hv, site >> 6);
+    for (int k = 0; k < NPL; ++k) hv[k] = __shfl(h[k], src);
+    return select_slot_f32<NPL>(hv, f32_slot<NPL>(site));
 }
 
 __device__ inline float readlane_f32(float v, uint32_t lane)
@@ -2376,6 +2398,37 @@ __device__ inline void kahan_add(float &h, float &c, float add)
     h = t;
 }
 
+// kahan_add(h, c, m * r) for m = +-2: the product is exact (a power-of-two scaling), so the fused
+// y = m * r - c rounds once, exactly as the separate product and difference do
+__device__ inline void kahan_add2(float &h, float &c, float m, float r)
+{
+    const float y = __builtin_fmaf(m, r, -c);
+    const float t = h + y;
+    c = (t - h) - y;
+    h = t;
+}
+
+// slots [k0, k0 + S) of this lane from a row (or configuration) stored in site order, G at a time
+template <int NPL, int S, class T>
+__device__ inline void load_slots_f32(const T *base, uint32_t lane, uint32_t k0, T (&r)[S])
+{
+    constexpr uint32_t G = f32_group<NPL>();
+    static_assert(S % G == 0, "whole runs");
+    if constexpr (G == 4 && sizeof(T) == 4) {
+#pragma unroll
+        for (int q = 0; q < S / 4; ++q) {
+            const float4 x = *reinterpret_cast<const float4 *>(base + f32_site<NPL>(lane, k0 + 4 * q));
+            r[4 * q] = x.x;
+            r[4 * q + 1] = x.y;
+            r[4 * q + 2] = x.z;
+            r[4 * q + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < S; ++k) r[k] = base[f32_site<NPL>(lane, k0 + uint32_t(k))];
+    }
+}
+
 template <int NPL>
 __device__ inline void load_config32(const Args &a, uint32_t slot, uint64_t &neg,
                                      float (&h)[NPL], float (&c)[NPL])
@@ -2385,10 +2438,11 @@ __device__ inline void load_config32(const Args &a, uint32_t slot, uint64_t &neg
     neg = 0;
 #pragma unroll
     for (int k = 0; k < NPL; ++k) {
-        const float2 p = a.fields32[base + k * kWave + lane];
+        const uint32_t site = f32_site<NPL>(lane, uint32_t(k));
+        const float2 p = a.fields32[base + site];
         h[k] = p.x;
         c[k] = p.y;
-        if (a.spins[base + k * kWave + lane] < 0) neg |= uint64_t(1) << k;
+        if (a.spins[base + site] < 0) neg |= uint64_t(1) << k;
     }
 }
 
@@ -2400,8 +2454,9 @@ __device__ inline void store_config32(const Args &a, uint32_t slot, uint64_t neg
     const size_t base = size_t(slot) * a.np;
 #pragma unroll
     for (int k = 0; k < NPL; ++k) {
-        a.fields32[base + k * kWave + lane] = make_float2(h[k], c[k]);
-        a.spins[base + k * kWave + lane] = ((neg >> k) & 1) ? int8_t(-1) : int8_t(1);
+        const uint32_t site = f32_site<NPL>(lane, uint32_t(k));
+        a.fields32[base + site] = make_float2(h[k], c[k]);
+        a.spins[base + site] = ((neg >> k) & 1) ? int8_t(-1) : int8_t(1);
     }
 }
 
@@ -2411,28 +2466,71 @@ __device__ inline void rows_kahan(const Args &a, uint32_t p_site, uint32_t m_hi,
                                   uint32_t count, float (&h)[NPL], float (&c)[NPL])
 {
     const uint32_t lane = lane_id();
-    constexpr int kRows = (64 / NPL) < 1 ? 1 : ((64 / NPL) > 8 ? 8 : (64 / NPL));
+    if constexpr (NPL >= 32) {
+        // The accepted rows stream in four parts of S slots. Each part's registers are refilled
+        // with the next accepted row's part as soon as they are added, so the next row's loads
+        // are in flight during this row's additions; after the last row they load that row
+        // again, which keeps the loads free of branches. The rows are unrolled up to the batch
+        // width. Every site still takes the rows in acceptance order.
+        constexpr int P = 4, S = NPL / P;
+        uint64_t acc = __ballot(lane >= first && lane < first + count && m_hi != 0u);
+        if (acc == 0) return;
+        auto pop = [&](const float *&row, float &m) {
+            const uint32_t L = uint32_t(__ffsll((unsigned long long)acc) - 1);
+            acc &= acc - 1;
+            row = a.J32 + size_t(__builtin_amdgcn_readlane(p_site, L)) * a.np;
+            m = float(hi_double(__builtin_amdgcn_readlane(m_hi, L)));
+        };
+        uint32_t rest = uint32_t(__popcll(acc)) - 1; // rows after the current one
+        const float *row;
+        float m;
+        pop(row, m);
+        float buf[P][S];
 #pragma unroll
-    for (int g = 0; g < KB; g += kRows) {
-        if (uint32_t(g) >= count) break;
-        float r[kRows][NPL];
-        float m[kRows];
+        for (int p = 0; p < P; ++p) load_slots_f32<NPL, S>(row, lane, uint32_t(p * S), buf[p]);
+        auto step = [&](const float *next) {
 #pragma unroll
-        for (int j = 0; j < kRows; ++j) {
-            const uint32_t want = first + uint32_t(g + j);
-            const uint32_t L = min(want, uint32_t(kWave - 1));
-            const uint32_t site = __builtin_amdgcn_readlane(p_site, L);
-            const uint32_t mh = __builtin_amdgcn_readlane(m_hi, L);
-            m[j] = float(hi_double(want < uint32_t(kWave) && uint32_t(g + j) < count ? mh : 0u));
-            const float *row = a.J32 + size_t(site) * a.np;
+            for (int p = 0; p < P; ++p) {
 #pragma unroll
-            for (int k = 0; k < NPL; ++k) r[j][k] = row[k * kWave + lane];
+                for (int s = 0; s < S; ++s) kahan_add2(h[p * S + s], c[p * S + s], m, buf[p][s]);
+                load_slots_f32<NPL, S>(next, lane, uint32_t(p * S), buf[p]);
+                // the scheduler would otherwise sink the loads below the remaining parts
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        };
+#pragma unroll
+        for (int r = 0; r < KB; ++r) { // straight-line code, so loads are waited for one by one
+            const float *next = row;
+            float mn = m;
+            if (rest) pop(next, mn);
+            step(next);
+            if (rest == 0) break;
+            --rest;
+            row = next;
+            m = mn;
         }
+    } else {
+        constexpr int kRows = (64 / NPL) > 8 ? 8 : (64 / NPL);
 #pragma unroll
-        for (int j = 0; j < kRows; ++j) {
-            if (m[j] == 0.0f) continue; // a zero increment still moves c into h: skip it
+        for (int g = 0; g < KB; g += kRows) {
+            if (uint32_t(g) >= count) break;
+            float r[kRows][NPL];
+            float m[kRows];
 #pragma unroll
-            for (int k = 0; k < NPL; ++k) kahan_add(h[k], c[k], m[j] * r[j][k]);
+            for (int j = 0; j < kRows; ++j) {
+                const uint32_t want = first + uint32_t(g + j);
+                const uint32_t L = min(want, uint32_t(kWave - 1));
+                const uint32_t site = __builtin_amdgcn_readlane(p_site, L);
+                const uint32_t mh = __builtin_amdgcn_readlane(m_hi, L);
+                m[j] = float(hi_double(want < uint32_t(kWave) && uint32_t(g + j) < count ? mh : 0u));
+                load_slots_f32<NPL, NPL>(a.J32 + size_t(site) * a.np, lane, 0u, r[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < kRows; ++j) {
+                if (m[j] == 0.0f) continue; // a zero increment still moves c into h: skip it
+#pragma unroll
+                for (int k = 0; k < NPL; ++k) kahan_add2(h[k], c[k], m[j], r[j][k]);
+            }
         }
     }
 }
@@ -2489,11 +2587,11 @@ __global__ __launch_bounds__(kWave) void lookahead_f32_kernel(Args a, uint64_t r
             const uint32_t rem32 = remaining < uint64_t(kWave) ? uint32_t(remaining) : kWave;
             const uint32_t K = min(min(uint32_t(KB), rem32), kWave - used);
             float hr = gather_f32<NPL>(h, p_site), cr = gather_f32<NPL>(c, p_site);
-            const uint32_t src = p_site & (kWave - 1);
+            const uint32_t src = f32_lane<NPL>(p_site);
             const uint64_t bits =
                 (uint64_t(uint32_t(__shfl(int(uint32_t(neg >> 32)), int(src)))) << 32) |
                 uint32_t(__shfl(int(uint32_t(neg)), int(src)));
-            uint32_t dsig_hi = ((bits >> (p_site >> 6)) & 1) ? 0x40000000u : 0xC0000000u;
+            uint32_t dsig_hi = ((bits >> f32_slot<NPL>(p_site)) & 1) ? 0x40000000u : 0xC0000000u;
             float jc[KB];
 #pragma unroll
             for (int q = 0; q < KB; ++q) {
@@ -2587,8 +2685,9 @@ __global__ __launch_bounds__(kWave) void lookahead_f32_kernel(Args a, uint64_t r
                 st.ambiguous += __popcll(__ballot(amb));
             }
             if (mine && m_hi != 0)
-                __hip_atomic_fetch_xor(&flipbox[p_site & (kWave - 1)], uint64_t(1) << (p_site >> 6),
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_xor(&flipbox[f32_lane<NPL>(p_site)],
+                                       uint64_t(1) << f32_slot<NPL>(p_site), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
             __syncthreads();
             neg ^= flipbox[lane];
             flipbox[lane] = 0;

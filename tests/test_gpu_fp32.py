@@ -82,12 +82,17 @@ def f32_config(**kw):
     return RunConfig(**base)
 
 
-@pytest.mark.parametrize("geometry", [(3, 3, 3, 1.0, (0, 0, 1), True),
-                                      (5, 7, 3, 1.0, (0.6, 0.0, 0.8), True)])
-def test_fp32_lookahead_equals_sequential_kahan_walker(gpu, orc, geometry):
+# register slots per lane (NPL) 1, 2, 4, 16 and 32: runs of 1, 2 and 4 sites per lane, and the
+# four-part row pipeline at NPL >= 32; bins wide enough for the window to fit in LDS
+@pytest.mark.parametrize("geometry,bin_width", [((3, 3, 3, 1.0, (0, 0, 1), True), 0.5),
+                                                ((5, 7, 3, 1.0, (0.6, 0.0, 0.8), True), 0.5),
+                                                ((6, 6, 6, 1.0, (0.0, 0.6, 0.8), True), 0.5),
+                                                ((10, 10, 10, 1.0, (0, 0, 1), True), 2.0),
+                                                ((12, 12, 12, 1.0, (0.48, 0.6, 0.64), True), 4.0)])
+def test_fp32_lookahead_equals_sequential_kahan_walker(gpu, orc, geometry, bin_width):
     pos, axes, box, per = simple_cubic(*geometry)
     n = len(pos)
-    cfg = f32_config(emin=-6.0 * n, emax=6.0 * n, bin_width=0.5)
+    cfg = f32_config(emin=-6.0 * n, emax=6.0 * n, bin_width=bin_width)
     system = System(pos, axes, box, per, backend=gpu)
     e = Engine(system, cfg)
     e.initialize()
