@@ -238,7 +238,8 @@ int wlx_engine_check(wlx_engine *engine);
 int wlx_engine_exchange(wlx_engine *engine, uint64_t epoch, uint64_t *attempted,
                         uint64_t *accepted);
 /* The run_rewl epoch loop (parallel.cpp:376-534) for at most max_epochs epochs; stops early
- * when every walker is frozen or the attempt limit is reached. Resumable. */
+ * when every walker is frozen or the attempt limit is reached. Resumable. Returns after the
+ * device has finished the epochs; `status` may be NULL, and is then not read back. */
 int wlx_engine_run(wlx_engine *engine, uint64_t max_epochs, wlx_run_status *status);
 int wlx_engine_status(wlx_engine *engine, wlx_run_status *status);
 /* Replays `epochs_per_graph` epochs of wlx_engine_run (the sampling and exchange launches of

@@ -347,9 +347,12 @@ class Engine:
                           ctypes.byref(b))
         return a.value, b.value
 
-    def run(self, max_epochs: int = 2**63) -> abi.RunStatus:
-        st = abi.RunStatus()
-        self.backend.call("engine_run", self.handle, ctypes.c_uint64(max_epochs), ctypes.byref(st))
+    def run(self, max_epochs: int = 2**63, status: bool = True) -> abi.RunStatus | None:
+        """Runs up to `max_epochs` exchange epochs; returns after the device has finished them.
+        With status=False the run's status is not read back (None is returned)."""
+        st = abi.RunStatus() if status else None
+        self.backend.call("engine_run", self.handle, ctypes.c_uint64(max_epochs),
+                          ctypes.byref(st) if status else None)
         return st
 
     def status(self) -> abi.RunStatus:
