@@ -2758,29 +2758,49 @@ __global__ __launch_bounds__(kWave) void resync_mfma_kernel(const double *J, con
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) acc[x][y] = mfma_d4{0.0, 0.0, 0.0, 0.0};
-    // kResyncSteps K steps per round: all their loads are issued before their MFMAs, so one
-    // wave per SIMD keeps that many loads in flight
+    // kResyncSteps K steps per round. Rounds are double-buffered: the loads of round r + 1 are
+    // issued before the MFMAs of round r, so one wave per SIMD keeps the matrix core busy while
+    // the next operands arrive. Every load is unconditional (a conditional one compiles to a
+    // branch, after which the loads in flight are all waited for): a step k >= n reads row
+    // n - 1 and multiplies its spins by 0, so it adds nothing.
     constexpr int kResyncSteps = 8;
-    for (uint32_t k0 = 0; k0 < n; k0 += 4 * kResyncSteps) {
-        double a0[kResyncSteps], a1[kResyncSteps], b0[kResyncSteps], b1[kResyncSteps];
+    double a0[2][kResyncSteps], a1[2][kResyncSteps], b0[2][kResyncSteps], b1[2][kResyncSteps];
+    auto load = [&](uint32_t k0, int buf) {
 #pragma unroll
         for (int u = 0; u < kResyncSteps; ++u) {
             const uint32_t k = k0 + 4 * uint32_t(u) + kk;
-            const bool real = k < n;
-            const double *jrow = J + size_t(real ? k : 0) * np + i0 + row;
-            const double *srow = S + size_t(real ? k : 0) * slots_padded + s0 + row;
-            a0[u] = real ? jrow[0] : 0.0;
-            a1[u] = real ? jrow[16] : 0.0;
-            b0[u] = real ? srow[0] : 0.0;
-            b1[u] = real ? srow[16] : 0.0;
+            const uint32_t kc = min(k, n - 1);
+            const double *jrow = J + size_t(kc) * np + i0 + row;
+            const double *srow = S + size_t(kc) * slots_padded + s0 + row;
+            a0[buf][u] = jrow[0];
+            a1[buf][u] = jrow[16];
+            b0[buf][u] = srow[0];
+            b1[buf][u] = srow[16];
         }
+    };
+    // the zeroing of steps beyond n happens here, after the scheduling barrier, so that it does
+    // not wait for the loads just issued
+    auto multiply = [&](uint32_t k0, int buf) {
 #pragma unroll
         for (int u = 0; u < kResyncSteps; ++u) {
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b0[u], acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b1[u], acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b0[u], acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b1[u], acc[1][1], 0, 0, 0);
+            const double keep = k0 + 4 * uint32_t(u) + kk < n ? 1.0 : 0.0;
+            const double c0 = b0[buf][u] * keep, c1 = b1[buf][u] * keep;
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[buf][u], c0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[buf][u], c1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[buf][u], c0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[buf][u], c1, acc[1][1], 0, 0, 0);
         }
+    };
+    constexpr uint32_t kRound = 4 * kResyncSteps;
+    load(0, 0);
+    for (uint32_t k0 = 0; k0 < n; k0 += 2 * kRound) {
+        load(k0 + kRound, 1);
+        __builtin_amdgcn_sched_barrier(0); // the loads stay ahead of the MFMAs
+        multiply(k0, 0);
+        if (k0 + kRound >= n) break;
+        load(k0 + 2 * kRound, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        multiply(k0 + kRound, 1);
     }
 #pragma unroll
     for (int x = 0; x < 2; ++x)
