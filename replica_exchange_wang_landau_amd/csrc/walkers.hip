@@ -2736,6 +2736,9 @@ __global__ void resync_spins_kernel(const int8_t *spins, uint32_t np, uint32_t s
 
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 
+// the resync's K steps are split in this many parts, each summed into its own partial of H
+constexpr uint32_t kResyncParts = 4;
+
 // H[s][i] = sum_k J[k][i] S[k][s] with v_mfma_f64_16x16x4_f64: one wave per 32 sites x 32
 // slots, four 16x16 accumulator blocks, K stepped by 4. J is bitwise symmetric, so the A
 // operand J[i][k] is read as J[k][i] (consecutive lanes, consecutive addresses). Operand
@@ -2749,8 +2752,9 @@ __global__ __launch_bounds__(kWave) void resync_mfma_kernel(const double *J, con
                                                            double *H)
 {
     const uint32_t lane = threadIdx.x;
-    const uint32_t tiles_i = np / 32;
-    const uint32_t ti = blockIdx.x % tiles_i, ts = blockIdx.x / tiles_i;
+    const uint32_t tiles_i = np / 32, tiles = tiles_i * (slots_padded / 32);
+    const uint32_t part = blockIdx.x / tiles, tile = blockIdx.x % tiles;
+    const uint32_t ti = tile % tiles_i, ts = tile / tiles_i;
     const uint32_t i0 = ti * 32, s0 = ts * 32;
     const uint32_t row = lane % 16, kk = lane / 16;
     mfma_d4 acc[2][2];
@@ -2759,11 +2763,16 @@ __global__ __launch_bounds__(kWave) void resync_mfma_kernel(const double *J, con
 #pragma unroll
         for (int y = 0; y < 2; ++y) acc[x][y] = mfma_d4{0.0, 0.0, 0.0, 0.0};
     // kResyncSteps K steps per round. Rounds are double-buffered: the loads of round r + 1 are
-    // issued before the MFMAs of round r, so one wave per SIMD keeps the matrix core busy while
-    // the next operands arrive. Every load is unconditional (a conditional one compiles to a
-    // branch, after which the loads in flight are all waited for): a step k >= n reads row
-    // n - 1 and multiplies its spins by 0, so it adds nothing.
+    // issued before the MFMAs of round r, so the matrix core keeps busy while the next operands
+    // arrive. Every load is unconditional (a conditional one compiles to a branch, after which
+    // the loads in flight are all waited for): a step outside [kb, ke) reads a valid row and
+    // multiplies its spins by 0, so it adds nothing. Block part p sums the K steps of its part
+    // into its own partial of H (kResyncParts partials, added in order by the finish kernel),
+    // which puts two waves on every SIMD at C4.
     constexpr int kResyncSteps = 8;
+    constexpr uint32_t kRound = 4 * kResyncSteps;
+    const uint32_t span = (n + kResyncParts * 2 * kRound - 1) / (kResyncParts * 2 * kRound) * 2 * kRound;
+    const uint32_t kb = min(n, part * span), ke = min(n, kb + span);
     double a0[2][kResyncSteps], a1[2][kResyncSteps], b0[2][kResyncSteps], b1[2][kResyncSteps];
     auto load = [&](uint32_t k0, int buf) {
 #pragma unroll
@@ -2783,7 +2792,7 @@ __global__ __launch_bounds__(kWave) void resync_mfma_kernel(const double *J, con
     auto multiply = [&](uint32_t k0, int buf) {
 #pragma unroll
         for (int u = 0; u < kResyncSteps; ++u) {
-            const double keep = k0 + 4 * uint32_t(u) + kk < n ? 1.0 : 0.0;
+            const double keep = k0 + 4 * uint32_t(u) + kk < ke ? 1.0 : 0.0;
             const double c0 = b0[buf][u] * keep, c1 = b1[buf][u] * keep;
             acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[buf][u], c0, acc[0][0], 0, 0, 0);
             acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[buf][u], c1, acc[0][1], 0, 0, 0);
@@ -2791,13 +2800,12 @@ __global__ __launch_bounds__(kWave) void resync_mfma_kernel(const double *J, con
             acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[buf][u], c1, acc[1][1], 0, 0, 0);
         }
     };
-    constexpr uint32_t kRound = 4 * kResyncSteps;
-    load(0, 0);
-    for (uint32_t k0 = 0; k0 < n; k0 += 2 * kRound) {
+    load(kb, 0);
+    for (uint32_t k0 = kb; k0 < ke; k0 += 2 * kRound) {
         load(k0 + kRound, 1);
         __builtin_amdgcn_sched_barrier(0); // the loads stay ahead of the MFMAs
         multiply(k0, 0);
-        if (k0 + kRound >= n) break;
+        if (k0 + kRound >= ke) break;
         load(k0 + 2 * kRound, 0);
         __builtin_amdgcn_sched_barrier(0);
         multiply(k0 + kRound, 1);
@@ -2809,7 +2817,7 @@ __global__ __launch_bounds__(kWave) void resync_mfma_kernel(const double *J, con
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const uint32_t i = i0 + 16 * x + 4 * r + kk, s = s0 + 16 * y + row;
-                if (s < slots) H[size_t(s) * np + i] = acc[x][y][r];
+                if (s < slots) H[(size_t(part) * slots + s) * np + i] = acc[x][y][r];
             }
 }
 
@@ -2821,8 +2829,11 @@ __global__ __launch_bounds__(kWave) void resync_finish_kernel(Args a, const doub
     const uint32_t s = blockIdx.x, lane = lane_id();
     const size_t base = size_t(s) * a.np;
     double field_err = 0.0, e = 0.0;
+    const size_t stride = size_t(a.walkers) * a.np; // between the partials of H
     for (uint32_t i = lane; i < a.n; i += kWave) {
-        const double hv = H[base + i];
+        double hv = H[base + i];
+#pragma unroll
+        for (uint32_t q = 1; q < kResyncParts; ++q) hv += H[q * stride + base + i];
         const float2 p = a.fields32[base + i];
         field_err = fmax(field_err, fabs((double(p.x) - double(p.y)) - hv));
         const double sigma = a.spins[base + i] < 0 ? -1.0 : 1.0;
@@ -3579,7 +3590,8 @@ struct wlx_engine {
         hipLaunchKernelGGL(resync_spins_kernel, dim3(256), dim3(256), 0, stream, d_spins, np,
                            walkers, slots_padded, d_S);
         REWL_HIP(hipGetLastError());
-        hipLaunchKernelGGL(resync_mfma_kernel, dim3((np / 32) * (slots_padded / 32)), dim3(kWave),
+        hipLaunchKernelGGL(resync_mfma_kernel,
+                           dim3(kResyncParts * (np / 32) * (slots_padded / 32)), dim3(kWave),
                            0, stream, sys->d_J64, d_S, uint32_t(sys->n), np, slots_padded,
                            walkers, d_H);
         REWL_HIP(hipGetLastError());
@@ -3940,7 +3952,7 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
         e->slots_padded = (W + 31) / 32 * 32;
         if (int st = device_alloc(&e->d_fields32, W * np)) return st;
         if (int st = device_alloc(&e->d_S, size_t(np) * e->slots_padded)) return st;
-        if (int st = device_alloc(&e->d_H, W * np)) return st;
+        if (int st = device_alloc(&e->d_H, kResyncParts * W * np)) return st;
         if (int st = device_alloc(&e->d_drift, 1)) return st;
         REWL_HIP(hipMemsetAsync(e->d_fields32, 0, W * np * sizeof(float2), e->stream));
         REWL_HIP(hipMemsetAsync(e->d_drift, 0, sizeof(Drift), e->stream));
