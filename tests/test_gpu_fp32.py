@@ -138,6 +138,32 @@ def test_resync_recomputes_fp64_fields_and_energies(gpu):
     assert count == 1 and 0.0 < f < 1e-4 and en < 1e-2  # energy drift in bins
 
 
+# N = 105: four K parts of 64 steps, the second partial and the last two empty; N = 1000: a
+# partial last part; 5 and 3 walkers leave padding slots in the 32-slot tiles
+@pytest.mark.parametrize("geometry,bin_width,walkers", [((5, 7, 3, 1.0, (0.6, 0.0, 0.8), True), 0.5, 5),
+                                                        ((10, 10, 10, 1.0, (0, 0, 1), True), 2.0, 3)])
+def test_resync_partial_and_empty_k_parts(gpu, geometry, bin_width, walkers):
+    """The resync's partial sums of H over K parts cover every step exactly once: after FP32
+    sampling, every configuration's fields equal the sequential local_fields within 1e-12
+    relative and its energy the FP64 total energy within 1e-12 relative."""
+    pos, axes, box, per = simple_cubic(*geometry)
+    n = len(pos)
+    cfg = f32_config(emin=-6.0 * n, emax=6.0 * n, bin_width=bin_width,
+                     walkers_per_window=walkers)
+    system = System(pos, axes, box, per, backend=gpu)
+    e = Engine(system, cfg)
+    e.initialize()
+    e.sample(3000)
+    e.resync()
+    spins = np.stack([e.read_walker(k).spins for k in range(e.walkers)])
+    fields, energies = system.fields(spins)
+    for k in range(e.walkers):
+        w = e.read_walker(k)
+        scale = np.maximum(1.0, np.abs(fields[k]))
+        assert np.max(np.abs(w.fields - fields[k]) / scale) <= 1e-12, k
+        assert abs(w.energy - energies[k]) <= 1e-12 * max(1.0, abs(energies[k])), k
+
+
 def test_fp32_drift_bounds_with_periodic_resync(gpu):
     """C2 geometry, FP32 fields resynced every 10 MCS: field drift stays below 1e-4 absolute
     (fields are O(10)) and energy drift below 1e-2 of a bin between resyncs."""
