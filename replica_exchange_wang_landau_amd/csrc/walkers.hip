@@ -2736,84 +2736,93 @@ __global__ void resync_spins_kernel(const int8_t *spins, uint32_t np, uint32_t s
 
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 
-// the resync's K steps are split in this many parts, each summed into its own partial of H
-constexpr uint32_t kResyncParts = 4;
+// the resync's K steps are split in this many parts, each summed into its own partial of H;
+// a wave computes 16·kResyncTileX sites x 16·kResyncTileY slots, kResyncSteps K steps a round
+constexpr uint32_t kResyncParts = 8;
+constexpr int kResyncTileX = 4, kResyncTileY = 4, kResyncSteps = 2;
 
-// H[s][i] = sum_k J[k][i] S[k][s] with v_mfma_f64_16x16x4_f64: one wave per 32 sites x 32
-// slots, four 16x16 accumulator blocks, K stepped by 4. J is bitwise symmetric, so the A
-// operand J[i][k] is read as J[k][i] (consecutive lanes, consecutive addresses). Operand
-// layout of the instruction on gfx950: lane l holds A[l % 16][l / 16], B[l / 16][l % 16] and
-// D[4 r + l / 16][l % 16] in accumulator register r (rows interleaved across the four lane
-// groups; checked with constant-row and constant-column operands). J has n rows of np
-// columns: rows k >= n (the padding of the last K step) enter as zero and are never read.
+// H[s][i] = sum_k J[k][i] S[k][s] with v_mfma_f64_16x16x4_f64: one wave per 16·kResyncTileX
+// sites x 16·kResyncTileY slots, kResyncTileX x kResyncTileY accumulator blocks of 16x16, K
+// stepped by 4. J is bitwise symmetric, so the A operand J[i][k] is read as J[k][i]
+// (consecutive lanes, consecutive addresses). Operand layout of the instruction on gfx950: lane
+// l holds A[l % 16][l / 16], B[l / 16][l % 16] and D[4 r + l / 16][l % 16] in accumulator
+// register r (rows interleaved across the four lane groups; checked with constant-row and
+// constant-column operands). J has n rows of np columns: rows k >= n (the padding of the last
+// K step) enter as zero and are never read.
 __global__ __launch_bounds__(kWave) void resync_mfma_kernel(const double *J, const double *S,
                                                            uint32_t n, uint32_t np,
                                                            uint32_t slots_padded, uint32_t slots,
                                                            double *H)
 {
+    constexpr int TX = kResyncTileX, TY = kResyncTileY;
     const uint32_t lane = threadIdx.x;
-    const uint32_t tiles_i = np / 32, tiles = tiles_i * (slots_padded / 32);
+    const uint32_t tiles_i = np / (16 * TX), tiles = tiles_i * (slots_padded / (16 * TY));
     const uint32_t part = blockIdx.x / tiles, tile = blockIdx.x % tiles;
     const uint32_t ti = tile % tiles_i, ts = tile / tiles_i;
-    const uint32_t i0 = ti * 32, s0 = ts * 32;
+    const uint32_t i0 = ti * 16 * TX, s0 = ts * 16 * TY;
     const uint32_t row = lane % 16, kk = lane / 16;
-    mfma_d4 acc[2][2];
+    mfma_d4 acc[TX][TY];
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+    for (int x = 0; x < TX; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y) acc[x][y] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+        for (int y = 0; y < TY; ++y) acc[x][y] = mfma_d4{0.0, 0.0, 0.0, 0.0};
     // kResyncSteps K steps per round. Rounds are double-buffered: the loads of round r + 1 are
     // issued before the MFMAs of round r, so the matrix core keeps busy while the next operands
     // arrive. Every load is unconditional (a conditional one compiles to a branch, after which
     // the loads in flight are all waited for): a step outside [kb, ke) reads a valid row and
     // multiplies its spins by 0, so it adds nothing. Block part p sums the K steps of its part
     // into its own partial of H (kResyncParts partials, added in order by the finish kernel),
-    // which puts two waves on every SIMD at C4.
-    constexpr int kResyncSteps = 8;
-    constexpr uint32_t kRound = 4 * kResyncSteps;
+    // which puts several waves on every SIMD at C4.
+    constexpr int kSteps = kResyncSteps;
+    constexpr uint32_t kRound = 4 * kSteps;
     const uint32_t span = (n + kResyncParts * 2 * kRound - 1) / (kResyncParts * 2 * kRound) * 2 * kRound;
     const uint32_t kb = min(n, part * span), ke = min(n, kb + span);
-    double a0[2][kResyncSteps], a1[2][kResyncSteps], b0[2][kResyncSteps], b1[2][kResyncSteps];
+    double av[2][TX][kSteps], bv[2][TY][kSteps];
     auto load = [&](uint32_t k0, int buf) {
 #pragma unroll
-        for (int u = 0; u < kResyncSteps; ++u) {
+        for (int u = 0; u < kSteps; ++u) {
             const uint32_t k = k0 + 4 * uint32_t(u) + kk;
             const uint32_t kc = min(k, n - 1);
             const double *jrow = J + size_t(kc) * np + i0 + row;
             const double *srow = S + size_t(kc) * slots_padded + s0 + row;
-            a0[buf][u] = jrow[0];
-            a1[buf][u] = jrow[16];
-            b0[buf][u] = srow[0];
-            b1[buf][u] = srow[16];
+#pragma unroll
+            for (int x = 0; x < TX; ++x) av[buf][x][u] = jrow[16 * x];
+#pragma unroll
+            for (int y = 0; y < TY; ++y) bv[buf][y][u] = srow[16 * y];
         }
     };
     // the zeroing of steps beyond n happens here, after the scheduling barrier, so that it does
     // not wait for the loads just issued
     auto multiply = [&](uint32_t k0, int buf) {
 #pragma unroll
-        for (int u = 0; u < kResyncSteps; ++u) {
+        for (int u = 0; u < kSteps; ++u) {
             const double keep = k0 + 4 * uint32_t(u) + kk < ke ? 1.0 : 0.0;
-            const double c0 = b0[buf][u] * keep, c1 = b1[buf][u] * keep;
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[buf][u], c0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[buf][u], c1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[buf][u], c0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[buf][u], c1, acc[1][1], 0, 0, 0);
+            double c[TY];
+#pragma unroll
+            for (int y = 0; y < TY; ++y) c[y] = bv[buf][y][u] * keep;
+#pragma unroll
+            for (int x = 0; x < TX; ++x)
+#pragma unroll
+                for (int y = 0; y < TY; ++y)
+                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[buf][x][u], c[y], acc[x][y],
+                                                                     0, 0, 0);
         }
     };
+    // whole pairs of rounds, the steps past ke zeroed by `keep`: no exit between the two halves,
+    // which would make the compiler keep a second copy of the accumulators
     load(kb, 0);
     for (uint32_t k0 = kb; k0 < ke; k0 += 2 * kRound) {
         load(k0 + kRound, 1);
         __builtin_amdgcn_sched_barrier(0); // the loads stay ahead of the MFMAs
         multiply(k0, 0);
-        if (k0 + kRound >= ke) break;
         load(k0 + 2 * kRound, 0);
         __builtin_amdgcn_sched_barrier(0);
         multiply(k0 + kRound, 1);
     }
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+    for (int x = 0; x < TX; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y)
+        for (int y = 0; y < TY; ++y)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const uint32_t i = i0 + 16 * x + 4 * r + kk, s = s0 + 16 * y + row;
@@ -3591,7 +3600,9 @@ struct wlx_engine {
                            walkers, slots_padded, d_S);
         REWL_HIP(hipGetLastError());
         hipLaunchKernelGGL(resync_mfma_kernel,
-                           dim3(kResyncParts * (np / 32) * (slots_padded / 32)), dim3(kWave),
+                           dim3(kResyncParts * (np / (16 * kResyncTileX)) *
+                                (slots_padded / (16 * kResyncTileY))),
+                           dim3(kWave),
                            0, stream, sys->d_J64, d_S, uint32_t(sys->n), np, slots_padded,
                            walkers, d_H);
         REWL_HIP(hipGetLastError());
@@ -3949,7 +3960,8 @@ int wlx_engine_create(wlx_system *sys, const wlx_config *c, wlx_engine **out)
     if (c->fields == WLX_FIELDS_FP32_KAHAN) {
         e->f32 = true;
         if (int st = system_j32(sys)) return st;
-        e->slots_padded = (W + 31) / 32 * 32;
+        // whole slot tiles of the resync
+        e->slots_padded = (W + 16 * kResyncTileY - 1) / (16 * kResyncTileY) * (16 * kResyncTileY);
         if (int st = device_alloc(&e->d_fields32, W * np)) return st;
         if (int st = device_alloc(&e->d_S, size_t(np) * e->slots_padded)) return st;
         if (int st = device_alloc(&e->d_H, kResyncParts * W * np)) return st;
