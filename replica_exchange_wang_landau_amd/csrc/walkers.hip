@@ -2537,7 +2537,7 @@ __device__ inline void rows_kahan(const Args &a, uint32_t p_site, uint32_t m_hi,
 
 // Parameters by value (the kernel-argument segment), not from a table like the other sampling
 // kernels: FP32-field engines never join a batch, and at 64 register slots per lane the table
-// copy costs registers (C4 ran 5 % slower with it).
+// copy costs registers (N = 4096 with 256 walkers ran 5 % slower with it).
 template <int NPL>
 __global__ __launch_bounds__(kWave) void lookahead_f32_kernel(Args a, uint64_t request,
                                                               uint32_t flags)
@@ -2747,8 +2747,8 @@ constexpr int kResyncTileX = 4, kResyncTileY = 4, kResyncSteps = 2;
 // (consecutive lanes, consecutive addresses). Operand layout of the instruction on gfx950: lane
 // l holds A[l % 16][l / 16], B[l / 16][l % 16] and D[4 r + l / 16][l % 16] in accumulator
 // register r (rows interleaved across the four lane groups; checked with constant-row and
-// constant-column operands). J has n rows of np columns: rows k >= n (the padding of the last
-// K step) enter as zero and are never read.
+// constant-column operands). J has n rows of np columns: a K step k >= n reads row n - 1 and
+// enters with zero spins.
 __global__ __launch_bounds__(kWave) void resync_mfma_kernel(const double *J, const double *S,
                                                            uint32_t n, uint32_t np,
                                                            uint32_t slots_padded, uint32_t slots,
@@ -2772,7 +2772,7 @@ __global__ __launch_bounds__(kWave) void resync_mfma_kernel(const double *J, con
     // the loads in flight are all waited for): a step outside [kb, ke) reads a valid row and
     // multiplies its spins by 0, so it adds nothing. Block part p sums the K steps of its part
     // into its own partial of H (kResyncParts partials, added in order by the finish kernel),
-    // which puts several waves on every SIMD at C4.
+    // which puts two waves on every SIMD at N = 4096 with 256 configurations.
     constexpr int kSteps = kResyncSteps;
     constexpr uint32_t kRound = 4 * kSteps;
     const uint32_t span = (n + kResyncParts * 2 * kRound - 1) / (kResyncParts * 2 * kRound) * 2 * kRound;
@@ -2791,8 +2791,8 @@ __global__ __launch_bounds__(kWave) void resync_mfma_kernel(const double *J, con
             for (int y = 0; y < TY; ++y) bv[buf][y][u] = srow[16 * y];
         }
     };
-    // the zeroing of steps beyond n happens here, after the scheduling barrier, so that it does
-    // not wait for the loads just issued
+    // the zeroing of steps outside [kb, ke) happens here, after the scheduling barrier, so that
+    // it does not wait for the loads just issued
     auto multiply = [&](uint32_t k0, int buf) {
 #pragma unroll
         for (int u = 0; u < kSteps; ++u) {
