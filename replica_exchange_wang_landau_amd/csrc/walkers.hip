@@ -2831,15 +2831,21 @@ __global__ __launch_bounds__(kWave) void resync_mfma_kernel(const double *J, con
 }
 
 // one wave per slot: drift of the running pairs and energy, then restart from H
-__global__ __launch_bounds__(kWave) void resync_finish_kernel(Args a, const double *H,
-                                                             Drift *drift)
+// kResyncFinishThreads threads per slot (four waves, so that the partial sums' loads of many
+// sites are in flight at once)
+constexpr uint32_t kResyncFinishThreads = 256;
+
+__global__ __launch_bounds__(kResyncFinishThreads) void resync_finish_kernel(Args a,
+                                                                            const double *H,
+                                                                            Drift *drift)
 {
-    __shared__ double part[kWave];
-    const uint32_t s = blockIdx.x, lane = lane_id();
+    __shared__ double part[kResyncFinishThreads];
+    __shared__ double err[kResyncFinishThreads / kWave];
+    const uint32_t s = blockIdx.x, t = threadIdx.x;
     const size_t base = size_t(s) * a.np;
     double field_err = 0.0, e = 0.0;
     const size_t stride = size_t(a.walkers) * a.np; // between the partials of H
-    for (uint32_t i = lane; i < a.n; i += kWave) {
+    for (uint32_t i = t; i < a.n; i += kResyncFinishThreads) {
         double hv = H[base + i];
 #pragma unroll
         for (uint32_t q = 1; q < kResyncParts; ++q) hv += H[q * stride + base + i];
@@ -2850,14 +2856,16 @@ __global__ __launch_bounds__(kWave) void resync_finish_kernel(Args a, const doub
         const float hf = float(hv);
         a.fields32[base + i] = make_float2(hf, float(double(hf) - hv));
     }
-    part[lane] = e;
+    part[t] = e;
+    for (int m = 1; m < kWave; m <<= 1) field_err = fmax(field_err, __shfl_xor(field_err, m));
+    if (lane_id() == 0) err[t / kWave] = field_err;
     __syncthreads();
-    for (int width = kWave / 2; width > 0; width >>= 1) {
-        if (lane < uint32_t(width)) part[lane] += part[lane + width];
+    for (uint32_t width = kResyncFinishThreads / 2; width > 0; width >>= 1) {
+        if (t < width) part[t] += part[t + width];
         __syncthreads();
     }
-    for (int m = 1; m < kWave; m <<= 1) field_err = fmax(field_err, __shfl_xor(field_err, m));
-    if (lane == 0) {
+    if (t == 0) {
+        for (uint32_t w = 1; w < kResyncFinishThreads / kWave; ++w) field_err = fmax(field_err, err[w]);
         const double e64 = part[0];
         const double rel = fabs(a.energy[s] - e64) / a.grid.width; // drift in bins
         atomicMax(&drift->max_field, (unsigned long long)__double_as_longlong(field_err));
@@ -3606,8 +3614,8 @@ struct wlx_engine {
                            0, stream, sys->d_J64, d_S, uint32_t(sys->n), np, slots_padded,
                            walkers, d_H);
         REWL_HIP(hipGetLastError());
-        hipLaunchKernelGGL(resync_finish_kernel, dim3(walkers), dim3(kWave), 0, stream, args(),
-                           d_H, d_drift);
+        hipLaunchKernelGGL(resync_finish_kernel, dim3(walkers), dim3(kResyncFinishThreads), 0,
+                           stream, args(), d_H, d_drift);
         REWL_HIP(hipGetLastError());
         ++resyncs;
         return WLX_OK;
